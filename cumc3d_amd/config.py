@@ -1,0 +1,123 @@
+"""Run configuration for the MI355X MHD update step.
+
+The reference fixes its configuration at compile time through Fortran
+PARAMETERs in solver/mhd/parameter.h:34-115 (geometry, dimension, boundary
+flags, box, grid size, CFL, Riemann-solver and reconstruction switches) plus a
+model parameter file (model/riemann/src/param.h, model/torus-cyn/src/param.h).
+Here the same settings are a runtime value, passed to the C-ABI through
+``cumc3d_desc`` (include/cumc3d_hip.h).  Field names follow the reference's
+PARAMETER names where one exists.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Tuple
+
+# Riemann solvers (solver/mhd/parameter.h:91-101 LF_flag/HLL_flag/HLLC_flag/HLLD_flag)
+RIEMANN = {"lf": 0, "hll": 1, "hllc": 2, "hlld": 3}
+# Reconstructions (parameter.h:103-115 tvdmm/tvdmc/tvdvl/ppmc/weno flags)
+RECON = {"tvdmm": 0, "tvdvl": 1, "tvdmc": 2, "ppm": 3, "weno": 4}
+# In-step model hooks (model/riemann and model/torus-cyn)
+MODELS = {"riemann": 0, "torus": 1}
+# Arithmetic of the spatial operator
+PRECISION = {"fp64": 0, "fp32": 1}
+
+
+@dataclasses.dataclass
+class Config:
+    nx: int
+    ny: int = 1
+    nz: int = 1
+    n_dim: int = 1
+    coordinate_flag: int = 0              # 0 Cartesian, 1 cylindrical (r, phi, z)
+    boundary_flag: Tuple[int, ...] = (1, 1, 1, 1, 1, 1)
+    x_start: float = 0.0
+    x_end: float = 1.0
+    y_start: float = 0.0
+    y_end: float = 1.0
+    z_start: float = 0.0
+    z_end: float = 1.0
+    cfl: float = 0.4
+    ggas: float = 5.0 / 3.0
+    total_time: float = 1.0e30
+    riemann: str = "hlld"
+    recon: str = "tvdmc"
+    model: str = "riemann"
+    # model/riemann/src/custom.f90:143-155: constant 0.1 downward gravity for
+    # the Rayleigh-Taylor tests (test_model 6 and 18).
+    rt_gravity: bool = False
+    # model/torus-cyn/src/param.h:19 Paczynski-Wiita radius (0 = Newtonian, GM=1)
+    r_sh: float = 0.0
+    precision: str = "fp64"
+
+    def __post_init__(self):
+        self.boundary_flag = tuple(int(b) for b in self.boundary_flag)
+        if len(self.boundary_flag) != 6:
+            raise ValueError("boundary_flag needs 6 entries")
+        if self.riemann not in RIEMANN:
+            raise ValueError(f"unknown Riemann solver {self.riemann!r}")
+        if self.recon not in RECON:
+            raise ValueError(f"unknown reconstruction {self.recon!r}")
+        if self.model not in MODELS:
+            raise ValueError(f"unknown model {self.model!r}")
+        if self.precision not in PRECISION:
+            raise ValueError(f"unknown precision {self.precision!r}")
+        if self.coordinate_flag not in (0, 1):
+            raise ValueError("only Cartesian (0) and cylindrical (1) geometry are built")
+        if self.n_dim == 1 and (self.ny != 1 or self.nz != 1):
+            raise ValueError("n_dim=1 needs ny=nz=1")
+        if self.n_dim == 2 and self.nz != 1:
+            raise ValueError("n_dim=2 needs nz=1")
+
+    # ghosted extents: -2..n+3 on each axis (solver/mhd/setup.f90:348-351)
+    @property
+    def shape(self):
+        return (self.nz + 6, self.ny + 6, self.nx + 6)
+
+    @property
+    def zones(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def with_(self, **kw) -> "Config":
+        return dataclasses.replace(self, **kw)
+
+
+# ---------------------------------------------------------------------------
+# Named workloads.  Sizes and physics follow the five measurement configurations
+# of this project; small variants of them are used as parity cases.
+# ---------------------------------------------------------------------------
+
+def briowu_1d(nx: int = 800) -> Config:
+    """Brio-Wu shock tube: 800 cells, gamma=2, HLLD + TVD-MC, CFL 0.4, t=0.1
+    (model/riemann/src/riemann1d.f90:229-250, test_model=10)."""
+    return Config(nx=nx, n_dim=1, cfl=0.4, ggas=2.0, total_time=0.1,
+                  riemann="hlld", recon="tvdmc", boundary_flag=(1, 1, 1, 1, 1, 1))
+
+
+def orszag_tang_3d(n: int = 128, recon: str = "ppm", riemann: str = "hlld") -> Config:
+    """3D periodic Orszag-Tang variant, gamma=5/3, HLLD + PPM."""
+    return Config(nx=n, ny=n, nz=n, n_dim=3, cfl=0.4, ggas=5.0 / 3.0,
+                  riemann=riemann, recon=recon, boundary_flag=(0, 0, 0, 0, 0, 0))
+
+
+def blast_3d(n: int = 256, recon: str = "weno", riemann: str = "hlld") -> Config:
+    """3D magnetised blast in a periodic unit box, HLLD + WENO-Z, t=0.02."""
+    return Config(nx=n, ny=n, nz=n, n_dim=3, cfl=0.4, ggas=5.0 / 3.0, total_time=0.02,
+                  riemann=riemann, recon=recon, boundary_flag=(0, 0, 0, 0, 0, 0))
+
+
+def turbulence_3d(n: int = 256, recon: str = "weno", riemann: str = "hlld") -> Config:
+    """3D periodic decaying MHD turbulence, t=0.5."""
+    return Config(nx=n, ny=n, nz=n, n_dim=3, cfl=0.4, ggas=5.0 / 3.0, total_time=0.5,
+                  riemann=riemann, recon=recon, boundary_flag=(0, 0, 0, 0, 0, 0))
+
+
+def torus_cyl(nr: int = 192, nphi: int = 128, nz: int = 192, recon: str = "ppm",
+              riemann: str = "hlld") -> Config:
+    """Cylindrical (r, phi, z) accretion torus in a point-mass potential (GM=1)."""
+    return Config(nx=nr, ny=nphi, nz=nz, n_dim=3, coordinate_flag=1,
+                  x_start=2.0, x_end=26.0, y_start=0.0, y_end=2.0 * math.pi,
+                  z_start=-12.0, z_end=12.0, cfl=0.4, ggas=5.0 / 3.0,
+                  riemann=riemann, recon=recon, model="torus", r_sh=0.0,
+                  boundary_flag=(1, 1, 0, 0, 1, 1))

@@ -1,0 +1,319 @@
+"""Initial conditions and the binary state-file format.
+
+A state is a dict of float64 arrays of shape (nz+6, ny+6, nx+6) over the
+ghosted box (-2..n+3 per axis, solver/mhd/setup.f90:348-351):
+
+    rho, vx, vy, vz, p      cell-centred primitives (prim(irho..itau))
+    bx, by, bz              face-centred field: bx[l,k,j] is Bx on face j+1/2
+                            (prim(ibx..ibz), solver/mhd/fromrvetou.f90:54-56)
+    eps                     specific internal energy (epsilon)
+    rho_floor, p_floor, eps_floor   (torus model only, model/torus-cyn/src/def.f90)
+
+The generators build the five measurement workloads; the Brio-Wu tube follows
+model/riemann/src/riemann1d.f90:229-250.  Random perturbations use a counter
+hash (splitmix64 of seed and interior cell index) so every consumer of an IC
+sees the same numbers.
+
+``write_ic`` emits the file read by the reference oracle's file-fed GET_MODEL
+(oracle/ref_harness/ic_plain.f90) and by the C-ABI upload path: a stream of
+int32[6] header, float64[2] (ggas, total_time), prim in Fortran order
+(8, nx+6, ny+6, nz+6), epsilon, and optionally the three floor arrays.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .config import Config
+from .grid import Grid
+
+PRIM_ORDER = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz")
+MAGIC = 0x4D433344  # 'MC3D'
+
+
+# ---------------------------------------------------------------------------
+# counter-based random numbers
+# ---------------------------------------------------------------------------
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    x = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = x + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed: int, index: np.ndarray) -> np.ndarray:
+    """U[0,1) from the hash of (seed, index)."""
+    key = (np.uint64(seed & 0xFFFFFFFF) << np.uint64(32)) ^ np.asarray(index, dtype=np.uint64)
+    return (splitmix64(key) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def interior_uniform(cfg: Config, seed: int) -> np.ndarray:
+    """One U[0,1) number per interior cell, broadcast on the ghosted box (ghosts get 0.5)."""
+    out = np.full(cfg.shape, 0.5)
+    l, k, j = np.meshgrid(np.arange(cfg.nz), np.arange(cfg.ny), np.arange(cfg.nx), indexing="ij")
+    idx = j + cfg.nx * (k + cfg.ny * l)
+    out[3:cfg.nz + 3, 3:cfg.ny + 3, 3:cfg.nx + 3] = uniform01(seed, idx.ravel()).reshape(idx.shape)
+    return out
+
+
+def empty_state(cfg: Config) -> dict:
+    return {name: np.zeros(cfg.shape) for name in PRIM_ORDER + ("eps",)}
+
+
+# ---------------------------------------------------------------------------
+# file format
+# ---------------------------------------------------------------------------
+
+def write_ic(path: str, cfg: Config, st: dict) -> None:
+    has_floor = int("rho_floor" in st)
+    prim = np.stack([st[n] for n in PRIM_ORDER], axis=-1)      # (nz+6, ny+6, nx+6, 8)
+    with open(path, "wb") as f:
+        np.array([MAGIC, 1, cfg.nx, cfg.ny, cfg.nz, has_floor], dtype=np.int32).tofile(f)
+        np.array([cfg.ggas, cfg.total_time], dtype=np.float64).tofile(f)
+        np.ascontiguousarray(prim, dtype=np.float64).tofile(f)
+        np.ascontiguousarray(st["eps"], dtype=np.float64).tofile(f)
+        if has_floor:
+            for n in ("rho_floor", "p_floor", "eps_floor"):
+                np.ascontiguousarray(st[n], dtype=np.float64).tofile(f)
+
+
+def read_ic(path: str):
+    with open(path, "rb") as f:
+        hdr = np.fromfile(f, dtype=np.int32, count=6)
+        if hdr[0] != MAGIC:
+            raise ValueError("not a cumc3d IC file")
+        nx, ny, nz, has_floor = (int(v) for v in hdr[2:6])
+        ggas, total_time = np.fromfile(f, dtype=np.float64, count=2)
+        shape = (nz + 6, ny + 6, nx + 6)
+        g = int(np.prod(shape))
+        prim = np.fromfile(f, dtype=np.float64, count=8 * g).reshape(shape + (8,))
+        st = {n: np.ascontiguousarray(prim[..., i]) for i, n in enumerate(PRIM_ORDER)}
+        st["eps"] = np.fromfile(f, dtype=np.float64, count=g).reshape(shape)
+        if has_floor:
+            for n in ("rho_floor", "p_floor", "eps_floor"):
+                st[n] = np.fromfile(f, dtype=np.float64, count=g).reshape(shape)
+    return (nx, ny, nz, float(ggas), float(total_time)), st
+
+
+# ---------------------------------------------------------------------------
+# generators
+# ---------------------------------------------------------------------------
+
+def brio_wu(cfg: Config) -> dict:
+    """Brio-Wu tube (riemann1d.f90:229-250): cells j=0..nx of every row get the
+    left or right state by x(j) <= 0.5; Bx = 0.75 on those faces."""
+    g = Grid(cfg)
+    st = empty_state(cfg)
+    xc = g.x.c                                     # x(-2..nx+3)
+    sl = slice(2, cfg.nx + 3)                      # j = 0..nx
+    left = xc[sl] <= 0.5
+    st["rho"][..., sl] = np.where(left, 1.0, 0.125)
+    st["p"][..., sl] = np.where(left, 1.0, 0.1)
+    st["by"][..., sl] = np.where(left, 1.0, -1.0)
+    st["bx"][..., sl] = 0.75
+    st["eps"][..., sl] = st["p"][..., sl] / st["rho"][..., sl] / (cfg.ggas - 1.0)
+    return st
+
+
+def _face_field_from_az(cfg: Config, g: Grid, az_fn):
+    """Bx, By on faces from a z-edge vector potential Az(xF, yF) (Cartesian):
+    Bx(j,k) = (Az(j,k) - Az(j,k-1))/dy(k),  By(j,k) = -(Az(j,k) - Az(j-1,k))/dx(j),
+    which has zero discrete divergence."""
+    xf = g.x.faces  # xF(-3..nx+3)
+    yf = g.y.faces
+    AZ = az_fn(xf[None, :], yf[:, None])       # AZ[k+3, j+3] = Az(xF(j), yF(k))
+    dy = g.y.d[:, None]                         # dy(-2..ny+3)
+    dx = g.x.d[None, :]
+    bx = (AZ[1:, 1:] - AZ[:-1, 1:]) / dy        # index [k+2, j+2] for k,j in -2..n+3
+    by = -(AZ[1:, 1:] - AZ[1:, :-1]) / dx
+    return bx, by
+
+
+def orszag_tang(cfg: Config, seed: int = 20261202) -> dict:
+    """3D Orszag-Tang variant: rho=25/(36 pi)(1 +- 1% noise), p=5/(12 pi),
+    v=(-sin 2pi y, sin 2pi x, 0.1 sin 2pi z), face B from
+    Az=(cos 4pi x/(4pi) + cos 2pi y/(2pi))/sqrt(4 pi)."""
+    g = Grid(cfg)
+    X, Y, Z = g.centres()
+    st = empty_state(cfg)
+    u = interior_uniform(cfg, seed)
+    st["rho"][:] = 25.0 / (36.0 * math.pi) * (1.0 + 0.02 * (u - 0.5))
+    st["p"][:] = 5.0 / (12.0 * math.pi)
+    st["vx"][:] = -np.sin(2 * math.pi * Y) + 0 * X
+    st["vy"][:] = np.sin(2 * math.pi * X) + 0 * Y
+    st["vz"][:] = 0.1 * np.sin(2 * math.pi * Z) + 0 * X
+    s4 = math.sqrt(4 * math.pi)
+    bx, by = _face_field_from_az(cfg, g, lambda x, y: (np.cos(4 * math.pi * x) / (4 * math.pi)
+                                                       + np.cos(2 * math.pi * y) / (2 * math.pi)) / s4)
+    st["bx"][:] = bx[None, :, :]
+    st["by"][:] = by[None, :, :]
+    st["eps"][:] = st["p"] / st["rho"] / (cfg.ggas - 1.0)
+    return st
+
+
+def blast(cfg: Config, seed: int = 20261202) -> dict:
+    """Magnetised blast: rho=1 (+-1% noise), p=0.1 (10 inside r<0.1 around the
+    box centre), v=0, uniform B=(10/sqrt2, 10/sqrt2, 0)/sqrt(4 pi)."""
+    g = Grid(cfg)
+    X, Y, Z = g.centres()
+    st = empty_state(cfg)
+    u = interior_uniform(cfg, seed)
+    st["rho"][:] = 1.0 + 0.02 * (u - 0.5)
+    cx = 0.5 * (cfg.x_start + cfg.x_end)
+    cy = 0.5 * (cfg.y_start + cfg.y_end)
+    cz = 0.5 * (cfg.z_start + cfg.z_end)
+    r2 = (X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2
+    st["p"][:] = np.where(r2 < 0.01, 10.0, 0.1)
+    b = 10.0 / math.sqrt(2.0) / math.sqrt(4 * math.pi)
+    st["bx"][:] = b
+    st["by"][:] = b
+    st["eps"][:] = st["p"] / st["rho"] / (cfg.ggas - 1.0)
+    return st
+
+
+def _solenoidal_modes(n: int, kmax: float, slope: float, seed: int, rng_stream: int):
+    """Random-phase solenoidal vector field on an n^3 periodic grid with
+    |k| <= kmax and amplitude |a_k| ~ k**slope; returns real arrays (3, n, n, n)
+    indexed [comp, l, k, j]."""
+    kk = np.fft.fftfreq(n, d=1.0 / n)
+    KZ, KY, KX = np.meshgrid(kk, kk, kk, indexing="ij")
+    K2 = KX * KX + KY * KY + KZ * KZ
+    kmag = np.sqrt(K2)
+    mask = (kmag > 0) & (kmag <= kmax)
+    idx = np.arange(n ** 3, dtype=np.uint64).reshape(n, n, n)
+    field = []
+    for c in range(3):
+        ph = uniform01(seed + 7919 * (3 * rng_stream + c), idx)
+        am = uniform01(seed + 104729 * (3 * rng_stream + c) + 1, idx)
+        g = np.sqrt(-2.0 * np.log(np.maximum(am, 1e-300)))     # Rayleigh amplitude
+        f = np.zeros((n, n, n), dtype=np.complex128)
+        f[mask] = g[mask] * np.exp(2j * math.pi * ph[mask]) * kmag[mask] ** slope
+        field.append(f)
+    fx, fy, fz = field
+    # project out the compressive part: f - k (k.f)/k^2
+    with np.errstate(invalid="ignore", divide="ignore"):
+        kdotf = np.where(mask, (KX * fx + KY * fy + KZ * fz) / np.where(K2 > 0, K2, 1), 0)
+    fx = fx - KX * kdotf
+    fy = fy - KY * kdotf
+    fz = fz - KZ * kdotf
+    return np.stack([np.fft.ifftn(f).real for f in (fx, fy, fz)])
+
+
+def _periodic_ghost(a: np.ndarray) -> np.ndarray:
+    """Embed an (nz, ny, nx) periodic interior into the ghosted (nz+6, ny+6, nx+6) box."""
+    return np.pad(a, 3, mode="wrap")
+
+
+def turbulence(cfg: Config, seed: int = 20261202) -> dict:
+    """Decaying turbulence: rho=1, beta=1, rms Mach 1 solenoidal velocity with
+    |k|<=8 and E(k) ~ k^-2, mean field B0 along z plus a solenoidal dB at 50%
+    of B0 built from an edge vector potential (discrete div B = 0)."""
+    n = cfg.nx
+    assert cfg.ny == n and cfg.nz == n, "turbulence needs a cubic grid"
+    g = Grid(cfg)
+    gam = cfg.ggas
+    p0 = 1.0 / gam                          # c_s = 1
+    b0 = math.sqrt(2.0 * p0)                # beta = 2 p / B0^2 = 1
+    v = _solenoidal_modes(n, 8.0, -2.0, seed, 0)     # amplitude ~ k^-2 gives E(k) ~ k^-2
+    vrms = math.sqrt(float(np.mean(np.sum(v * v, axis=0))))
+    v *= 1.0 / vrms                          # rms Mach 1 with c_s = 1
+    st = empty_state(cfg)
+    st["rho"][:] = 1.0
+    st["p"][:] = p0
+    st["vx"][:] = _periodic_ghost(v[0])
+    st["vy"][:] = _periodic_ghost(v[1])
+    st["vz"][:] = _periodic_ghost(v[2])
+    # vector potential on edges: A_x at (x_j, y_{k+1/2}, z_{l+1/2}), etc. (same
+    # periodic array indices), then face B = curl A.
+    a = _solenoidal_modes(n, 8.0, -3.0, seed, 1)
+    ax, ay, az = (np.asarray(c) for c in a)
+    h = 1.0 / n
+    # Bx(face j, k, l) = (Az(j,k) - Az(j,k-1))/dy - (Ay(j,l) - Ay(j,l-1))/dz
+    bx = (az - np.roll(az, 1, axis=1)) / h - (ay - np.roll(ay, 1, axis=0)) / h
+    by = (ax - np.roll(ax, 1, axis=0)) / h - (az - np.roll(az, 1, axis=2)) / h
+    bz = (ay - np.roll(ay, 1, axis=2)) / h - (ax - np.roll(ax, 1, axis=1)) / h
+    brms = math.sqrt(float(np.mean(bx * bx + by * by + bz * bz)))
+    s = 0.5 * b0 / brms
+    st["bx"][:] = _periodic_ghost(bx * s)
+    st["by"][:] = _periodic_ghost(by * s)
+    st["bz"][:] = _periodic_ghost(bz * s + b0)
+    del g
+    st["eps"][:] = st["p"] / st["rho"] / (gam - 1.0)
+    return st
+
+
+def torus(cfg: Config, seed: int = 20261202, r_in: float = 6.0, r_max: float = 10.0,
+          rho_floor: float = 1.0e-6, beta: float = 100.0, a_cut: float = 0.2,
+          p_noise: float = 0.01) -> dict:
+    """Constant-angular-momentum torus in the point-mass potential Phi=-1/R
+    (GM=1, r_sh=0) with inner edge r_in and pressure maximum at r_max,
+    polytropic p=K rho^gamma, rho_max=1.  Poloidal loops from
+    A_phi ~ max(rho - a_cut rho_max, 0) normalised to minimum beta, 1% pressure
+    noise.  Floors follow model/torus-cyn/src/model.f90:266-276 (interior
+    cells; the ghost cells of the floor arrays are zero)."""
+    assert cfg.coordinate_flag == 1
+    g = Grid(cfg)
+    gam = cfg.ggas
+    R = g.x.c[None, None, :]
+    Zc = g.z.c[:, None, None]
+    phi = lambda r, z: -1.0 / np.sqrt(r * r + z * z)
+    l2 = r_max                                # Keplerian specific angular momentum^2 at r_max
+    c_const = phi(r_in, 0.0) + l2 / (2.0 * r_in * r_in)
+    h_max = c_const - phi(r_max, 0.0) - l2 / (2.0 * r_max * r_max)
+    k_poly = h_max * (gam - 1.0) / gam        # rho_max = 1
+    h = c_const - phi(R, Zc) - l2 / (2.0 * R * R)
+    h = np.maximum(h, 0.0)
+    rho_t = (h * (gam - 1.0) / (gam * k_poly)) ** (1.0 / (gam - 1.0))
+    inside = (rho_t > rho_floor) & (R >= r_in)
+    shape = cfg.shape
+    st = empty_state(cfg)
+    st["rho"][:] = np.broadcast_to(np.where(inside, rho_t, rho_floor), shape)
+    st["vy"][:] = np.broadcast_to(np.where(inside, math.sqrt(l2) / R, 0.0), shape)
+    st["p"][:] = k_poly * st["rho"] ** gam
+    u = interior_uniform(cfg, seed)
+    st["p"][:] = np.where(np.broadcast_to(inside, shape),
+                          st["p"] * (1.0 + 2.0 * p_noise * (u - 0.5)), st["p"])
+    st["eps"][:] = st["p"] / st["rho"] / (gam - 1.0)
+    # A_phi at the (r, z) corners (xF(j), zF(l)); axisymmetric
+    xf = g.x.faces[None, :]
+    zf = g.z.faces[:, None]
+    hc = np.maximum(c_const - phi(xf, zf) - l2 / (2.0 * xf * xf), 0.0)
+    rc = (hc * (gam - 1.0) / (gam * k_poly)) ** (1.0 / (gam - 1.0))
+    A = np.maximum(rc - a_cut, 0.0)           # A[l+3, j+3]
+    dz = g.z.d[:, None]
+    xc = g.x.c[None, :]
+    dx = g.x.d[None, :]
+    # model/torus-cyn/src/model.f90:144-145 (curl of A_phi in cylindrical coordinates)
+    br = -(A[1:, 1:] - A[:-1, 1:]) / dz                                   # [l+2, j+2]
+    bz = (xf[:, 1:] * A[1:, 1:] - xf[:, :-1] * A[1:, :-1]) / (xc * dx)
+    # normalise to minimum beta over cells with field
+    brc = 0.5 * (br[:, 1:] + br[:, :-1])
+    bzc = 0.5 * (bz[1:, :] + bz[:-1, :])
+    b2 = brc[1:, :] ** 2 + bzc[:, 1:] ** 2
+    pin = k_poly * np.maximum(st["rho"][1:, 0, 1:], 0) ** gam
+    with np.errstate(divide="ignore"):
+        betas = np.where(b2 > 0, 2.0 * pin / np.where(b2 > 0, b2, 1.0), np.inf)
+    bscale = math.sqrt(float(np.min(betas)) / beta) if np.isfinite(np.min(betas)) else 0.0
+    st["bx"][:] = (br * bscale)[:, None, :]
+    st["bz"][:] = (bz * bscale)[:, None, :]
+    fl = np.zeros(shape)
+    fl[3:-3, 3:-3, 3:-3] = rho_floor
+    st["rho_floor"] = fl
+    st["p_floor"] = k_poly * fl ** gam
+    with np.errstate(divide="ignore", invalid="ignore"):
+        st["eps_floor"] = np.where(fl > 0, k_poly * fl ** (gam - 1.0) / (gam - 1.0), 0.0)
+    return st
+
+
+GENERATORS = {
+    "briowu": brio_wu,
+    "orszag_tang": orszag_tang,
+    "blast": blast,
+    "turbulence": turbulence,
+    "torus": torus,
+}

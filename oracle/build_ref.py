@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Build the reference oracle binaries from the reference's own Fortran sources.
+
+Recipe (test infrastructure only):
+  * the reference sources are compiled where they lie under /root/reference;
+    nothing is copied into this repository.  Each case gets a scratch
+    directory outside the repository holding symbolic links to the sources and
+    the case's generated ``parameter.h`` (the reference's compile-time
+    configuration file, solver/mhd/parameter.h), because a Fortran INCLUDE
+    searches the including file's directory first;
+  * getgrid.f90:133,142,151 use the PGI extension NOT(logical); the flag
+    -DNOT=.NOT. maps it to the standard operator without editing the file;
+  * main.f90 (HDF5 output) and dataio.f90 are replaced by the harness driver
+    oracle/ref_harness/driver.f90, and the model's GET_MODEL by a file-fed
+    initial model (oracle/ref_harness/ic_plain.f90 / ic_torus.f90);
+  * flang -O2 -ffp-contract=off -fopenmp, as planned for the CPU baseline.
+Outputs: oracle/_ref/<case>/cumc3d_ref.
+
+usage: python oracle/build_ref.py [case ...]     (default: every case)
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+from cases import CASES  # noqa: E402
+
+REF = os.environ.get("CUMC3D_REFERENCE", "/root/reference")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+SCRATCH = os.environ.get("CUMC3D_REF_SCRATCH", "/tmp/cumc3d_ref_build")
+OUT = os.path.join(HERE, "_ref")
+FFLAGS = ["-cpp", "-O2", "-ffp-contract=off", "-fopenmp", "-DDIVB", "-DFIXPOLE"]
+
+# solver/mhd/source.list minus main.f90, dataio.f90 (HDF5) and openacc.f90
+# (device data regions only), in module dependency order.
+SOLVER = ["definition.f90", "rsolver_module.f90", "mhd_module.f90", "weno_module.f90",
+          "ppmc_module.f90", "tvd_module.f90", "boundary.f90", "fromrvetou.f90",
+          "getgrid.f90", "initial.f90", "rungekutta.f90", "setup.f90", "spatial.f90",
+          "update.f90"]
+MODEL_SRC = {
+    "riemann": ("model/riemann/src", ["custom.f90", "findpressure.f90", "getepsilon.f90",
+                                      "soundspeed.f90"]),
+    "torus": ("model/torus-cyn/src", ["def.f90", "custom.f90", "findpressure.f90",
+                                      "getepsilon.f90", "soundspeed.f90"]),
+}
+
+
+def fort_real(v: float) -> str:
+    s = repr(float(v))
+    if "e" in s:
+        m, e = s.split("e")
+        return f"{m}D{int(e)}"
+    return s + "D0"
+
+
+def fort_bool(b: bool) -> str:
+    return ".true." if b else ".false."
+
+
+def parameter_h(cfg) -> str:
+    with open(os.path.join(HERE, "ref_harness", "parameter.h.in")) as f:
+        t = f.read()
+    rep = {
+        "@COORD@": str(cfg.coordinate_flag), "@NDIM@": str(cfg.n_dim),
+        "@BC@": ",".join(str(b) for b in cfg.boundary_flag),
+        "@X0@": fort_real(cfg.x_start), "@X1@": fort_real(cfg.x_end),
+        "@Y0@": fort_real(cfg.y_start), "@Y1@": fort_real(cfg.y_end),
+        "@Z0@": fort_real(cfg.z_start), "@Z1@": fort_real(cfg.z_end),
+        "@NX@": str(cfg.nx), "@NY@": str(cfg.ny), "@NZ@": str(cfg.nz),
+        "@CFL@": fort_real(cfg.cfl),
+        "@LF@": fort_bool(cfg.riemann == "lf"), "@HLL@": fort_bool(cfg.riemann == "hll"),
+        "@HLLC@": fort_bool(cfg.riemann == "hllc"), "@HLLD@": fort_bool(cfg.riemann == "hlld"),
+        "@TVDMM@": fort_bool(cfg.recon == "tvdmm"), "@TVDMC@": fort_bool(cfg.recon == "tvdmc"),
+        "@TVDVL@": fort_bool(cfg.recon == "tvdvl"), "@PPM@": fort_bool(cfg.recon == "ppm"),
+        "@WENO@": fort_bool(cfg.recon == "weno"),
+    }
+    for k, v in rep.items():
+        t = t.replace(k, v)
+    return t
+
+
+def torus_param_h(cfg) -> str:
+    # Names are those of model/torus-cyn/src/param.h:9-61; custom.f90 uses
+    # r_sh (CUSTOM_SOURCE) and refine_grid (CUSTOM_GRID).
+    return "\n".join([
+        "! torus-cyn model parameters for the oracle build",
+        "REAL*8, PARAMETER :: p_beta = 100.0D0",
+        "LOGICAL, PARAMETER :: normalize_by_vol = .false.",
+        "LOGICAL, PARAMETER :: normalize_by_minbeta = .true.",
+        f"REAL*8, PARAMETER :: r_sh = {fort_real(cfg.r_sh)}",
+        "REAL*8, PARAMETER :: gamma = (5.0d0/3.0d0)",
+        "REAL*8, PARAMETER :: q_grad = 2.0d0",
+        "REAL*8, PARAMETER :: s_in = 6.0d0",
+        "REAL*8, PARAMETER :: s_max = 10.0d0",
+        "LOGICAL, PARAMETER :: refine_grid = .false.",
+        "LOGICAL, PARAMETER :: corona = .false.",
+        "REAL*8, PARAMETER :: a_corn = 1.0d0",
+        "REAL*8, PARAMETER :: a_eta = 1.0d-4",
+        "REAL*8, PARAMETER :: rho_cut = 2.0d-1",
+        "REAL*8, PARAMETER :: rho_fac = 1.0d-6",
+        "REAL*8, PARAMETER :: rho_max = 1.0d0",
+        ""])
+
+
+def run(cmd, cwd):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+
+def build_case(name: str) -> str:
+    case = CASES[name]
+    cfg = case.cfg
+    work = os.path.join(SCRATCH, name)
+    if os.path.isdir(work):
+        shutil.rmtree(work)
+    os.makedirs(work)
+    with open(os.path.join(work, "parameter.h"), "w") as f:
+        f.write(parameter_h(cfg))
+    mdir, msrc = MODEL_SRC[cfg.model]
+    if cfg.model == "torus":
+        with open(os.path.join(work, "param.h"), "w") as f:
+            f.write(torus_param_h(cfg))
+    else:
+        # model/riemann/src/custom.f90 includes param.h for test_model; 0 selects
+        # no extra gravity source (custom.f90:143).
+        with open(os.path.join(work, "param.h"), "w") as f:
+            f.write("INTEGER, PARAMETER :: test_model = 0\n")
+    srcs = [(os.path.join(REF, "solver/mhd", s), s) for s in SOLVER]
+    srcs += [(os.path.join(REF, mdir, s), "model_" + s) for s in msrc]
+    harness = ["harness_io.f90", "ic_torus.f90" if cfg.model == "torus" else "ic_plain.f90",
+               "driver.f90"]
+    objs = []
+    for path, link in srcs:
+        if not os.path.isfile(path):
+            raise FileNotFoundError(path)
+        os.symlink(path, os.path.join(work, link))
+        extra = ["-DNOT=.NOT."] if link == "getgrid.f90" else []
+        obj = link.replace(".f90", ".o")
+        run([FLANG, *FFLAGS, *extra, "-c", link, "-o", obj], work)
+        objs.append(obj)
+    for h in harness:
+        obj = h.replace(".f90", ".o")
+        run([FLANG, *FFLAGS, "-c", os.path.join(HERE, "ref_harness", h), "-o", obj], work)
+        objs.append(obj)
+    outdir = os.path.join(OUT, name)
+    os.makedirs(outdir, exist_ok=True)
+    libdir = os.path.join(os.path.dirname(os.path.dirname(FLANG)), "lib")
+    run([FLANG, "-fopenmp", "-o", os.path.join(outdir, "cumc3d_ref"), *objs,
+         f"-Wl,-rpath,{libdir}"], work)
+    return outdir
+
+
+def available() -> bool:
+    return os.path.isdir(os.path.join(REF, "solver", "mhd")) and os.path.exists(FLANG)
+
+
+def main(argv):
+    names = argv or list(CASES)
+    if not available():
+        print("reference sources or flang not found; oracle/_ref not built")
+        return 0
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        futs = {ex.submit(build_case, n): n for n in names}
+        for fu in cf.as_completed(futs):
+            print("built", futs[fu], "->", fu.result())
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

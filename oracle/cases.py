@@ -1,0 +1,73 @@
+"""Parity cases shared by the oracle recipes and the tests (test infrastructure).
+
+Each case names a configuration, an initial-condition generator and a step
+count.  The reference oracle (oracle/_ref/<case>/cumc3d_ref, built by
+oracle/build_ref.py from the reference sources) and the C restatement
+(oracle/mhd_oracle.c) both run these cases; golden fixtures in tests/golden/
+hold the reference's outputs for them.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from cumc3d_amd.config import Config, briowu_1d  # noqa: E402
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    cfg: Config
+    ic: str
+    steps: int
+
+
+def _ot(n, recon, riemann, bc=(0, 0, 0, 0, 0, 0)):
+    return Config(nx=n, ny=n, nz=n, n_dim=3, cfl=0.4, ggas=5.0 / 3.0,
+                  riemann=riemann, recon=recon, boundary_flag=bc)
+
+
+CASES = {c.name: c for c in [
+    # config 1 exactly: Brio-Wu, 800 cells, HLLD + TVD-MC, gamma=2, CFL 0.4, t=0.1
+    Case("c1_briowu", briowu_1d(800), "briowu", 100000),
+    # 3D periodic Orszag-Tang variant (config 2 physics at 12^3), every reconstruction
+    Case("ot_hlld_tvdmm", _ot(12, "tvdmm", "hlld"), "orszag_tang", 12),
+    Case("ot_hlld_tvdvl", _ot(12, "tvdvl", "hlld"), "orszag_tang", 12),
+    Case("ot_hlld_tvdmc", _ot(12, "tvdmc", "hlld"), "orszag_tang", 12),
+    Case("ot_hlld_ppm", _ot(12, "ppm", "hlld"), "orszag_tang", 12),
+    Case("ot_hlld_weno", _ot(12, "weno", "hlld"), "orszag_tang", 12),
+    # every Riemann solver
+    Case("ot_lf_ppm", _ot(12, "ppm", "lf"), "orszag_tang", 12),
+    Case("ot_hll_ppm", _ot(12, "ppm", "hll"), "orszag_tang", 12),
+    Case("ot_hllc_ppm", _ot(12, "ppm", "hllc"), "orszag_tang", 12),
+    # outflow and reflecting boundaries in 3D
+    Case("ot_bc_mixed", _ot(12, "weno", "hlld", bc=(2, 2, 1, 1, 2, 1)), "orszag_tang", 10),
+    # magnetised blast (config 3 physics at 16^3)
+    Case("blast_weno", Config(nx=16, ny=16, nz=16, n_dim=3, cfl=0.4, ggas=5.0 / 3.0,
+                              riemann="hlld", recon="weno", boundary_flag=(0,) * 6),
+         "blast", 10),
+    # 2D (n_dim=2, nz=1) exercises the 2D face-EMF fill (mhd_module.f90:343-356)
+    Case("ot2d_hlld_ppm", Config(nx=24, ny=24, nz=1, n_dim=2, cfl=0.4, ggas=5.0 / 3.0,
+                                 riemann="hlld", recon="ppm", boundary_flag=(0, 0, 0, 0, 0, 0)),
+         "orszag_tang", 20),
+    # cylindrical torus with floors, gravity, no-inflow clamps (config 5 physics, small)
+    Case("torus_hlld_ppm", Config(nx=24, ny=8, nz=24, n_dim=3, coordinate_flag=1,
+                                  x_start=2.0, x_end=26.0, y_start=0.0, y_end=2.0 * math.pi,
+                                  z_start=-12.0, z_end=12.0, cfl=0.4, ggas=5.0 / 3.0,
+                                  riemann="hlld", recon="ppm", model="torus",
+                                  boundary_flag=(1, 1, 0, 0, 1, 1)),
+         "torus", 10),
+    # cylindrical axis (3) and equatorial (4) symmetry flags
+    Case("torus_sym_tvdmc", Config(nx=16, ny=8, nz=12, n_dim=3, coordinate_flag=1,
+                                   x_start=2.0, x_end=26.0, y_start=0.0, y_end=2.0 * math.pi,
+                                   z_start=0.0, z_end=12.0, cfl=0.4, ggas=5.0 / 3.0,
+                                   riemann="hllc", recon="tvdmc", model="torus",
+                                   boundary_flag=(3, 1, 0, 0, 4, 1)),
+         "torus", 8),
+]}

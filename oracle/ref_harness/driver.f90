@@ -1,0 +1,66 @@
+! Harness driver for the reference oracle (test infrastructure only).
+! Runs the reference's own step sequence exactly as solver/mhd/main.f90:339-422
+! does it (INITIAL_MODEL, INITIAL_UPDATE, then FINDDT / dt clip / RUNGEKUTTA),
+! without the HDF5 writer, and dumps the final state as raw binary.
+!
+! usage: cumc3d_ref IC_FILE OUT_FILE MAX_STEPS [THREADS]
+PROGRAM REF_HARNESS
+USE DEFINITION
+USE MHD_MODULE
+USE HARNESS_IO
+IMPLICIT NONE
+CHARACTER(len=1024) :: out_path, arg
+INTEGER :: max_steps, nthreads, u, cnt, c0, c1, crate
+REAL*8, ALLOCATABLE :: dts(:)
+REAL*8 :: loop_seconds
+INTEGER, EXTERNAL :: OMP_GET_NUM_PROCS
+
+CALL GET_COMMAND_ARGUMENT(1, ic_path)
+CALL GET_COMMAND_ARGUMENT(2, out_path)
+CALL GET_COMMAND_ARGUMENT(3, arg)
+READ(arg, *) max_steps
+nthreads = OMP_GET_NUM_PROCS()
+IF (COMMAND_ARGUMENT_COUNT() >= 4) THEN
+  CALL GET_COMMAND_ARGUMENT(4, arg)
+  READ(arg, *) nthreads
+END IF
+CALL OMP_SET_NUM_THREADS(nthreads)
+
+n_iter = 0
+n_step = 0
+global_time = 0.0D0
+CALL INITIAL_MODEL
+CALL INITIAL_UPDATE
+
+ALLOCATE(dts(MAX(max_steps,1)))
+cnt = 0
+CALL SYSTEM_CLOCK(c0, crate)
+DO WHILE (global_time < total_time .AND. n_step < max_steps)
+  CALL FINDDT
+  IF (global_time + dt > total_time) THEN
+    dt = total_time - global_time
+  END IF
+  CALL RUNGEKUTTA
+  n_step = n_step + 1
+  global_time = global_time + dt
+  cnt = cnt + 1
+  dts(cnt) = dt
+END DO
+CALL SYSTEM_CLOCK(c1)
+loop_seconds = DBLE(c1 - c0)/DBLE(crate)
+
+OPEN(UNIT=999, FILE=TRIM(out_path)//'.divb', STATUS='REPLACE')
+CALL FIND_DIVB
+CLOSE(999)
+
+OPEN(NEWUNIT=u, FILE=TRIM(out_path), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE', ACTION='WRITE')
+WRITE(u) nx, ny, nz, n_step, nthreads
+WRITE(u) global_time, maxDivB, loop_seconds
+WRITE(u) prim
+WRITE(u) epsilon
+WRITE(u) bcell
+WRITE(u) cs
+WRITE(u) cons
+WRITE(u) dts(1:cnt)
+CLOSE(u)
+END PROGRAM REF_HARNESS

@@ -1,0 +1,61 @@
+"""Readers and runners for the reference oracle binaries (test infrastructure).
+
+``run_ref`` executes oracle/_ref/<case>/cumc3d_ref (built by oracle/build_ref.py
+from the reference's Fortran sources) on an IC file and returns its final state.
+The dump layout is written by oracle/ref_harness/driver.f90.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def ref_binary(case: str) -> str:
+    return os.path.join(HERE, "_ref", case, "cumc3d_ref")
+
+
+def have_ref(case: str) -> bool:
+    return os.path.isfile(ref_binary(case))
+
+
+def read_dump(path: str) -> dict:
+    with open(path, "rb") as f:
+        nx, ny, nz, nstep, nthreads = (int(v) for v in np.fromfile(f, dtype=np.int32, count=5))
+        t, maxdivb, seconds = np.fromfile(f, dtype=np.float64, count=3)
+        shape = (nz + 6, ny + 6, nx + 6)
+        g = int(np.prod(shape))
+        prim = np.fromfile(f, dtype=np.float64, count=8 * g).reshape(shape + (8,))
+        eps = np.fromfile(f, dtype=np.float64, count=g).reshape(shape)
+        bcell = np.fromfile(f, dtype=np.float64, count=3 * g).reshape(shape + (3,))
+        cs = np.fromfile(f, dtype=np.float64, count=g).reshape(shape)
+        cons = np.fromfile(f, dtype=np.float64, count=8 * g).reshape(shape + (8,))
+        dts = np.fromfile(f, dtype=np.float64, count=nstep)
+    names = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz")
+    out = {n: np.ascontiguousarray(prim[..., i]) for i, n in enumerate(names)}
+    out.update(eps=eps, cs=cs, bcx=np.ascontiguousarray(bcell[..., 0]),
+               bcy=np.ascontiguousarray(bcell[..., 1]), bcz=np.ascontiguousarray(bcell[..., 2]),
+               cons=cons, dts=dts, nstep=nstep, t=float(t), maxdivb=float(maxdivb),
+               seconds=float(seconds), threads=nthreads)
+    return out
+
+
+def run_ref(case: str, ic_path: str, steps: int, threads: int = 1, timeout: float = 600) -> dict:
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "dump.bin")
+        env = dict(os.environ)
+        # HLLDNM reads usstar(ibn), which it never assigns
+        # (solver/mhd/rsolver_module.f90:704-741); the value comes from
+        # whatever the OpenMP-private copy's malloc returned.  glibc's
+        # MALLOC_PERTURB_=255 zero-fills every allocation, which pins that
+        # value to 0, the value fresh memory holds.
+        env["MALLOC_PERTURB_"] = "255"
+        r = subprocess.run([ref_binary(case), ic_path, out, str(steps), str(threads)],
+                           capture_output=True, text=True, timeout=timeout, env=env)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference oracle failed: {r.stderr[-2000:]}")
+        return read_dump(out)
