@@ -1,0 +1,722 @@
+// Host side of libcumc3d_hip.so: context, device state, launch sequencing
+// and the C-ABI declared in include/cumc3d_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cumc3d_hip.h"
+#include "kernels.hpp"
+
+using namespace cumc3d;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                         \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess)                                                             \
+            return fail(CUMC3D_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+struct cumc3d_ctx {
+    cumc3d_desc d;
+    int nx, ny, nz;
+    long NX, NY, NZ, G;
+    hipStream_t stream = nullptr;
+    double *pool = nullptr;       // all per-cell fields
+    double *met = nullptr;        // grid metrics and weights
+    DevScalars *S = nullptr;
+    unsigned long long *blockmax = nullptr;
+    long nblockmax = 0;
+    double *P[5], *eps, *cs, *Bc[3], *B[2][3], *Uo[8], *F[3][7], *E[3], *floors[3];
+    int cur = 0;                  // B[cur] holds the current face field
+    int bfac[6][12];
+    Axis ax[3];
+    bool uploaded = false, initialized = false, dt_pending = false;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;   // 7 marks per stage, TSTAGES stages
+    int tstage = 0;               // stages recorded in the current run
+    double tsum[7] = {0};
+    long tcnt[7] = {0};
+};
+
+// ---------------------------------------------------------------------------
+// layout conversion kernels (upload/download only)
+// ---------------------------------------------------------------------------
+__global__ void k_split_prim(const double *aos, double *rho, double *vx, double *vy, double *vz, double *p,
+                             double *bx0, double *by0, double *bz0, double *bx1, double *by1, double *bz1, long G) {
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= G) return;
+    const double *q = aos + 8 * c;
+    rho[c] = q[0]; vx[c] = q[1]; vy[c] = q[2]; vz[c] = q[3]; p[c] = q[4];
+    bx0[c] = q[5]; by0[c] = q[6]; bz0[c] = q[7];
+    bx1[c] = q[5]; by1[c] = q[6]; bz1[c] = q[7];
+}
+
+__global__ void k_join(double *aos, int nv, const double *f0, const double *f1, const double *f2, const double *f3,
+                       const double *f4, const double *f5, const double *f6, const double *f7, long G) {
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= G) return;
+    const double *f[8] = {f0, f1, f2, f3, f4, f5, f6, f7};
+    for (int i = 0; i < nv; ++i) aos[(long)nv * c + i] = f[i][c];
+}
+
+// INITIAL_UPDATE interior part: CUSTOM_CHECKRHO, bcell of FROMRVETOU and
+// FINDPRESSURE on the interior (initial.f90:91-108).
+__global__ void k_init_interior(Args a) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)a.nx * a.ny * a.nz) return;
+    const int j = 1 + (int)(t % a.nx), k = 1 + (int)((t / a.nx) % a.ny), l = 1 + (int)(t / ((long)a.nx * a.ny));
+    const long c = cidx(a, j, k, l);
+    double rho = a.P[0][c], eps = a.eps[c];
+    double v[3] = {a.P[1][c], a.P[2][c], a.P[3][c]};
+    checkrho(a, c, rho, v, eps);
+    a.P[0][c] = rho; a.P[1][c] = v[0]; a.P[2][c] = v[1]; a.P[3][c] = v[2];
+    a.eps[c] = eps;
+    a.Bc[0][c] = bavg<0>(a, j, a.Bin[0][c], a.Bin[0][c - 1]);
+    a.Bc[1][c] = bavg<1>(a, j, a.Bin[1][c], a.Bin[1][c - a.NX]);
+    a.Bc[2][c] = bavg<2>(a, j, a.Bin[2][c], a.Bin[2][c - a.NX * a.NY]);
+    double p, cs;
+    pressure(a, c, rho, eps, p, cs);
+    a.P[4][c] = p;
+    a.cs[c] = cs;
+}
+
+// standalone FINDDT (rungekutta.f90:256-323)
+__global__ __launch_bounds__(256) void k_finddt(Args a) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double dtloc = 1.0e5;
+    if (t < (long)a.nx * a.ny * a.nz) {
+        const int j = 1 + (int)(t % a.nx), k = 1 + (int)((t / a.nx) % a.ny), l = 1 + (int)(t / ((long)a.nx * a.ny));
+        const long c = cidx(a, j, k, l);
+        double v[3] = {a.P[1][c], a.P[2][c], a.P[3][c]};
+        double b[3] = {a.Bc[0][c], a.Bc[1][c], a.Bc[2][c]};
+        dtloc = cfl_dt(a, j, k, l, a.cs[c], a.P[0][c], v, b);
+    }
+    dtloc = block_min_f64(dtloc, a.S);
+    if (threadIdx.x == 0) atomicMin(&a.S->dtmin_bits, (unsigned long long)__double_as_longlong(dtloc));
+}
+
+// dt = MIN(1e5, cell minimum), clipped to total_time (main.f90:384-389); a
+// step that starts at or past total_time gets dt = 0 and is a no-op in time.
+__global__ void k_dt_finalize(DevScalars *S) {
+    S->active = S->t < S->total_time;
+    if (!S->active) {
+        // keep the CFL minimum of the current state for a later continuation
+        S->dt = 0.0;
+        return;
+    }
+    double m = __longlong_as_double((long long)S->dtmin_bits);
+    double dt = fmin2(1.0e5, m);
+    if (S->t + dt > S->total_time) dt = S->total_time - S->t;
+    S->dt = dt;
+    S->dtmin_bits = (unsigned long long)__double_as_longlong(1.0e300);
+}
+
+__global__ void k_time_advance(DevScalars *S, double *dtlog, int slot) {
+    if (S->t < S->total_time) {
+        S->t = S->t + S->dt;
+        S->nstep += 1;
+    }
+    if (dtlog) dtlog[slot] = S->dt;
+}
+
+__global__ void k_set_dt(DevScalars *S, double dt) {
+    S->dt = dt;
+    S->active = 1;
+}
+
+// FIND_DIVB (mhd_module.f90:133-189) as a max reduction on bit patterns
+__global__ __launch_bounds__(256) void k_divb(Args a, unsigned long long *out) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0.0;
+    if (t < (long)a.nx * a.ny * a.nz) {
+        const int j = 1 + (int)(t % a.nx), k = 1 + (int)((t / a.nx) % a.ny), l = 1 + (int)(t / ((long)a.nx * a.ny));
+        const long c = cidx(a, j, k, l), sy = a.NX, sz = a.NX * a.NY;
+        const double *bx = a.Bin[0], *by = a.Bin[1], *bz = a.Bin[2];
+        double dx = a.ax[0].d[j + 3], dy = a.ax[1].d[k + 3], dz = a.ax[2].d[l + 3];
+        if (a.coord == 0)
+            d = (bx[c] - bx[c - 1]) / (dx) + (by[c] - by[c - sy]) / (dy) + (bz[c] - bz[c - sz]) / (dz);
+        else
+            d = (a.ax[0].xF[j + 3] * bx[c] - a.ax[0].xF[j + 2] * bx[c - 1]) / (a.ax[0].c[j + 3] * dx) +
+                (by[c] - by[c - sy]) / (a.ax[0].c[j + 3] * dy) + (bz[c] - bz[c - sz]) / (dz);
+        d = fabsT(d);
+    }
+    // max of non-negative doubles == max of their bit patterns
+    unsigned long long bits = (unsigned long long)__double_as_longlong(d);
+    bits = block_max_u64(bits);
+    if (threadIdx.x == 0) atomicMax(out, bits);
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+static Args make_args(cumc3d_ctx *x) {
+    Args a;
+    memset(&a, 0, sizeof(a));
+    a.nx = x->nx; a.ny = x->ny; a.nz = x->nz;
+    a.ndim = x->d.n_dim;
+    a.coord = x->d.coordinate_flag;
+    a.model = x->d.model;
+    a.rt_gravity = x->d.rt_gravity;
+    a.bn_mode = x->d.hlld_bn_mode;
+    for (int i = 0; i < 6; ++i) a.bc[i] = x->d.boundary_flag[i];
+    memcpy(a.bfac, x->bfac, sizeof(a.bfac));
+    a.NX = x->NX; a.NY = x->NY; a.G = x->G;
+    a.g1 = x->d.ggas - 1.0;
+    a.cfl = x->d.cfl;
+    a.r_sh = x->d.r_sh;
+    for (int d = 0; d < 3; ++d) a.ax[d] = x->ax[d];
+    for (int i = 0; i < 5; ++i) a.P[i] = x->P[i];
+    a.eps = x->eps;
+    a.cs = x->cs;
+    for (int d = 0; d < 3; ++d) {
+        a.Bc[d] = x->Bc[d];
+        a.Bin[d] = x->B[x->cur][d];
+        a.Bout[d] = x->B[1 - x->cur][d];
+        a.E[d] = x->E[d];
+        for (int i = 0; i < 7; ++i) a.F[d][i] = x->F[d][i];
+    }
+    for (int i = 0; i < 8; ++i) a.Uo[i] = x->Uo[i];
+    a.rfl = x->floors[0];
+    a.pfl = x->floors[1];
+    a.efl = x->floors[2];
+    a.S = x->S;
+    a.blockmax = x->blockmax;
+    return a;
+}
+
+static inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// SETUP_EQNS parity factors (setup.f90:102-326); returns false on the
+// combinations the reference rejects with STOP.
+static bool setup_bfac(const int *bf, int bfac[6][12]) {
+    for (int s = 0; s < 6; ++s)
+        for (int i = 0; i < 12; ++i) bfac[s][i] = 1;
+    enum { IVX = 2, IVY = 3, IVZ = 4, IBX = 6, IBY = 7, IBZ = 8, IEX = 9, IEY = 10, IEZ = 11 };
+    for (int side = 0; side < 2; ++side) {
+        int *b = bfac[side], f = bf[side];
+        if (f == 2) { b[IVX] = -1; b[IBX] = -1; }
+        else if (f == 3) { b[IVX] = -1; b[IBX] = -1; b[IVZ] = -1; b[IBZ] = -1; }
+        else if (f == 4) return false;
+    }
+    for (int side = 0; side < 2; ++side) {
+        int *b = bfac[2 + side], f = bf[2 + side];
+        if (f == 2) { b[IVY] = -1; b[IBY] = -1; }
+        else if (f == 3) { b[IVY] = -1; b[IBY] = -1; b[IVZ] = -1; b[IBZ] = -1; }
+        else if (f == 4) { b[IVY] = -1; b[IBX] = -1; b[IBZ] = -1; }
+    }
+    for (int side = 0; side < 2; ++side) {
+        int *b = bfac[4 + side], f = bf[4 + side];
+        if (f == 2) { b[IVZ] = -1; b[IBZ] = -1; }
+        else if (f == 3) return false;
+        else if (f == 4) { b[IVZ] = -1; b[IBX] = -1; b[IBY] = -1; }
+    }
+    for (int s = 0; s < 6; ++s) {
+        int *b = bfac[s];
+        if (b[IVZ] * b[IBY] * b[IVY] * b[IBZ] < 0) return false;
+        b[IEX] = b[IVZ] * b[IBY];
+        if (b[IVX] * b[IBZ] * b[IVZ] * b[IBX] < 0) return false;
+        b[IEY] = b[IVX] * b[IBZ];
+        if (b[IVY] * b[IBX] * b[IVX] * b[IBY] < 0) return false;
+        b[IEZ] = b[IVY] * b[IBX];
+    }
+    return true;
+}
+
+// Weights of TVD_WEIGHT / PPM_INTERPOLANT / WENO_INTERPOLANT
+// (tvd_module.f90:42-53, ppmc_module.f90:91-145, weno_module.f90:117-149)
+static void weights(int recon, const std::vector<double> &dd, int n, std::vector<double> &w) {
+    w.assign((size_t)(n + 7) * NW, 0.0);
+    auto D = [&](int j) { return dd[j + 3]; };
+    if (recon <= 2) {
+        for (int j = -1; j <= n + 2; ++j) {
+            w[(j + 3) * NW + 0] = (D(j) + D(j + 1)) / D(j);
+            w[(j + 3) * NW + 1] = (D(j) + D(j - 1)) / D(j);
+        }
+    } else if (recon == 3) {
+        for (int j = 0; j <= n + 1; ++j) {
+            double dm2 = D(j - 2), dm1 = D(j - 1), dc = D(j), dp1 = D(j + 1), dp2 = D(j + 2);
+            double a1R = dc / (dc + dp1), a1L = dm1 / (dm1 + dc);
+            double aR = 2.0 * dp1 * dc / (dp1 + dc), aL = 2.0 * dc * dm1 / (dc + dm1);
+            double deltaXR = dm1 + dc + dp1 + dp2, deltaXL = dm2 + dm1 + dc + dp1;
+            double z1R = (dm1 + dc) / (2.0 * dc + dp1), z1L = (dm2 + dm1) / (2.0 * dm1 + dc);
+            double zR = (dp2 + dp1) / (2.0 * dp1 + dc), zL = (dp1 + dc) / (2.0 * dc + dm1);
+            double cc1 = dc / (dm1 + dc + dp1), cp1 = dp1 / (dc + dp1 + dp2), cm1 = dm1 / (dm2 + dm1 + dc);
+            double cc2 = (2.0 * dm1 + dc) / (dp1 + dc), cp2 = (2.0 * dc + dp1) / (dp2 + dp1);
+            double cm2 = (2.0 * dm2 + dm1) / (dc + dm1);
+            double cc3 = (dc + 2.0 * dp1) / (dm1 + dc), cp3 = (dp1 + 2.0 * dp2) / (dc + dp1);
+            double cm3 = (dm1 + 2.0 * dc) / (dm2 + dm1);
+            double *o = &w[(j + 3) * NW];
+            o[0] = a1L; o[1] = aL * (z1L - zL) / deltaXL; o[2] = (-dm1 * z1L) / deltaXL; o[3] = dc * zL / deltaXL;
+            o[4] = a1R; o[5] = aR * (z1R - zR) / deltaXR; o[6] = (-dc * z1R) / deltaXR; o[7] = dp1 * zR / deltaXR;
+            o[8] = cm1 * cm2; o[9] = cm1 * cm3; o[10] = cc1 * cc2; o[11] = cc1 * cc3; o[12] = cp1 * cp2;
+            o[13] = cp1 * cp3;
+        }
+    } else {
+        for (int j = 0; j <= n + 1; ++j) {
+            double dm2 = D(j - 2), dm1 = D(j - 1), dc = D(j), dp1 = D(j + 1), dp2 = D(j + 2);
+            double a12 = dp2 / (dp1 + dp2), a01 = dp1 / (dc + dp1), a10 = dc / (dm1 + dc), a21 = dm1 / (dm2 + dm1);
+            double *o = &w[(j + 3) * NW];
+            o[0] = 1.0 - a01; o[1] = a01; o[2] = 1.0 - a12; o[3] = a12; o[4] = 1.0 - a10;
+            o[5] = a10; o[6] = 1.0 - a21; o[7] = a21; o[8] = 1.0 - a21; o[9] = a12;
+        }
+    }
+}
+
+static int nseg_for(int nx, int na, int nd) {
+    long cols = (long)nx * na;
+    long want = (1L << 19) / (cols > 0 ? cols : 1);   // about 8k waves in flight
+    if (want < 1) want = 1;
+    if (want > nd + 1) want = nd + 1;
+    return (int)want;
+}
+
+// ---------------------------------------------------------------------------
+// flux dispatch over (reconstruction, Riemann solver)
+// ---------------------------------------------------------------------------
+template <int R, int S>
+static void launch_sweep_rs(cumc3d_ctx *x, const Args &a, int D, hipStream_t st) {
+    if (D == 0) {
+        long waves = ((long)(x->nx + 2) * x->ny * x->nz + 62) / 63 + 1;
+        unsigned grid = (unsigned)((waves * 64 + 255) / 256);
+        hipLaunchKernelGGL((k_flux_x<double, R, S>), dim3(grid), dim3(256), 0, st, a);
+        if (S == 3 && a.bn_mode == 2)
+            hipLaunchKernelGGL((k_carry<double, R, 0>), dim3(1), dim3(1024), 0, st, a, (int)grid);
+    } else if (D == 1) {
+        int ns = nseg_for(x->nx, x->nz, x->ny);
+        unsigned grid = nblk((long)x->nx * x->nz * ns);
+        hipLaunchKernelGGL((k_flux_yz<double, R, S, 1>), dim3(grid), dim3(256), 0, st, a, ns);
+        if (S == 3 && a.bn_mode == 2)
+            hipLaunchKernelGGL((k_carry<double, R, 1>), dim3(1), dim3(1024), 0, st, a, (int)grid);
+    } else {
+        int ns = nseg_for(x->nx, x->ny, x->nz);
+        unsigned grid = nblk((long)x->nx * x->ny * ns);
+        hipLaunchKernelGGL((k_flux_yz<double, R, S, 2>), dim3(grid), dim3(256), 0, st, a, ns);
+        if (S == 3 && a.bn_mode == 2)
+            hipLaunchKernelGGL((k_carry<double, R, 2>), dim3(1), dim3(1024), 0, st, a, (int)grid);
+    }
+}
+
+template <int R>
+static void launch_sweep_r(cumc3d_ctx *x, const Args &a, int D, hipStream_t st) {
+    switch (x->d.riemann) {
+        case 0: launch_sweep_rs<R, 0>(x, a, D, st); break;
+        case 1: launch_sweep_rs<R, 1>(x, a, D, st); break;
+        case 2: launch_sweep_rs<R, 2>(x, a, D, st); break;
+        default: launch_sweep_rs<R, 3>(x, a, D, st); break;
+    }
+}
+
+static void launch_sweep(cumc3d_ctx *x, const Args &a, int D, hipStream_t st) {
+    switch (x->d.recon) {
+        case 0: launch_sweep_r<0>(x, a, D, st); break;
+        case 1: launch_sweep_r<1>(x, a, D, st); break;
+        case 2: launch_sweep_r<2>(x, a, D, st); break;
+        case 3: launch_sweep_r<3>(x, a, D, st); break;
+        default: launch_sweep_r<4>(x, a, D, st); break;
+    }
+}
+
+static long blocks_needed(cumc3d_ctx *x) {
+    long waves = ((long)(x->nx + 2) * x->ny * x->nz + 62) / 63 + 1;
+    long bx = (waves * 64 + 255) / 256;
+    long by = nblk((long)x->nx * x->nz * nseg_for(x->nx, x->nz, x->ny));
+    long bz = nblk((long)x->nx * x->ny * nseg_for(x->nx, x->ny, x->nz));
+    long m = bx > by ? bx : by;
+    return m > bz ? m : bz;
+}
+
+// ---------------------------------------------------------------------------
+// stage / step
+// ---------------------------------------------------------------------------
+static constexpr int TSTAGES = 192;
+static inline void tmark(cumc3d_ctx *x, int slot) {
+    if (x->timing && x->tstage < TSTAGES) hipEventRecord(x->ev[x->tstage * 7 + slot], x->stream);
+}
+
+static void run_stage(cumc3d_ctx *x, int stage) {
+    hipStream_t st = x->stream;
+    Args a = make_args(x);
+    const long nn = (long)x->nx * x->ny * x->nz;
+    tmark(x, 0);
+    launch_sweep(x, a, 0, st);
+    tmark(x, 1);
+    if (x->d.n_dim > 1) launch_sweep(x, a, 1, st);
+    tmark(x, 2);
+    if (x->d.n_dim > 2) launch_sweep(x, a, 2, st);
+    tmark(x, 3);
+    hipLaunchKernelGGL(k_edge, dim3(nblk((long)(x->nx + 1) * (x->ny + 1) * (x->nz + 1))), dim3(256), 0, st, a);
+    tmark(x, 4);
+    if (stage == 1) hipLaunchKernelGGL(k_update<1>, dim3(nblk(nn)), dim3(256), 0, st, a);
+    else if (stage == 2) hipLaunchKernelGGL(k_update<2>, dim3(nblk(nn)), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_update<3>, dim3(nblk(nn)), dim3(256), 0, st, a);
+    tmark(x, 5);
+    x->cur = 1 - x->cur;
+    Args g = make_args(x);
+    hipLaunchKernelGGL(k_ghost, dim3(nblk(x->G)), dim3(256), 0, st, g);
+    tmark(x, 6);
+    if (x->timing && x->tstage < TSTAGES) x->tstage += 1;
+}
+
+// per-kernel-class device times from the recorded events (after a sync)
+static void accumulate_timing(cumc3d_ctx *x) {
+    int dims = x->d.n_dim;
+    for (int st = 0; st < x->tstage; ++st) {
+        hipEvent_t *e = &x->ev[st * 7];
+        for (int s = 0; s < 6; ++s) {
+            if ((s == 1 && dims < 2) || (s == 2 && dims < 3)) continue;
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, e[s], e[s + 1]) != hipSuccess) continue;
+            x->tsum[s] += ms;
+            x->tcnt[s] += 1;
+        }
+    }
+}
+
+static int launch_finddt(cumc3d_ctx *x) {
+    Args a = make_args(x);
+    const long nn = (long)x->nx * x->ny * x->nz;
+    hipLaunchKernelGGL(k_finddt, dim3(nblk(nn)), dim3(256), 0, x->stream, a);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *cumc3d_error_string(int code) {
+    switch (code) {
+        case CUMC3D_OK: return "ok";
+        case CUMC3D_EINVAL: return "invalid argument";
+        case CUMC3D_EHIP: return "HIP runtime error";
+        case CUMC3D_ENOMEM: return "device allocation failed";
+        case CUMC3D_ESTATE: return "call out of order";
+        case CUMC3D_ENONFINITE: return "non-finite or non-positive time step";
+        default: return "unknown error";
+    }
+}
+
+const char *cumc3d_last_error(void) { return g_err.c_str(); }
+
+int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
+    g_err.clear();
+    if (!desc || !out) return fail(CUMC3D_EINVAL, "null descriptor or output pointer");
+    *out = nullptr;
+    const cumc3d_desc &d = *desc;
+    if (d.nx < 1 || d.ny < 1 || d.nz < 1) return fail(CUMC3D_EINVAL, "grid sizes must be >= 1");
+    if (d.n_dim < 1 || d.n_dim > 3) return fail(CUMC3D_EINVAL, "n_dim must be 1, 2 or 3");
+    if (d.n_dim == 1 && (d.ny != 1 || d.nz != 1)) return fail(CUMC3D_EINVAL, "n_dim=1 needs ny=nz=1");
+    if (d.n_dim == 2 && d.nz != 1) return fail(CUMC3D_EINVAL, "n_dim=2 needs nz=1");
+    if (d.coordinate_flag != 0 && d.coordinate_flag != 1)
+        return fail(CUMC3D_EINVAL, "coordinate_flag must be 0 (Cartesian) or 1 (cylindrical)");
+    if (d.riemann < 0 || d.riemann > 3) return fail(CUMC3D_EINVAL, "unknown Riemann solver");
+    if (d.recon < 0 || d.recon > 4) return fail(CUMC3D_EINVAL, "unknown reconstruction");
+    if (d.model < 0 || d.model > 1) return fail(CUMC3D_EINVAL, "unknown model");
+    if (d.hlld_bn_mode < 0 || d.hlld_bn_mode > 2) return fail(CUMC3D_EINVAL, "unknown hlld_bn_mode");
+    if (d.precision != CUMC3D_FP64) return fail(CUMC3D_EINVAL, "only CUMC3D_FP64 is available in this build");
+    if (!d.xF || !d.yF || !d.zF) return fail(CUMC3D_EINVAL, "face coordinate arrays are required");
+    for (int i = 0; i < 6; ++i)
+        if (d.boundary_flag[i] < 0 || d.boundary_flag[i] > 4) return fail(CUMC3D_EINVAL, "boundary flag out of range");
+    int nmin = d.recon >= 3 ? 2 : 1;
+    int nd3[3] = {d.nx, d.ny, d.nz};
+    for (int a = 0; a < d.n_dim; ++a)
+        for (int s = 0; s < 2; ++s)
+            if (d.boundary_flag[2 * a + s] != 1 && nd3[a] < 3)
+                return fail(CUMC3D_EINVAL, "periodic or mirror boundaries need at least 3 cells on that axis");
+    (void)nmin;
+
+    cumc3d_ctx *x = new cumc3d_ctx();
+    x->d = d;
+    x->nx = d.nx; x->ny = d.ny; x->nz = d.nz;
+    x->NX = d.nx + 6; x->NY = d.ny + 6; x->NZ = d.nz + 6;
+    x->G = x->NX * x->NY * x->NZ;
+    if (!setup_bfac(d.boundary_flag, x->bfac)) {
+        delete x;
+        return fail(CUMC3D_EINVAL, "boundary flag combination rejected by SETUP_EQNS (setup.f90:120-322)");
+    }
+    // grid metrics: GetGrid (getgrid.f90:43-63) from the face coordinates
+    const double *Fc[3] = {d.xF, d.yF, d.zF};
+    std::vector<double> hm;
+    long off[3][4];
+    for (int a = 0; a < 3; ++a) {
+        int n = nd3[a];
+        std::vector<double> f(Fc[a], Fc[a] + n + 7), dd(n + 7, 0.0), cc(n + 7, 0.0), w;
+        for (int j = -2; j <= n + 3; ++j) {
+            dd[j + 3] = f[j + 3] - f[j + 2];
+            cc[j + 3] = (f[j + 3] + f[j + 2]) / 2;
+        }
+        weights(d.recon, dd, n, w);
+        off[a][0] = (long)hm.size(); hm.insert(hm.end(), f.begin(), f.end());
+        off[a][1] = (long)hm.size(); hm.insert(hm.end(), dd.begin(), dd.end());
+        off[a][2] = (long)hm.size(); hm.insert(hm.end(), cc.begin(), cc.end());
+        off[a][3] = (long)hm.size(); hm.insert(hm.end(), w.begin(), w.end());
+    }
+    int rc = 0;
+    auto cleanup = [&]() { cumc3d_finalize(x); };
+    if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete x;
+        return fail(CUMC3D_EHIP, "hipStreamCreate failed");
+    }
+    if (hipMalloc(&x->met, hm.size() * sizeof(double)) != hipSuccess) { cleanup(); return fail(CUMC3D_ENOMEM, "metrics"); }
+    if (hipMemcpy(x->met, hm.data(), hm.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        cleanup();
+        return fail(CUMC3D_EHIP, "metric upload");
+    }
+    for (int a = 0; a < 3; ++a) {
+        x->ax[a].xF = x->met + off[a][0];
+        x->ax[a].d = x->met + off[a][1];
+        x->ax[a].c = x->met + off[a][2];
+        x->ax[a].w = x->met + off[a][3];
+    }
+    // per-cell fields
+    const int nfields = 5 + 2 + 3 + 6 + 8 + 21 + 3 + 3;
+    size_t bytes = (size_t)nfields * x->G * sizeof(double);
+    if (hipMalloc(&x->pool, bytes) != hipSuccess) {
+        cleanup();
+        return fail(CUMC3D_ENOMEM, "state allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    if (hipMemset(x->pool, 0, bytes) != hipSuccess) { cleanup(); return fail(CUMC3D_EHIP, "memset"); }
+    double *p = x->pool;
+    auto take = [&]() { double *r = p; p += x->G; return r; };
+    for (int i = 0; i < 5; ++i) x->P[i] = take();
+    x->eps = take();
+    x->cs = take();
+    for (int i = 0; i < 3; ++i) x->Bc[i] = take();
+    for (int b = 0; b < 2; ++b)
+        for (int i = 0; i < 3; ++i) x->B[b][i] = take();
+    for (int i = 0; i < 8; ++i) x->Uo[i] = take();
+    for (int dd = 0; dd < 3; ++dd)
+        for (int i = 0; i < 7; ++i) x->F[dd][i] = take();
+    for (int i = 0; i < 3; ++i) x->E[i] = take();
+    for (int i = 0; i < 3; ++i) x->floors[i] = take();
+    if (hipMalloc(&x->S, sizeof(DevScalars)) != hipSuccess) { cleanup(); return fail(CUMC3D_ENOMEM, "scalars"); }
+    DevScalars s0;
+    memset(&s0, 0, sizeof(s0));
+    s0.total_time = 1.0e300;
+    s0.active = 1;
+    s0.dtmin_bits = (unsigned long long)0x7E37E43C8800759CULL;   // bits of 1e300
+    if (hipMemcpy(x->S, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess) { cleanup(); return fail(CUMC3D_EHIP, "scalars"); }
+    x->nblockmax = blocks_needed(x) + 16;
+    if (hipMalloc(&x->blockmax, x->nblockmax * sizeof(unsigned long long)) != hipSuccess) {
+        cleanup();
+        return fail(CUMC3D_ENOMEM, "blockmax");
+    }
+    hipMemset(x->blockmax, 0, x->nblockmax * sizeof(unsigned long long));
+    *out = x;
+    return rc;
+}
+
+int cumc3d_upload(cumc3d_ctx *x, const double *prim, const double *eps, const double *floors3) {
+    if (!x || !prim || !eps) return fail(CUMC3D_EINVAL, "null argument");
+    if (x->d.model == CUMC3D_MODEL_TORUS && !floors3) return fail(CUMC3D_EINVAL, "torus model needs floor arrays");
+    double *tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, (size_t)8 * x->G * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(tmp, prim, (size_t)8 * x->G * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    hipLaunchKernelGGL(k_split_prim, dim3(nblk(x->G)), dim3(256), 0, x->stream, tmp, x->P[0], x->P[1], x->P[2],
+                       x->P[3], x->P[4], x->B[0][0], x->B[0][1], x->B[0][2], x->B[1][0], x->B[1][1], x->B[1][2], x->G);
+    HIPCHK(hipMemcpyAsync(x->eps, eps, x->G * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    if (floors3)
+        for (int i = 0; i < 3; ++i)
+            HIPCHK(hipMemcpyAsync(x->floors[i], floors3 + (size_t)i * x->G, x->G * sizeof(double),
+                                  hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipFree(tmp));
+    x->cur = 0;
+    x->uploaded = true;
+    x->initialized = false;
+    return 0;
+}
+
+int cumc3d_initial_update(cumc3d_ctx *x) {
+    if (!x) return fail(CUMC3D_EINVAL, "null context");
+    if (!x->uploaded) return fail(CUMC3D_ESTATE, "upload an initial model first");
+    Args a = make_args(x);
+    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, x->stream, x->S, 0.0);   // also re-activates the stage kernels
+    hipLaunchKernelGGL(k_init_interior, dim3(nblk((long)x->nx * x->ny * x->nz)), dim3(256), 0, x->stream, a);
+    hipLaunchKernelGGL(k_ghost, dim3(nblk(x->G)), dim3(256), 0, x->stream, a);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->initialized = true;
+    x->dt_pending = false;
+    return 0;
+}
+
+int cumc3d_finddt(cumc3d_ctx *x, double *dt) {
+    if (!x || !dt) return fail(CUMC3D_EINVAL, "null argument");
+    if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
+    if (!x->dt_pending) launch_finddt(x);
+    x->dt_pending = false;
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, &x->S->dtmin_bits, sizeof(bits), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    double m;
+    memcpy(&m, &bits, sizeof(m));
+    *dt = m < 1.0e5 ? m : 1.0e5;
+    unsigned long long reset = 0x7E37E43C8800759CULL;
+    HIPCHK(hipMemcpyAsync(&x->S->dtmin_bits, &reset, sizeof(reset), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (!(*dt > 0.0) || !std::isfinite(*dt)) return fail(CUMC3D_ENONFINITE, "FINDDT produced a non-positive dt");
+    return 0;
+}
+
+int cumc3d_rk3_step(cumc3d_ctx *x, double dt) {
+    if (!x) return fail(CUMC3D_EINVAL, "null context");
+    if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
+    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, x->stream, x->S, dt);
+    for (int s = 1; s <= 3; ++s) run_stage(x, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(x->stream));
+    // stage 3 reduced the CFL minimum of the new state (FINDDT of the next step)
+    x->dt_pending = true;
+    return 0;
+}
+
+int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_step, double *t, double *dts,
+               int *steps_done) {
+    if (!x || !t) return fail(CUMC3D_EINVAL, "null argument");
+    if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
+    double *dlog = nullptr;
+    if (dts && max_steps > 0) HIPCHK(hipMalloc(&dlog, (size_t)max_steps * sizeof(double)));
+    // set t and total_time
+    DevScalars h;
+    HIPCHK(hipMemcpy(&h, x->S, sizeof(h), hipMemcpyDeviceToHost));
+    h.t = *t;
+    h.total_time = total_time;
+    h.nstep = 0;
+    HIPCHK(hipMemcpy(x->S, &h, sizeof(h), hipMemcpyHostToDevice));
+    if (x->timing) {
+        if (x->ev.empty()) {
+            x->ev.resize((size_t)TSTAGES * 7);
+            for (auto &e : x->ev) HIPCHK(hipEventCreate(&e));
+        }
+        for (int i = 0; i < 7; ++i) { x->tsum[i] = 0; x->tcnt[i] = 0; }
+        x->tstage = 0;
+    }
+    hipEvent_t s0 = nullptr, s1 = nullptr;
+    if (x->timing) {
+        HIPCHK(hipEventCreate(&s0));
+        HIPCHK(hipEventCreate(&s1));
+        HIPCHK(hipEventRecord(s0, x->stream));
+    }
+    int n = 0;
+    double tcur = *t;
+    for (; n < max_steps; ++n) {
+        if (sync_each_step && !(tcur < total_time)) break;
+        // FINDDT: the previous stage 3 already reduced the minimum
+        if (!x->dt_pending) launch_finddt(x);
+        hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, x->stream, x->S);
+        for (int s = 1; s <= 3; ++s) run_stage(x, s);
+        x->dt_pending = true;
+        hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(1), 0, x->stream, x->S, dlog, n);
+        if (sync_each_step) {
+            HIPCHK(hipMemcpyAsync(&tcur, &x->S->t, sizeof(double), hipMemcpyDeviceToHost, x->stream));
+            HIPCHK(hipStreamSynchronize(x->stream));
+        }
+    }
+    if (x->timing) {
+        HIPCHK(hipEventRecord(s1, x->stream));
+        HIPCHK(hipEventSynchronize(s1));
+        float ms = 0;
+        hipEventElapsedTime(&ms, s0, s1);
+        accumulate_timing(x);
+        x->tsum[6] = ms;
+        x->tcnt[6] = n > 0 ? n : 1;
+        hipEventDestroy(s0);
+        hipEventDestroy(s1);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(&h, x->S, sizeof(h), hipMemcpyDeviceToHost));
+    *t = h.t;
+    int done = h.nstep;
+    if (steps_done) *steps_done = done;
+    if (dts && dlog) {
+        HIPCHK(hipMemcpy(dts, dlog, (size_t)(done < max_steps ? done : max_steps) * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    }
+    if (dlog) HIPCHK(hipFree(dlog));
+    if (h.nonfinite) {
+        h.nonfinite = 0;
+        HIPCHK(hipMemcpy(x->S, &h, sizeof(h), hipMemcpyHostToDevice));
+        return fail(CUMC3D_ENONFINITE, "a cell's CFL time step was NaN or non-positive (NaN or negative energy in the state)");
+    }
+    if (!std::isfinite(h.dt) || h.dt < 0.0) return fail(CUMC3D_ENONFINITE, "time step became non-finite");
+    return 0;
+}
+
+int cumc3d_download(cumc3d_ctx *x, double *prim, double *eps, double *bcell, double *cs) {
+    if (!x) return fail(CUMC3D_EINVAL, "null context");
+    double *tmp = nullptr;
+    if (prim || bcell) HIPCHK(hipMalloc(&tmp, (size_t)8 * x->G * sizeof(double)));
+    double **B = x->B[x->cur];
+    if (prim) {
+        hipLaunchKernelGGL(k_join, dim3(nblk(x->G)), dim3(256), 0, x->stream, tmp, 8, x->P[0], x->P[1], x->P[2],
+                           x->P[3], x->P[4], B[0], B[1], B[2], x->G);
+        HIPCHK(hipMemcpyAsync(prim, tmp, (size_t)8 * x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+        HIPCHK(hipStreamSynchronize(x->stream));
+    }
+    if (bcell) {
+        hipLaunchKernelGGL(k_join, dim3(nblk(x->G)), dim3(256), 0, x->stream, tmp, 3, x->Bc[0], x->Bc[1], x->Bc[2],
+                           x->Bc[2], x->Bc[2], x->Bc[2], x->Bc[2], x->Bc[2], x->G);
+        HIPCHK(hipMemcpyAsync(bcell, tmp, (size_t)3 * x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+        HIPCHK(hipStreamSynchronize(x->stream));
+    }
+    if (eps) HIPCHK(hipMemcpyAsync(eps, x->eps, x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    if (cs) HIPCHK(hipMemcpyAsync(cs, x->cs, x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (tmp) HIPCHK(hipFree(tmp));
+    return 0;
+}
+
+int cumc3d_divb(cumc3d_ctx *x, double *maxdivb) {
+    if (!x || !maxdivb) return fail(CUMC3D_EINVAL, "null argument");
+    unsigned long long *d = nullptr, h = 0;
+    HIPCHK(hipMalloc(&d, sizeof(*d)));
+    HIPCHK(hipMemsetAsync(d, 0, sizeof(*d), x->stream));
+    Args a = make_args(x);
+    hipLaunchKernelGGL(k_divb, dim3(nblk((long)x->nx * x->ny * x->nz)), dim3(256), 0, x->stream, a, d);
+    HIPCHK(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipFree(d));
+    memcpy(maxdivb, &h, sizeof(double));
+    return 0;
+}
+
+int cumc3d_finalize(cumc3d_ctx *x) {
+    if (!x) return 0;
+    if (x->stream) hipStreamSynchronize(x->stream);
+    for (auto &e : x->ev) hipEventDestroy(e);
+    if (x->pool) hipFree(x->pool);
+    if (x->met) hipFree(x->met);
+    if (x->S) hipFree(x->S);
+    if (x->blockmax) hipFree(x->blockmax);
+    if (x->stream) hipStreamDestroy(x->stream);
+    delete x;
+    return 0;
+}
+
+int cumc3d_set_timing(cumc3d_ctx *x, int enable) {
+    if (!x) return fail(CUMC3D_EINVAL, "null context");
+    x->timing = enable != 0;
+    return 0;
+}
+
+int cumc3d_get_timing(cumc3d_ctx *x, double *ms, int n) {
+    if (!x || !ms) return fail(CUMC3D_EINVAL, "null argument");
+    for (int i = 0; i < n && i < 7; ++i) ms[i] = x->tcnt[i] ? x->tsum[i] / x->tcnt[i] : 0.0;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,723 @@
+// Kernels of one SSP-RK3 stage of the MHD step on MI355X (gfx950).
+//
+// Data layout: every field is a separate array (SoA) over the ghosted box
+// -2..n+3 per axis with x fastest, cell (j,k,l) at (j+2) + NX*((k+2) + NY*(l+2)),
+// NX=nx+6, NY=ny+6 -- the reference's cell ordering (setup.f90:348-351) with
+// the variable index moved out to the slowest position.
+//
+// Per stage: x/y/z interface fluxes (reconstruction + face states + Riemann
+// solver), the HLLD carry fix-up per sweep, edge EMFs, the fused update
+// (flux divergence, sources, RK combine, CT, cons->prim->cons, floors, EOS,
+// CFL minimum in stage 3) and the ghost fill.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "physics.hpp"
+
+namespace cumc3d {
+
+constexpr int NW = 14;   // weight slots per cell (PPM needs 14, WENO 10, TVD 2)
+
+struct DevScalars {
+    double dt;                        // dt of the current step
+    double t;                         // global_time
+    double total_time;
+    unsigned long long dtmin_bits;    // CFL min (bit pattern of a positive double)
+    double carry[3];                  // HLLD usstar carried per field component (x,y,z)
+    int nstep;
+    int active;                       // 0 once global_time >= total_time: stage kernels exit
+    int nonfinite;                    // set when a cell's CFL time step is NaN or not positive
+    int pad;
+};
+
+struct Axis {
+    const double *xF;   // face coordinate, index j+3 (j=-3..n+3)
+    const double *d;    // width, index j+3
+    const double *c;    // centre, index j+3
+    const double *w;    // reconstruction weights, (j+3)*NW
+};
+
+struct Args {
+    int nx, ny, nz, ndim, coord, model, rt_gravity, bn_mode;
+    int bc[6];
+    int bfac[6][12];        // [2*axis+side][var 1..11], SETUP_EQNS parity factors
+    long NX, NY, G;
+    double g1, cfl, r_sh;
+    Axis ax[3];
+    double *P[5];           // rho, vx, vy, vz, p
+    double *eps, *cs;
+    double *Bc[3];          // cell-centred field (bcell)
+    double *Bin[3];         // face field at stage start (prim(ibx..ibz))
+    double *Bout[3];        // face field written by the update
+    double *Uo[8];          // u_old
+    double *F[3][7];        // interface fluxes: 5 hydro + 2 face EMFs
+    double *E[3];           // edge EMFs efield_x/y/z
+    const double *rfl, *pfl, *efl;   // torus floors
+    DevScalars *S;
+    unsigned long long *blockmax;    // per-block max of double-star interface keys
+};
+
+__device__ __forceinline__ long cidx(const Args &a, int j, int k, int l) {
+    return (long)(j + 2) + a.NX * ((long)(k + 2) + a.NY * (long)(l + 2));
+}
+
+// Ghost-cell source along one axis (BOUNDARYP_NM, boundary.f90:423-745):
+// flag 0 periodic, 1 outflow, >=2 parity mirror.  side: -1 interior, 0 inner, 1 outer.
+__device__ __forceinline__ int gmap(int i, int n, int fin, int fout, int &side) {
+    if (i < 1) {
+        side = 0;
+        int m = 1 - i;
+        return fin == 0 ? n + 1 - m : (fin == 1 ? 1 : m);
+    }
+    if (i > n) {
+        side = 1;
+        int m = i - n;
+        return fout == 0 ? m : (fout == 1 ? n : n + 1 - m);
+    }
+    side = -1;
+    return i;
+}
+
+// parity factor of variable var for a ghost on (axis, side); constant array
+// indices only, so the kernel argument block is never copied to scratch
+__device__ __forceinline__ double gsign(const Args &a, int axis, int side, int var) {
+    if (side < 0) return 1.0;
+    int f, b;
+    if (axis == 0) { f = side ? a.bc[1] : a.bc[0]; b = side ? a.bfac[1][var] : a.bfac[0][var]; }
+    else if (axis == 1) { f = side ? a.bc[3] : a.bc[2]; b = side ? a.bfac[3][var] : a.bfac[2][var]; }
+    else { f = side ? a.bc[5] : a.bc[4]; b = side ? a.bfac[5][var] : a.bfac[4][var]; }
+    return f >= 2 ? (double)b : 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// interface fluxes
+// ---------------------------------------------------------------------------
+
+// Serial position of interface (j,k,l) in the reference's Riemann loop
+// (rsolver_module.f90:596-598: l outer, j inner; the sweep axis starts at 0,
+// the others at 1), plus one.
+template <int D>
+__device__ __forceinline__ unsigned long long serial_key(const Args &a, int j, int k, int l) {
+    if constexpr (D == 0) return ((unsigned long long)(l - 1) * a.ny + (k - 1)) * (a.nx + 1) + j + 1;
+    else if constexpr (D == 1) return ((unsigned long long)(l - 1) * (a.ny + 1) + k) * a.nx + (j - 1) + 1;
+    else return ((unsigned long long)l * a.ny + (k - 1)) * a.nx + (j - 1) + 1;
+}
+
+// 7-variable stencil of cell c along D: rho, vx, vy, vz, p and the two
+// transverse cell-centred field components.
+template <class T, int R, int D>
+__device__ __forceinline__ void load_stencil(const Args &a, long c, long st, T s[7][5]) {
+    constexpr int t1 = (D == 0) ? 1 : 0;
+    constexpr int t2 = (D == 2) ? 1 : 2;
+    const double *src[7] = {a.P[0], a.P[1], a.P[2], a.P[3], a.P[4], a.Bc[t1], a.Bc[t2]};
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+        if constexpr (R >= 3) {
+            s[v][0] = (T)src[v][c - 2 * st];
+            s[v][4] = (T)src[v][c + 2 * st];
+        }
+        s[v][1] = (T)src[v][c - st];
+        s[v][2] = (T)src[v][c];
+        s[v][3] = (T)src[v][c + st];
+    }
+}
+
+template <class T> __device__ __forceinline__ void load_w(const double *w, T out[NW], int n) {
+#pragma unroll
+    for (int i = 0; i < NW; ++i)
+        if (i < n) out[i] = (T)w[i];
+}
+
+template <int R> constexpr int nweights() { return R <= 2 ? 2 : (R == 3 ? 14 : 10); }
+
+template <class T, int R, int D>
+__device__ __forceinline__ void recon_at(const Args &a, int j, int k, int l, long st, Face<T> &fm, Face<T> &fp) {
+    long c = cidx(a, j, k, l);
+    T s[7][5];
+    load_stencil<T, R, D>(a, c, st, s);
+    int i = (D == 0) ? j : (D == 1 ? k : l);
+    T w[NW];
+    load_w<T>(a.ax[D].w + (long)(i + 3) * NW, w, nweights<R>());
+    recon_cell<T, R>(w, s, (T)a.g1, fm, fp);
+}
+
+template <int D> __device__ __forceinline__ long stride(const Args &a) {
+    return D == 0 ? 1L : (D == 1 ? a.NX : a.NX * a.NY);
+}
+
+// Store the flux of interface (j,k,l) along D: 5 hydro fluxes and the two face
+// EMFs of MHD_FLUX (mhd_module.f90:229-271).  Slot 5 holds the face EMF with
+// the lower field index, slot 6 the other.
+template <class T, int D>
+__device__ __forceinline__ void store_flux(const Args &a, long c, const RiemannOut<T> &o) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) a.F[D][i][c] = (double)o.f[i];
+    if constexpr (D == 0) { a.F[0][5][c] = (double)o.f[7]; a.F[0][6][c] = -(double)o.f[6]; }
+    else if constexpr (D == 1) { a.F[1][5][c] = -(double)o.f[7]; a.F[1][6][c] = (double)o.f[5]; }
+    else { a.F[2][5][c] = (double)o.f[6]; a.F[2][6][c] = -(double)o.f[5]; }
+}
+
+__device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v) {
+    __shared__ unsigned long long red[16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_down(v, off, 64);
+        v = o > v ? o : v;
+    }
+    int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int nw = (blockDim.x + 63) >> 6;
+        for (int i = 1; i < nw; ++i) v = red[i] > v ? red[i] : v;
+    }
+    return v;
+}
+
+// x sweep.  The (nx+2) cells j=0..nx+1 of every row are laid end to end; each
+// 64-lane wave reconstructs 64 consecutive cells and solves the 63 interfaces
+// between them, taking the right neighbour's left face with a lane shuffle.
+// Waves overlap by one cell, so every cell is reconstructed about 64/63 times.
+template <class T, int R, int S>
+__global__ __launch_bounds__(256) void k_flux_x(Args a) {
+    if (!a.S->active) return;
+    const int lane = threadIdx.x & 63;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long g = wave * 63 + lane;
+    const long RL = a.nx + 2;
+    const long gtot = RL * a.ny * a.nz;
+    const bool live = g < gtot;
+    int j = 0, k = 1, l = 1;
+    if (live) {
+        long r = g / RL;
+        j = (int)(g - r * RL);
+        k = 1 + (int)(r % a.ny);
+        l = 1 + (int)(r / a.ny);
+    }
+    Face<T> fm, fp;
+    if (live) recon_at<T, R, 0>(a, j, k, l, 1, fm, fp);
+    Face<T> nm;
+    nm.rho = __shfl_down(fm.rho, 1, 64);
+    nm.v[0] = __shfl_down(fm.v[0], 1, 64);
+    nm.v[1] = __shfl_down(fm.v[1], 1, 64);
+    nm.v[2] = __shfl_down(fm.v[2], 1, 64);
+    nm.p = __shfl_down(fm.p, 1, 64);
+    nm.bt1 = __shfl_down(fm.bt1, 1, 64);
+    nm.bt2 = __shfl_down(fm.bt2, 1, 64);
+    nm.eps = __shfl_down(fm.eps, 1, 64);
+    nm.cs = __shfl_down(fm.cs, 1, 64);
+    unsigned long long key = 0;
+    if (live && lane < 63 && j <= a.nx && g + 1 < gtot) {
+        long c = cidx(a, j, k, l);
+        T bn = (T)a.Bin[0][c];
+        Q<T> qL = to_q<T, 0>(fp, bn), qR = to_q<T, 0>(nm, bn);
+        RiemannOut<T> o;
+        riemann<T, 0, S>(qL, qR, fp.eps, nm.eps, fp.cs, nm.cs, a.bn_mode, (T)a.S->carry[0], o);
+        store_flux<T, 0>(a, c, o);
+        if (o.ds) key = serial_key<0>(a, j, k, l);
+    }
+    if constexpr (S == 3) {
+        if (a.bn_mode == 2) {
+            key = block_max_u64(key);
+            if (threadIdx.x == 0) a.blockmax[blockIdx.x] = key;
+        }
+    }
+}
+
+// y and z sweeps.  Threads run along x (coalesced); each thread marches along
+// the sweep axis over a segment of interfaces [i0,i1), reconstructing cells
+// i0..i1 once and keeping the previous cell's right face in registers.
+template <class T, int R, int S, int D>
+__global__ __launch_bounds__(256) void k_flux_yz(Args a, int nseg) {
+    if (!a.S->active) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int na = (D == 1) ? a.nz : a.ny;    // the other transverse axis
+    const int nd = (D == 1) ? a.ny : a.nz;    // sweep axis
+    const long total = (long)a.nx * na * nseg;
+    unsigned long long key = 0;
+    if (t < total) {
+        int j = 1 + (int)(t % a.nx);
+        long rest = t / a.nx;
+        int o2 = 1 + (int)(rest % na);
+        int s = (int)(rest / na);
+        int i0 = (int)(((long)s * (nd + 1)) / nseg);
+        int i1 = (int)(((long)(s + 1) * (nd + 1)) / nseg);
+        const long st = stride<D>(a);
+        Face<T> prev;
+        for (int i = i0; i <= i1; ++i) {
+            int k = (D == 1) ? i : o2;
+            int l = (D == 1) ? o2 : i;
+            Face<T> fm, fp;
+            recon_at<T, R, D>(a, j, k, l, st, fm, fp);
+            if (i > i0) {
+                int kk = (D == 1) ? i - 1 : k;
+                int ll = (D == 1) ? l : i - 1;
+                long c = cidx(a, j, kk, ll);
+                T bn = (T)a.Bin[D][c];
+                Q<T> qL = to_q<T, D>(prev, bn), qR = to_q<T, D>(fm, bn);
+                RiemannOut<T> o;
+                riemann<T, D, S>(qL, qR, prev.eps, fm.eps, prev.cs, fm.cs, a.bn_mode, (T)a.S->carry[D], o);
+                store_flux<T, D>(a, c, o);
+                if (o.ds) {
+                    unsigned long long kk2 = serial_key<D>(a, j, kk, ll);
+                    key = kk2 > key ? kk2 : key;
+                }
+            }
+            prev = fp;
+        }
+    }
+    if constexpr (S == 3) {
+        if (a.bn_mode == 2) {
+            key = block_max_u64(key);
+            if (threadIdx.x == 0) a.blockmax[blockIdx.x] = key;
+        }
+    }
+}
+
+// After a sweep: find the last double-star interface in the reference's serial
+// loop order and recompute its double-star transverse field, which the serial
+// reference leaves behind in usstarL/usstarR and reads back as usstar(ibn) in
+// later sweeps (rsolver_module.f90:704-741).
+template <class T, int R, int D>
+__global__ __launch_bounds__(1024) void k_carry(Args a, int nblocks) {
+    if (!a.S->active) return;
+    unsigned long long key = 0;
+    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
+        unsigned long long v = a.blockmax[i];
+        key = v > key ? v : key;
+    }
+    key = block_max_u64(key);
+    if (threadIdx.x != 0 || key == 0) return;
+    unsigned long long q = key - 1;
+    int j, k, l;
+    if constexpr (D == 0) {
+        j = (int)(q % (a.nx + 1)); q /= (a.nx + 1);
+        k = 1 + (int)(q % a.ny); l = 1 + (int)(q / a.ny);
+    } else if constexpr (D == 1) {
+        j = 1 + (int)(q % a.nx); q /= a.nx;
+        k = (int)(q % (a.ny + 1)); l = 1 + (int)(q / (a.ny + 1));
+    } else {
+        j = 1 + (int)(q % a.nx); q /= a.nx;
+        k = 1 + (int)(q % a.ny); l = (int)(q / a.ny);
+    }
+    const long st = stride<D>(a);
+    Face<T> lm, lp, rm, rp;
+    recon_at<T, R, D>(a, j, k, l, st, lm, lp);
+    recon_at<T, R, D>(a, j + (D == 0), k + (D == 1), l + (D == 2), st, rm, rp);
+    T bn = (T)a.Bin[D][cidx(a, j, k, l)];
+    Q<T> qL = to_q<T, D>(lp, bn), qR = to_q<T, D>(rm, bn);
+    RiemannOut<T> o;
+    riemann<T, D, 3>(qL, qR, lp.eps, rm.eps, lp.cs, rm.cs, a.bn_mode, (T)a.S->carry[D], o);
+    a.S->carry[(D + 1) % 3] = (double)o.uss_t1;
+    a.S->carry[(D + 2) % 3] = (double)o.uss_t2;
+}
+
+// ---------------------------------------------------------------------------
+// constrained transport: edge EMFs (flux_ct, mhd_module.f90:310-426)
+// ---------------------------------------------------------------------------
+
+// cell-centred EMF -v x B of an interior cell (mhd_module.f90:316-318)
+template <int E> __device__ __forceinline__ double ecell_raw(const Args &a, long c) {
+    double vx = a.P[1][c], vy = a.P[2][c], vz = a.P[3][c];
+    double bx = a.Bc[0][c], by = a.Bc[1][c], bz = a.Bc[2][c];
+    if constexpr (E == 0) return vz * by - vy * bz;
+    else if constexpr (E == 1) return vx * bz - vz * bx;
+    else return vy * bx - vx * by;
+}
+
+// ecell after BOUNDARY_EFIELD (mhd_module.f90:478-840): ghost cells take the
+// mapped interior value times the parity factors of component E.
+template <int E> __device__ __forceinline__ double ecell_at(const Args &a, int j, int k, int l) {
+    int sx, sy, sz;
+    int jj = gmap(j, a.nx, a.bc[0], a.bc[1], sx);
+    int kk = gmap(k, a.ny, a.bc[2], a.bc[3], sy);
+    int ll = gmap(l, a.nz, a.bc[4], a.bc[5], sz);
+    double s = gsign(a, 0, sx, 9 + E) * gsign(a, 1, sy, 9 + E) * gsign(a, 2, sz, 9 + E);
+    double v = ecell_raw<E>(a, cidx(a, jj, kk, ll));
+    return (sx < 0 && sy < 0 && sz < 0) ? v : s * v;
+}
+
+// Face EMF eface of component E at faces normal to D, after the 1D/2D fill
+// (mhd_module.f90:328-356) and BOUNDARY_EFIELD: face index along D is real,
+// the two cell-centred axes are ghost-mapped.
+template <int E, int D> __device__ __forceinline__ double eface_at(const Args &a, int j, int k, int l) {
+    int sx = -1, sy = -1, sz = -1;
+    if constexpr (D != 0) j = gmap(j, a.nx, a.bc[0], a.bc[1], sx);
+    if constexpr (D != 1) k = gmap(k, a.ny, a.bc[2], a.bc[3], sy);
+    if constexpr (D != 2) l = gmap(l, a.nz, a.bc[4], a.bc[5], sz);
+    double v;
+    if (D < a.ndim) {
+        constexpr int slot = (D == 0) ? (E == 1 ? 5 : 6) : (D == 1 ? (E == 0 ? 5 : 6) : (E == 0 ? 5 : 6));
+        v = a.F[D][slot][cidx(a, j, k, l)];
+    } else {
+        if constexpr (D == 0) j = 1;
+        else if constexpr (D == 1) k = 1;
+        else l = 1;
+        v = ecell_raw<E>(a, cidx(a, j, k, l));
+    }
+    if (sx < 0 && sy < 0 && sz < 0) return v;
+    return (gsign(a, 0, sx, 9 + E) * gsign(a, 1, sy, 9 + E) * gsign(a, 2, sz, 9 + E)) * v;
+}
+
+// Edge EMFs (mhd_module.f90:414-423) on the edges the CT update reads:
+// efield_x for j=1..nx, efield_y for k=1..ny, efield_z for l=1..nz
+// (0..n along the other axes).
+__global__ __launch_bounds__(256) void k_edge(Args a) {
+    if (!a.S->active) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n1 = a.nx + 1, n2 = a.ny + 1;
+    if (t >= n1 * n2 * (a.nz + 1)) return;
+    int j = (int)(t % n1), k = (int)((t / n1) % n2), l = (int)(t / (n1 * n2));
+    long c = cidx(a, j, k, l);
+    if (j >= 1)
+        a.E[0][c] = 0.50 * (eface_at<0, 1>(a, j, k, l) + eface_at<0, 1>(a, j, k, l + 1) + eface_at<0, 2>(a, j, k, l) +
+                            eface_at<0, 2>(a, j, k + 1, l)) -
+                    0.25 * (ecell_at<0>(a, j, k, l) + ecell_at<0>(a, j, k, l + 1) + ecell_at<0>(a, j, k + 1, l) +
+                            ecell_at<0>(a, j, k + 1, l + 1));
+    if (k >= 1)
+        a.E[1][c] = 0.50 * (eface_at<1, 0>(a, j, k, l) + eface_at<1, 0>(a, j, k, l + 1) + eface_at<1, 2>(a, j, k, l) +
+                            eface_at<1, 2>(a, j + 1, k, l)) -
+                    0.25 * (ecell_at<1>(a, j, k, l) + ecell_at<1>(a, j, k, l + 1) + ecell_at<1>(a, j + 1, k, l) +
+                            ecell_at<1>(a, j + 1, k, l + 1));
+    if (l >= 1)
+        a.E[2][c] = 0.50 * (eface_at<2, 0>(a, j, k, l) + eface_at<2, 0>(a, j, k + 1, l) + eface_at<2, 1>(a, j, k, l) +
+                            eface_at<2, 1>(a, j + 1, k, l)) -
+                    0.25 * (ecell_at<2>(a, j, k, l) + ecell_at<2>(a, j, k + 1, l) + ecell_at<2>(a, j + 1, k, l) +
+                            ecell_at<2>(a, j + 1, k + 1, l));
+}
+
+// ---------------------------------------------------------------------------
+// update: FLUX_DIFF + sources + l_operator + RK combine + FROMUTORVE +
+// CUSTOM_CHECKRHO + FROMRVETOU + FINDPRESSURE (+ FINDDT) for one interior cell
+// ---------------------------------------------------------------------------
+
+// l_rk of the face fields (l_operator, spatial.f90:639-680)
+__device__ __forceinline__ double lB_x(const Args &a, int jf, int k, int l) {
+    const double *ey = a.E[1], *ez = a.E[2];
+    long c = cidx(a, jf, k, l), sy = a.NX, sz = a.NX * a.NY;
+    double dy = a.ax[1].d[k + 3], dz = a.ax[2].d[l + 3];
+    if (a.coord == 0) return -((ez[c] - ez[c - sy]) / (dy) - (ey[c] - ey[c - sz]) / (dz));
+    double xF = a.ax[0].xF[jf + 3];
+    return -(ez[c] - ez[c - sy]) / (xF * dy) + (ey[c] - ey[c - sz]) / (dz);
+}
+__device__ __forceinline__ double lB_y(const Args &a, int j, int kf, int l) {
+    const double *ex = a.E[0], *ez = a.E[2];
+    long c = cidx(a, j, kf, l), sz = a.NX * a.NY;
+    double dx = a.ax[0].d[j + 3], dz = a.ax[2].d[l + 3];
+    if (a.coord == 0) return -((ex[c] - ex[c - sz]) / (dz) - (ez[c] - ez[c - 1]) / (dx));
+    return -(ex[c] - ex[c - sz]) / (dz) + (ez[c] - ez[c - 1]) / (dx);
+}
+__device__ __forceinline__ double lB_z(const Args &a, int j, int k, int lf) {
+    const double *ex = a.E[0], *ey = a.E[1];
+    long c = cidx(a, j, k, lf), sy = a.NX;
+    double dx = a.ax[0].d[j + 3], dy = a.ax[1].d[k + 3];
+    if (a.coord == 0) return -((ey[c] - ey[c - 1]) / (dx) - (ex[c] - ex[c - sy]) / (dy));
+    double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2], x = a.ax[0].c[j + 3];
+    return -(xF * ey[c] - xFm * ey[c - 1]) / (x * dx) + (ex[c] - ex[c - sy]) / (x * dy);
+}
+
+template <int STAGE>
+__device__ __forceinline__ double rk(double uo, double u, double dt, double L) {
+    if constexpr (STAGE == 1) return uo + dt * L;
+    else if constexpr (STAGE == 2) return (3.0 / 4.0) * uo + (1.0 / 4.0) * u + (1.0 / 4.0) * dt * L;
+    else return (1.0 / 3.0) * uo + (2.0 / 3.0) * u + (2.0 / 3.0) * dt * L;
+}
+
+// CUSTOM_CHECKRHO of the model (model/riemann/src/custom.f90:115-121,
+// model/torus-cyn/src/custom.f90:254-271)
+__device__ __forceinline__ void checkrho(const Args &a, long c, double &rho, double v[3], double &eps) {
+    if (a.model == 0) {
+        eps = fmax2(eps, 0.0);
+    } else {
+        double diff = rho - a.rfl[c];
+        double f = fmax2(fsign1(diff), 0.0);
+        rho = f * rho + (1.0 - f) * a.rfl[c];
+        v[0] = f * v[0]; v[1] = f * v[1]; v[2] = f * v[2];
+        eps = f * eps + (1.0 - f) * a.efl[c];
+        if (eps < 0.0) eps = a.efl[c];
+    }
+}
+
+// FINDPRESSURE of the model (model/riemann/src/findpressure.f90:17-28,
+// model/torus-cyn/src/findpressure.f90:97-112)
+__device__ __forceinline__ void pressure(const Args &a, long c, double rho, double eps, double &p, double &cs) {
+    double dpdrho, dpdeps;
+    if (a.model == 0 || rho > a.rfl[c]) {
+        p = rho * eps * a.g1;
+        dpdrho = eps * a.g1;
+        dpdeps = rho * a.g1;
+    } else {
+        p = a.pfl[c];
+        dpdrho = a.efl[c] * a.g1;
+        dpdeps = a.rfl[c] * a.g1;
+    }
+    cs = sqrtT(dpdrho + dpdeps * p / (rho * rho));
+}
+
+// FINDDT per cell (rungekutta.f90:272-311)
+__device__ __forceinline__ double cfl_dt(const Args &a, int j, int k, int l, double cs, double rho, const double v[3],
+                                         const double b[3]) {
+    double a2 = cs * cs, a4 = a2 * a2;
+    double b2x = (b[0] * b[0] / rho), b2y = (b[1] * b[1] / rho), b2z = (b[2] * b[2] / rho);
+    double b2 = b2x + b2y + b2z, b4 = b2 * b2;
+    double cfx = sqrtT(0.5 * (a2 + b2 + sqrtT((a4 + 2.0 * a2 * b2 + b4) - 4.0 * a2 * b2x)));
+    double cfy = sqrtT(0.5 * (a2 + b2 + sqrtT((a4 + 2.0 * a2 * b2 + b4) - 4.0 * a2 * b2y)));
+    double cfz = sqrtT(0.5 * (a2 + b2 + sqrtT((a4 + 2.0 * a2 * b2 + b4) - 4.0 * a2 * b2z)));
+    double lam = fmax2(fmax2(fabsT(v[0]) + cfx, fabsT(v[1]) + cfy), fabsT(v[2]) + cfz);
+    double h = a.ax[0].d[j + 3];
+    if (a.coord == 0) {
+        if (a.ndim > 1) h = fmin2(h, a.ax[1].d[k + 3]);
+        if (a.ndim > 2) h = fmin2(h, a.ax[2].d[l + 3]);
+    } else {
+        if (a.ndim > 1 && a.ny > 1) h = fmin2(h, a.ax[0].c[j + 3] * a.ax[1].d[k + 3]);
+        if (a.ndim > 2) h = fmin2(h, a.ax[2].d[l + 3]);
+    }
+    return h * a.cfl / lam;
+}
+
+// Block minimum of per-cell time steps.  A NaN or non-positive candidate
+// (a cell with NaN or negative energy) raises S->nonfinite and is left out.
+__device__ __forceinline__ double block_min_f64(double v, DevScalars *S) {
+    if (!(v > 0.0)) {
+        atomicOr(&S->nonfinite, 1);
+        v = 1.0e300;
+    }
+    __shared__ double red[16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_down(v, off, 64);
+        v = fmin2(v, o);
+    }
+    int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int nw = (blockDim.x + 63) >> 6;
+        for (int i = 1; i < nw; ++i) v = fmin2(v, red[i]);
+    }
+    return v;
+}
+
+// geometry-weighted cell average of face field component D (fromrvetou.f90:153-195)
+template <int D>
+__device__ __forceinline__ double bavg(const Args &a, int j, double up, double lo) {
+    if (D == 0 && a.coord == 1) {
+        double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2];
+        return (xF * up + xFm * lo) / (xF + xFm);
+    }
+    return 0.5 * (up + lo);
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_update(Args a) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nn = (long)a.nx * a.ny * a.nz;
+    if (!a.S->active) {
+        // a step past total_time: keep the state, but the host swaps the face
+        // buffers after every stage, so carry the interior faces across
+        if (t < nn) {
+            const int j = 1 + (int)(t % a.nx), k = 1 + (int)((t / a.nx) % a.ny), l = 1 + (int)(t / ((long)a.nx * a.ny));
+            const long c = cidx(a, j, k, l);
+            for (int d = 0; d < 3; ++d) a.Bout[d][c] = a.Bin[d][c];
+        }
+        return;
+    }
+    double dtloc = 1.0e5;
+    if (t < nn) {
+        const int j = 1 + (int)(t % a.nx), k = 1 + (int)((t / a.nx) % a.ny), l = 1 + (int)(t / ((long)a.nx * a.ny));
+        const long c = cidx(a, j, k, l), sy = a.NX, sz = a.NX * a.NY;
+        const double dt = a.S->dt;
+        // stage-start primitives and conserved variables (FROMRVETOU, fromrvetou.f90:428-438)
+        double rho = a.P[0][c], vx = a.P[1][c], vy = a.P[2][c], vz = a.P[3][c];
+        double bx = a.Bc[0][c], by = a.Bc[1][c], bz = a.Bc[2][c];
+        double eps = a.eps[c];
+        double u[5], uo[5];
+        {
+            double vsq = vx * vx + vy * vy + vz * vz, bsq = bx * bx + by * by + bz * bz;
+            u[0] = rho;
+            u[1] = vx * rho;
+            u[2] = vy * rho;
+            u[3] = vz * rho;
+            u[4] = rho * (eps + 0.5 * vsq) + 0.5 * bsq;
+        }
+        if (STAGE == 1) {
+            for (int i = 0; i < 5; ++i) { uo[i] = u[i]; a.Uo[i][c] = u[i]; }
+        } else {
+            for (int i = 0; i < 5; ++i) uo[i] = a.Uo[i][c];
+        }
+        // sources: GET_SOURCE (spatial.f90:195-226) + CUSTOM_SOURCE
+        double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (a.coord == 1) {
+            double bsq = bx * bx + by * by + bz * bz, xj = a.ax[0].c[j + 3];
+            sc[1] = sc[1] + (a.P[4][c] + rho * vy * vy + 0.5 * bsq - by * by) / (xj);
+            sc[2] = sc[2] - (rho * vx * vy - bx * by) / (xj);
+        }
+        if (a.model == 1) {
+            double diff = rho - a.rfl[c];
+            double f = fmax2(fsign1(diff), 0.0);
+            double xj = a.ax[0].c[j + 3], zl = a.ax[2].c[l + 3];
+            double radius = sqrtT(xj * xj + zl * zl);
+            double dphidr = xj / radius / ((radius - a.r_sh) * (radius - a.r_sh));
+            double dphidz = zl / radius / ((radius - a.r_sh) * (radius - a.r_sh));
+            sc[1] = sc[1] + (-f * rho * dphidr);
+            sc[3] = sc[3] + (-f * rho * dphidz);
+            sc[4] = sc[4] + (-f * rho * (vx * dphidr + vz * dphidz));
+        } else if (a.rt_gravity) {
+            sc[2] = sc[2] - 0.1 * rho;
+        }
+        // FLUX_DIFF (spatial.f90:436-583) + l_operator hydro part
+        double L[5];
+        {
+            double dx = a.ax[0].d[j + 3];
+            for (int i = 0; i < 5; ++i) {
+                const double *f = a.F[0][i];
+                double df;
+                if (a.coord == 1)
+                    df = (a.ax[0].xF[j + 3] * f[c] - a.ax[0].xF[j + 2] * f[c - 1]) / (a.ax[0].c[j + 3] * dx);
+                else df = (f[c] - f[c - 1]) / (dx);
+                L[i] = -df;
+            }
+            if (a.ndim > 1) {
+                double dy = a.ax[1].d[k + 3];
+                for (int i = 0; i < 5; ++i) {
+                    const double *f = a.F[1][i];
+                    double df = (a.coord == 1) ? (f[c] - f[c - sy]) / (a.ax[0].c[j + 3] * dy) : (f[c] - f[c - sy]) / (dy);
+                    L[i] = L[i] - df;
+                }
+            }
+            if (a.ndim > 2) {
+                double dz = a.ax[2].d[l + 3];
+                for (int i = 0; i < 5; ++i) {
+                    const double *f = a.F[2][i];
+                    L[i] = L[i] - (f[c] - f[c - sz]) / dz;
+                }
+            }
+            for (int i = 0; i < 5; ++i) L[i] = L[i] + sc[i];
+        }
+        double un[5];
+        for (int i = 0; i < 5; ++i) un[i] = rk<STAGE>(uo[i], u[i], dt, L[i]);
+        // CT update of the six faces of this cell (faces j-1/j, k-1/k, l-1/l)
+        double bn_up[3], bn_lo[3], bprim_lo[3];
+        {
+            long cl[3] = {c - 1, c - sy, c - sz};
+            int fidx[3] = {j, k, l};
+            double Lup[3] = {lB_x(a, j, k, l), lB_y(a, j, k, l), lB_z(a, j, k, l)};
+            double Llo[3] = {lB_x(a, j - 1, k, l), lB_y(a, j, k - 1, l), lB_z(a, j, k, l - 1)};
+            for (int d = 0; d < 3; ++d) {
+                double bup = a.Bin[d][c], blo = a.Bin[d][cl[d]];
+                double uoup, uolo;
+                if (STAGE == 1) {
+                    uoup = bup;
+                    uolo = blo;
+                    a.Uo[5 + d][c] = bup;
+                    if (fidx[d] == 1) a.Uo[5 + d][cl[d]] = blo;
+                } else {
+                    uoup = a.Uo[5 + d][c];
+                    uolo = a.Uo[5 + d][cl[d]];
+                }
+                bn_up[d] = rk<STAGE>(uoup, bup, dt, Lup[d]);
+                bn_lo[d] = rk<STAGE>(uolo, blo, dt, Llo[d]);
+                // the lowest face (index 0) keeps its stored value in prim
+                // (FROMUTORVE copies faces 1..n only, fromrvetou.f90:201-207)
+                bprim_lo[d] = (fidx[d] == 1) ? blo : bn_lo[d];
+            }
+        }
+        // FROMUTORVE (fromrvetou.f90:153-211)
+        double bce[3] = {bavg<0>(a, j, bn_up[0], bn_lo[0]), bavg<1>(a, j, bn_up[1], bn_lo[1]),
+                         bavg<2>(a, j, bn_up[2], bn_lo[2])};
+        double rn = un[0];
+        double v[3] = {un[1] / rn, un[2] / rn, un[3] / rn};
+        double bsq = bce[0] * bce[0] + bce[1] * bce[1] + bce[2] * bce[2];
+        double vsq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        double en = (un[4] - 0.5 * bsq) / rn - 0.5 * vsq;
+        checkrho(a, c, rn, v, en);
+        if (STAGE == 3) checkrho(a, c, rn, v, en);
+        // FROMRVETOU bcell (fromrvetou.f90:380-422)
+        double bcf[3] = {bavg<0>(a, j, bn_up[0], bprim_lo[0]), bavg<1>(a, j, bn_up[1], bprim_lo[1]),
+                         bavg<2>(a, j, bn_up[2], bprim_lo[2])};
+        double p, cs;
+        pressure(a, c, rn, en, p, cs);
+        a.P[0][c] = rn;
+        a.P[1][c] = v[0];
+        a.P[2][c] = v[1];
+        a.P[3][c] = v[2];
+        a.P[4][c] = p;
+        a.eps[c] = en;
+        a.cs[c] = cs;
+        for (int d = 0; d < 3; ++d) {
+            a.Bc[d][c] = bcf[d];
+            a.Bout[d][c] = bn_up[d];
+        }
+        if (STAGE == 3) dtloc = cfl_dt(a, j, k, l, cs, rn, v, bcf);
+    }
+    if constexpr (STAGE == 3) {
+        dtloc = block_min_f64(dtloc, a.S);
+        if (threadIdx.x == 0) atomicMin(&a.S->dtmin_bits, (unsigned long long)__double_as_longlong(dtloc));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ghost fill: BOUNDARYP_NM + BOUNDARY1D_NM(epsilon) + custom BCs, then
+// FINDPRESSURE on the ghost cells
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ghost(Args a) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.G) return;
+    const int j = (int)(t % a.NX) - 2, k = (int)((t / a.NX) % a.NY) - 2, l = (int)(t / (a.NX * a.NY)) - 2;
+    int sx, sy, sz;
+    const int jj = gmap(j, a.nx, a.bc[0], a.bc[1], sx);
+    const int kk = gmap(k, a.ny, a.bc[2], a.bc[3], sy);
+    const int ll = gmap(l, a.nz, a.bc[4], a.bc[5], sz);
+    const bool gx = sx >= 0, gy = sy >= 0, gz = sz >= 0;
+    if (!gx && !gy && !gz) return;
+    if (!a.S->active) {
+        // inactive step: Bin is the buffer the host just switched to and Bout
+        // the one holding the state; copy the ghost faces back
+        for (int d = 0; d < 3; ++d) a.Bin[d][t] = a.Bout[d][t];
+        return;
+    }
+    const long c = t;
+    // cell-centred primitives and bcell from the fully mapped interior cell
+    const long cs_ = cidx(a, jj, kk, ll);
+    double rho = a.P[0][cs_] * (gsign(a, 0, sx, 1) * gsign(a, 1, sy, 1) * gsign(a, 2, sz, 1));
+    double v[3];
+    for (int d = 0; d < 3; ++d)
+        v[d] = a.P[1 + d][cs_] * (gsign(a, 0, sx, 2 + d) * gsign(a, 1, sy, 2 + d) * gsign(a, 2, sz, 2 + d));
+    for (int d = 0; d < 3; ++d)
+        a.Bc[d][c] = a.Bc[d][cs_] * (gsign(a, 0, sx, 6 + d) * gsign(a, 1, sy, 6 + d) * gsign(a, 2, sz, 6 + d));
+    // epsilon: BOUNDARY1D_NM takes the y-outer branch from boundary_flag(3) (boundary.f90:243)
+    {
+        int ey = kk;
+        bool eps_y = gy;
+        if (sy == 1) {
+            int f3 = a.bc[2], f4 = a.bc[3];
+            int m = k - a.ny;
+            if (f3 == 0) ey = m;
+            else if (f4 == 1) ey = a.ny;
+            else if (f4 >= 2) ey = a.ny + 1 - m;
+            else { ey = k; eps_y = false; }
+        }
+        if (gx || eps_y || gz) a.eps[c] = a.eps[cidx(a, jj, ey, ll)];
+    }
+    // transverse face fields: Bx is filled in y/z ghosts, By in x/z, Bz in x/y
+    if (gy || gz) a.Bin[0][c] = a.Bin[0][cidx(a, j, kk, ll)] * (gsign(a, 1, sy, 6) * gsign(a, 2, sz, 6));
+    if (gx || gz) a.Bin[1][c] = a.Bin[1][cidx(a, jj, k, ll)] * (gsign(a, 0, sx, 7) * gsign(a, 2, sz, 7));
+    if (gx || gy) a.Bin[2][c] = a.Bin[2][cidx(a, jj, kk, l)] * (gsign(a, 0, sx, 8) * gsign(a, 1, sy, 8));
+    // torus no-inflow clamps (model/torus-cyn/src/custom.f90:156-161, 194-199)
+    if (a.model == 1) {
+        if (sx == 0) v[0] = fmin2(v[0], 0.0);
+        if (sx == 1) v[0] = fmax2(v[0], 0.0);
+        if (sz == 0) v[2] = fmin2(v[2], 0.0);
+        if (sz == 1) v[2] = fmax2(v[2], 0.0);
+    }
+    a.P[0][c] = rho;
+    a.P[1][c] = v[0];
+    a.P[2][c] = v[1];
+    a.P[3][c] = v[2];
+    double p, cs;
+    pressure(a, c, rho, a.eps[c], p, cs);
+    a.P[4][c] = p;
+    a.cs[c] = cs;
+}
+
+}  // namespace cumc3d

@@ -1,0 +1,202 @@
+"""Host-side binding of libcumc3d_hip.so (include/cumc3d_hip.h).
+
+``Solver`` mirrors the reference's driver sequence (solver/mhd/main.f90:339-422):
+``initial_model`` (INITIAL_MODEL + POPULATE_DEVICE), ``initial_update``
+(INITIAL_UPDATE), ``finddt`` (FINDDT), ``rungekutta`` (RUNGEKUTTA), ``run`` (the
+time loop), ``find_divb`` (FIND_DIVB) and ``state`` (the host update done by
+print_hydroprofile).  Everything runs in the HIP library; this module only
+marshals arguments.  It raises when the library is not built: there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .config import MODELS, PRECISION, RECON, RIEMANN, Config
+from .grid import Grid
+from .ic import PRIM_ORDER
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcumc3d_hip.so")
+
+PD = C.POINTER(C.c_double)
+
+# symbols declared in include/cumc3d_hip.h
+EXPORTS = ("cumc3d_init", "cumc3d_upload", "cumc3d_initial_update", "cumc3d_finddt",
+           "cumc3d_rk3_step", "cumc3d_run", "cumc3d_download", "cumc3d_divb",
+           "cumc3d_finalize", "cumc3d_error_string", "cumc3d_last_error",
+           "cumc3d_set_timing", "cumc3d_get_timing")
+
+BN_MODES = {"zero": 0, "field": 1, "carry": 2}
+
+
+class Desc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("n_dim", C.c_int),
+                ("coordinate_flag", C.c_int), ("boundary_flag", C.c_int * 6),
+                ("riemann", C.c_int), ("recon", C.c_int), ("model", C.c_int),
+                ("rt_gravity", C.c_int), ("hlld_bn_mode", C.c_int), ("precision", C.c_int),
+                ("cfl", C.c_double), ("ggas", C.c_double), ("r_sh", C.c_double),
+                ("xF", PD), ("yF", PD), ("zF", PD)]
+
+
+class CumcError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libcumc3d_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise CumcError(f"{path} is missing: build it with `python -m cumc3d_amd.build`")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.cumc3d_init.argtypes = [C.POINTER(Desc), C.POINTER(vp)]
+    L.cumc3d_upload.argtypes = [vp, PD, PD, PD]
+    L.cumc3d_initial_update.argtypes = [vp]
+    L.cumc3d_finddt.argtypes = [vp, PD]
+    L.cumc3d_rk3_step.argtypes = [vp, C.c_double]
+    L.cumc3d_run.argtypes = [vp, C.c_int, C.c_double, C.c_int, PD, PD, C.POINTER(C.c_int)]
+    L.cumc3d_download.argtypes = [vp, PD, PD, PD, PD]
+    L.cumc3d_divb.argtypes = [vp, PD]
+    L.cumc3d_finalize.argtypes = [vp]
+    L.cumc3d_error_string.argtypes = [C.c_int]
+    L.cumc3d_error_string.restype = C.c_char_p
+    L.cumc3d_last_error.restype = C.c_char_p
+    L.cumc3d_set_timing.argtypes = [vp, C.c_int]
+    L.cumc3d_get_timing.argtypes = [vp, PD, C.c_int]
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(PD)
+
+
+def _check(rc: int):
+    if rc != 0:
+        L = load_library()
+        raise CumcError(f"{L.cumc3d_error_string(rc).decode()}: {L.cumc3d_last_error().decode()}")
+
+
+class Solver:
+    """One device context for a configuration."""
+
+    def __init__(self, cfg: Config, hlld_bn_mode: str = "carry", grid: Grid | None = None):
+        self.cfg = cfg
+        self.grid = grid or Grid(cfg)
+        L = load_library()
+        self._faces = [np.ascontiguousarray(a.faces) for a in (self.grid.x, self.grid.y, self.grid.z)]
+        d = Desc()
+        d.nx, d.ny, d.nz, d.n_dim = cfg.nx, cfg.ny, cfg.nz, cfg.n_dim
+        d.coordinate_flag = cfg.coordinate_flag
+        for i, b in enumerate(cfg.boundary_flag):
+            d.boundary_flag[i] = b
+        d.riemann = RIEMANN[cfg.riemann]
+        d.recon = RECON[cfg.recon]
+        d.model = MODELS[cfg.model]
+        d.rt_gravity = int(cfg.rt_gravity)
+        d.hlld_bn_mode = BN_MODES[hlld_bn_mode]
+        d.precision = PRECISION[cfg.precision]
+        d.cfl, d.ggas, d.r_sh = cfg.cfl, cfg.ggas, cfg.r_sh
+        d.xF, d.yF, d.zF = (_p(f) for f in self._faces)
+        h = C.c_void_p()
+        _check(L.cumc3d_init(C.byref(d), C.byref(h)))
+        self._h = h
+        self.t = 0.0
+        self.nstep = 0
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load_library().cumc3d_finalize(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- reference driver sequence ---------------------------------------
+    def initial_model(self, state: dict):
+        """Upload GET_MODEL output (prim, epsilon, floors) to the device."""
+        cfg = self.cfg
+        prim = np.ascontiguousarray(np.stack([state[n] for n in PRIM_ORDER], axis=-1), dtype=np.float64)
+        if prim.shape != cfg.shape + (8,):
+            raise ValueError(f"state arrays must have shape {cfg.shape}")
+        eps = np.ascontiguousarray(state["eps"], dtype=np.float64)
+        floors = None
+        if "rho_floor" in state:
+            floors = np.ascontiguousarray(np.stack([state["rho_floor"], state["p_floor"], state["eps_floor"]]),
+                                          dtype=np.float64)
+        _check(load_library().cumc3d_upload(self._h, _p(prim), _p(eps), _p(floors)))
+        self.t = 0.0
+        self.nstep = 0
+
+    def initial_update(self):
+        _check(load_library().cumc3d_initial_update(self._h))
+
+    def finddt(self) -> float:
+        dt = C.c_double()
+        _check(load_library().cumc3d_finddt(self._h, C.byref(dt)))
+        return dt.value
+
+    def rungekutta(self, dt: float):
+        _check(load_library().cumc3d_rk3_step(self._h, dt))
+        self.t += dt
+        self.nstep += 1
+
+    def run(self, max_steps: int, total_time: float | None = None, sync_each_step: bool = True) -> np.ndarray:
+        """main.f90's loop on the device; returns the dt of every step taken."""
+        T = self.cfg.total_time if total_time is None else total_time
+        t = C.c_double(self.t)
+        dts = np.zeros(max(max_steps, 1))
+        n = C.c_int()
+        _check(load_library().cumc3d_run(self._h, max_steps, T, int(sync_each_step), C.byref(t), _p(dts),
+                                         C.byref(n)))
+        self.t = t.value
+        self.nstep += n.value
+        return dts[:n.value]
+
+    def find_divb(self) -> float:
+        m = C.c_double()
+        _check(load_library().cumc3d_divb(self._h, C.byref(m)))
+        return m.value
+
+    def state(self) -> dict:
+        """Host copy of prim, epsilon, bcell and cs (ghosted box arrays)."""
+        cfg = self.cfg
+        prim = np.empty(cfg.shape + (8,))
+        bcell = np.empty(cfg.shape + (3,))
+        eps = np.empty(cfg.shape)
+        cs = np.empty(cfg.shape)
+        _check(load_library().cumc3d_download(self._h, _p(prim), _p(eps), _p(bcell), _p(cs)))
+        out = {n: np.ascontiguousarray(prim[..., i]) for i, n in enumerate(PRIM_ORDER)}
+        out.update(eps=eps, cs=cs, bcx=np.ascontiguousarray(bcell[..., 0]),
+                   bcy=np.ascontiguousarray(bcell[..., 1]), bcz=np.ascontiguousarray(bcell[..., 2]))
+        return out
+
+    # -- measurement -------------------------------------------------------
+    def set_timing(self, on: bool):
+        _check(load_library().cumc3d_set_timing(self._h, int(on)))
+
+    def timing(self) -> dict:
+        ms = np.zeros(7)
+        _check(load_library().cumc3d_get_timing(self._h, _p(ms), 7))
+        keys = ("flux_x", "flux_y", "flux_z", "edge", "update", "ghost", "step")
+        return dict(zip(keys, ms.tolist()))
+
+
+def make_solver(cfg: Config, state: dict, **kw) -> Solver:
+    s = Solver(cfg, **kw)
+    s.initial_model(state)
+    s.initial_update()
+    return s

@@ -1,0 +1,119 @@
+/*
+ * cumc3d_hip.h - C-ABI of the MI355X implementation of CUMC3D's solver/mhd
+ * update step (libcumc3d_hip.so).
+ *
+ * The reference runs the step through argument-less Fortran subroutines that
+ * share module globals (solver/mhd/definition.f90:8-160) and keeps device
+ * copies alive in an OpenACC data region (solver/mhd/openacc.f90:5-47).  This
+ * library owns the device state instead; a context handle replaces the module
+ * globals.  Plain pointers and sizes only; every entry point returns 0 on
+ * success or a CUMC3D_E* code (cumc3d_error_string() gives the message).
+ *
+ * Host arrays use the reference's layout: prim is (imin:imax, -2:nx+3,
+ * -2:ny+3, -2:nz+3) in Fortran order, i.e. the 8 variables
+ * (rho, vx, vy, vz, p, Bx, By, Bz) of one cell are contiguous and x runs
+ * fastest over cells; scalar fields (epsilon, cs, floors) are
+ * (-2:nx+3, -2:ny+3, -2:nz+3); bcell is (ibx:ibz, ...) like prim.
+ * Face fields follow the reference's staggering: prim(ibx,j,k,l) is Bx on
+ * face j+1/2 (solver/mhd/fromrvetou.f90:54-56).
+ */
+#ifndef CUMC3D_HIP_H
+#define CUMC3D_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CUMC3D_OK = 0,
+    CUMC3D_EINVAL = 1,      /* invalid descriptor or argument */
+    CUMC3D_EHIP = 2,        /* HIP runtime error */
+    CUMC3D_ENOMEM = 3,      /* device allocation failed */
+    CUMC3D_ESTATE = 4,      /* call out of order (e.g. step before upload) */
+    CUMC3D_ENONFINITE = 5   /* dt not finite or not positive: the state holds NaN/negative energy */
+};
+
+/* Riemann solvers: parameter.h:91-101 (LF_flag, HLL_flag, HLLC_flag, HLLD_flag) */
+enum { CUMC3D_LF = 0, CUMC3D_HLL = 1, CUMC3D_HLLC = 2, CUMC3D_HLLD = 3 };
+/* Reconstructions: parameter.h:103-115 (tvdmm, tvdvl, tvdmc, ppmc, weno flags) */
+enum { CUMC3D_TVDMM = 0, CUMC3D_TVDVL = 1, CUMC3D_TVDMC = 2, CUMC3D_PPM = 3, CUMC3D_WENO = 4 };
+/* In-step model hooks: model/riemann, model/torus-cyn */
+enum { CUMC3D_MODEL_RIEMANN = 0, CUMC3D_MODEL_TORUS = 1 };
+/* Arithmetic of the spatial operator */
+enum { CUMC3D_FP64 = 0, CUMC3D_FP32 = 1 };
+/* Value of HLLD's unassigned usstar(ibn) (rsolver_module.f90:704-741):
+ * CUMC3D_BN_CARRY reproduces the serial reference, CUMC3D_BN_FIELD uses the
+ * normal field (Miyoshi & Kusano 2005), CUMC3D_BN_ZERO uses 0. */
+enum { CUMC3D_BN_ZERO = 0, CUMC3D_BN_FIELD = 1, CUMC3D_BN_CARRY = 2 };
+
+typedef struct cumc3d_desc {
+    int nx, ny, nz;             /* parameter.h:70-72 */
+    int n_dim;                  /* parameter.h:37 */
+    int coordinate_flag;        /* parameter.h:34: 0 Cartesian, 1 cylindrical (r,phi,z) */
+    int boundary_flag[6];       /* parameter.h:52: x-in,x-out,y-in,y-out,z-in,z-out; 0..4 */
+    int riemann;                /* CUMC3D_LF .. CUMC3D_HLLD */
+    int recon;                  /* CUMC3D_TVDMM .. CUMC3D_WENO */
+    int model;                  /* CUMC3D_MODEL_* */
+    int rt_gravity;             /* model/riemann/src/custom.f90:143: 0.1 g downward in y */
+    int hlld_bn_mode;           /* CUMC3D_BN_* */
+    int precision;              /* CUMC3D_FP64 or CUMC3D_FP32 */
+    double cfl;                 /* parameter.h:81 */
+    double ggas;                /* adiabatic index (DEFINITION ggas) */
+    double r_sh;                /* model/torus-cyn/src/param.h:19 */
+    const double *xF, *yF, *zF; /* face coordinates xF(-3:nx+3) etc. (n+7 values; GETGRID output) */
+} cumc3d_desc;
+
+typedef struct cumc3d_ctx cumc3d_ctx;
+
+/* SETUP_EQNS + BUILD_HYDRO + BUILD_RIEMANN + BUILD_MHD + GETGRID metrics and
+ * the reconstruction weights (solver/mhd/initial.f90:7-61); allocates the
+ * device state.  The xF/yF/zF arrays are copied. */
+int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out);
+
+/* POPULATE_DEVICE (openacc.f90:5-24): copy the initial model (GET_MODEL
+ * output: prim and epsilon, plus rho/p/eps floor arrays for the torus
+ * model, NULL otherwise) to the device. */
+int cumc3d_upload(cumc3d_ctx *ctx, const double *prim, const double *eps, const double *floors3);
+
+/* INITIAL_UPDATE (initial.f90:86-116): floors, cons, BOUNDARY, UPDATE(0). */
+int cumc3d_initial_update(cumc3d_ctx *ctx);
+
+/* FINDDT (rungekutta.f90:221-332): CFL time step, returned to the host. */
+int cumc3d_finddt(cumc3d_ctx *ctx, double *dt);
+
+/* RUNGEKUTTA (rungekutta.f90:12-208): one SSP-RK3 step with the given dt. */
+int cumc3d_rk3_step(cumc3d_ctx *ctx, double dt);
+
+/* The time loop of main.f90:381-399 on the device: up to max_steps steps of
+ * FINDDT, dt clip to total_time, RUNGEKUTTA, global_time += dt, stopping once
+ * global_time >= total_time.  With sync_each_step = 0 the steps are queued
+ * without host round trips (steps past total_time become no-ops).  *t is the
+ * global time on entry and on return; dts (may be NULL) receives each dt. */
+int cumc3d_run(cumc3d_ctx *ctx, int max_steps, double total_time, int sync_each_step, double *t,
+               double *dts, int *steps_done);
+
+/* Copy the device state back (replaces !$ACC UPDATE HOST, dataio.f90:26).
+ * Any pointer may be NULL.  prim and bcell use the AoS layout above. */
+int cumc3d_download(cumc3d_ctx *ctx, double *prim, double *eps, double *bcell, double *cs);
+
+/* FIND_DIVB (mhd_module.f90:110-201): max |div B| over the interior. */
+int cumc3d_divb(cumc3d_ctx *ctx, double *maxdivb);
+
+/* CLEAR_DEVICE (openacc.f90:30-47): release the device state. */
+int cumc3d_finalize(cumc3d_ctx *ctx);
+
+const char *cumc3d_error_string(int code);
+/* Message of the last error on this thread (empty if none). */
+const char *cumc3d_last_error(void);
+
+/* Measurement helpers (no reference counterpart). */
+/* Mean device time in ms per launch of each kernel class over the last
+ * cumc3d_run call with timing enabled: [0] x-flux, [1] y-flux, [2] z-flux,
+ * [3] edge EMF, [4] update, [5] ghost fill, [6] whole step. */
+int cumc3d_set_timing(cumc3d_ctx *ctx, int enable);
+int cumc3d_get_timing(cumc3d_ctx *ctx, double *ms_per_launch, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -28,7 +28,9 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
-from cases import CASES  # noqa: E402
+from cases import BENCH_CASES, CASES  # noqa: E402
+
+ALL_CASES = {**CASES, **BENCH_CASES}
 
 REF = os.environ.get("CUMC3D_REFERENCE", "/root/reference")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
@@ -114,7 +116,7 @@ def run(cmd, cwd):
 
 
 def build_case(name: str) -> str:
-    case = CASES[name]
+    case = ALL_CASES[name]
     cfg = case.cfg
     work = os.path.join(SCRATCH, name)
     if os.path.isdir(work):
@@ -161,7 +163,7 @@ def available() -> bool:
 
 
 def main(argv):
-    names = argv or list(CASES)
+    names = argv or list(ALL_CASES)
     if not available():
         print("reference sources or flang not found; oracle/_ref not built")
         return 0

@@ -71,3 +71,12 @@ CASES = {c.name: c for c in [
                                    boundary_flag=(3, 1, 0, 0, 4, 1)),
          "torus", 8),
 ]}
+
+
+# CPU-baseline samples for bench.py: the reference solver on the benchmark
+# physics at a grid that one host run covers in about 10-30 s.
+BENCH_CASES = {c.name: c for c in [
+    Case("bench_blast128", Config(nx=128, ny=128, nz=128, n_dim=3, cfl=0.4, ggas=5.0 / 3.0,
+                                  total_time=0.02, riemann="hlld", recon="weno",
+                                  boundary_flag=(0,) * 6), "blast", 3),
+]}

@@ -47,8 +47,9 @@ struct oracle {
 
 /* Fortran intrinsics */
 static inline double fsign(double a, double b) { return copysign(fabs(a), b); }
-static inline double fmin2(double a, double b) { return (b < a) ? b : a; }
-static inline double fmax2(double a, double b) { return (b > a) ? b : a; }
+/* flang evaluates MIN(a,b) as a < b ? a : b (a NaN or a tie gives b) */
+static inline double fmin2(double a, double b) { return (a < b) ? a : b; }
+static inline double fmax2(double a, double b) { return (a > b) ? a : b; }
 
 /* ------------------------------------------------------------------------ */
 /* allocation                                                               */

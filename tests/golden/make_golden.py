@@ -1,0 +1,59 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures: outputs of the reference solver on the parity cases.
+
+Runs oracle/_ref/<case>/cumc3d_ref (the reference Fortran compiled from its own
+sources by oracle/build_ref.py) single-threaded with zero-filled allocations
+(oracle/refio.run_ref) on each case of oracle/cases.py and stores, per case,
+the dt sequence and the final state on cells 0..n+1 of every axis:
+rho, vx, vy, vz, p, bx, by, bz (face), eps, bcx, bcy, bcz.
+
+usage: python tests/golden/make_golden.py [case ...]
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from cumc3d_amd import ic  # noqa: E402
+from oracle.cases import CASES  # noqa: E402
+from oracle.refio import have_ref, run_ref  # noqa: E402
+
+FIELDS = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz", "eps", "bcx", "bcy", "bcz")
+
+
+def window(cfg):
+    """cells 0..n+1 on the active axes, cell 1 on inactive ones"""
+    sl = []
+    for n, active in ((cfg.nz, cfg.n_dim > 2), (cfg.ny, cfg.n_dim > 1), (cfg.nx, True)):
+        sl.append(slice(2, n + 4) if active else slice(3, 4))
+    return tuple(sl)
+
+
+def main(names):
+    for name in names or list(CASES):
+        case = CASES[name]
+        if not have_ref(name):
+            print("skip", name, "(reference binary not built)")
+            continue
+        st = ic.GENERATORS[case.ic](case.cfg)
+        with tempfile.TemporaryDirectory() as d:
+            p = os.path.join(d, "ic.bin")
+            ic.write_ic(p, case.cfg, st)
+            ref = run_ref(name, p, case.steps, threads=1)
+        w = window(case.cfg)
+        out = {f: ref[f][w] for f in FIELDS}
+        out["dts"] = ref["dts"]
+        out["t"] = np.array(ref["t"])
+        out["maxdivb"] = np.array(ref["maxdivb"])
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **out)
+        print(f"{name}: {ref['nstep']} steps -> {path} ({os.path.getsize(path) // 1024} KB)")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -1,0 +1,288 @@
+"""The oracle (C restatement, oracle/mhd_oracle.c) against the reference's own
+outputs and against closed forms.  CPU only."""
+import math
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import interior_window, load_golden
+from cumc3d_amd import ic
+from cumc3d_amd.config import Config
+from oracle import oracle as O
+from oracle.cases import CASES
+from oracle.refio import have_ref, run_ref
+
+FIELDS = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz", "eps", "bcx", "bcy", "bcz")
+FAST = [n for n in CASES if n != "c1_briowu"]
+
+
+def run_oracle(name):
+    case = CASES[name]
+    o = O.Oracle(case.cfg, ic.GENERATORS[case.ic](case.cfg))
+    dts = o.run(case.steps)
+    return o, dts
+
+
+@pytest.mark.parametrize("name", FAST + ["c1_briowu"])
+def test_oracle_matches_reference_fixture(name):
+    """Bit-exact against the reference Fortran's output for the same input."""
+    gold = load_golden(name)
+    o, dts = run_oracle(name)
+    assert np.array_equal(dts, gold["dts"]), "dt sequence differs"
+    w = interior_window(CASES[name].cfg)
+    for f in FIELDS:
+        got = o.get(f)[w]
+        assert np.array_equal(got, gold[f]), f"{f}: max diff {np.nanmax(np.abs(got - gold[f]))}"
+    assert o.divb() == float(gold["maxdivb"])
+
+
+@pytest.mark.parametrize("name", ["ot_hlld_ppm", "torus_hlld_ppm", "ot2d_hlld_ppm"])
+def test_oracle_matches_reference_binary(name):
+    """Re-run the reference binary (when built here) so the recipe itself is checked."""
+    if not have_ref(name):
+        pytest.skip("oracle/_ref not built")
+    case = CASES[name]
+    st = ic.GENERATORS[case.ic](case.cfg)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "ic.bin")
+        ic.write_ic(p, case.cfg, st)
+        ref = run_ref(name, p, case.steps)
+    o, dts = run_oracle(name)
+    assert np.array_equal(dts, ref["dts"])
+    w = interior_window(case.cfg)
+    for f in FIELDS:
+        assert np.array_equal(o.get(f)[w], ref[f][w]), f
+
+
+def test_hlld_carry_semantics_matter():
+    """The unassigned usstar(ibn) of HLLDNM changes the solution: the carried value
+    (serial reference) and the Miyoshi-Kusano value Bn give different results, and
+    only the carried one matches the reference fixture."""
+    name = "ot_hlld_ppm"
+    case = CASES[name]
+    gold = load_golden(name)
+    w = interior_window(case.cfg)
+    st = ic.GENERATORS[case.ic](case.cfg)
+    o1 = O.Oracle(case.cfg, st, hlld_bn_mode=1)
+    o1.run(case.steps)
+    assert not np.array_equal(o1.get("rho")[w], gold["rho"])
+
+
+# ---------------------------------------------------------------- closed forms
+
+W_TVD = np.array([2.0, 2.0])     # uniform grid: cf = cb = 2 (tvd_module.f90:42-53)
+
+
+def uniform_w(kind):
+    return O.ppm_weight([1, 1, 1, 1, 1]) if kind == "ppm" else O.weno_weight([1, 1, 1, 1, 1])
+
+
+@pytest.mark.parametrize("lim", [0, 1, 2])
+def test_tvd_exact_on_linear_data(lim):
+    for slope in (-1.7, 0.3, 2.5):
+        v = [3.0 - slope, 3.0, 3.0 + slope]
+        vm, vp = O.tvd(lim, W_TVD, *v)
+        assert vm == pytest.approx(3.0 - 0.5 * slope, rel=1e-15)
+        assert vp == pytest.approx(3.0 + 0.5 * slope, rel=1e-15)
+
+
+@pytest.mark.parametrize("lim", [0, 1, 2])
+def test_tvd_flat_at_extrema(lim):
+    # a local extremum gives a zero slope for every limiter
+    vm, vp = O.tvd(lim, W_TVD, 1.0, 2.0, 1.5)
+    assert vm == vp == 2.0
+
+
+def test_ppm_weights_uniform_grid():
+    """On a uniform grid PPM_INTERPOLANT reduces to CW84 eq. 1.6:
+    v_{j+1/2} = v_j + (v_{j+1}-v_j)/2 - (dm_{j+1} - dm_j)/6, slopes 1/2 (v_{j+1}-v_{j-1})."""
+    w = uniform_w("ppm")
+    assert w[4] == 0.5 and w[0] == 0.5
+    assert w[5] == 0.0 and w[1] == 0.0
+    assert w[6] == pytest.approx(-1.0 / 6.0) and w[7] == pytest.approx(1.0 / 6.0)
+    assert w[2] == pytest.approx(-1.0 / 6.0) and w[3] == pytest.approx(1.0 / 6.0)
+    for i in range(8, 14):
+        assert w[i] == pytest.approx(0.5)
+
+
+@pytest.mark.parametrize("kind", ["ppm", "weno"])
+def test_high_order_exact_on_linear_data(kind):
+    w = uniform_w(kind)
+    f = O.ppm if kind == "ppm" else O.weno
+    v = [1.0, 1.5, 2.0, 2.5, 3.0]
+    vm, vp = f(w, v)
+    assert vm == pytest.approx(1.75, rel=1e-14)
+    assert vp == pytest.approx(2.25, rel=1e-14)
+
+
+def test_weno_fifth_order_on_smooth_data():
+    """WENO-Z on point samples of a smooth function: the face error drops by ~2^5 per
+    halving of h (away from critical points)."""
+    w = uniform_w("weno")
+    errs = []
+    for h in (0.1, 0.05, 0.025):
+        x0 = 0.3
+        xs = x0 + h * np.arange(-2, 3)
+        # cell averages of sin on [x-h/2, x+h/2]
+        v = [(math.cos(x - h / 2) - math.cos(x + h / 2)) / h for x in xs]
+        vm, vp = O.weno(w, v)
+        errs.append(abs(vp - math.sin(x0 + h / 2)))
+    rate = math.log2(errs[0] / errs[1]), math.log2(errs[1] / errs[2])
+    assert min(rate) > 4.5
+
+
+@pytest.mark.parametrize("kind", ["ppm", "weno", "tvd2", "tvd0"])
+def test_reconstruction_mirror_symmetry(kind):
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        v = rng.normal(size=5)
+        if kind.startswith("tvd"):
+            lim = int(kind[-1])
+            a = O.tvd(lim, W_TVD, *v[1:4])
+            b = O.tvd(lim, W_TVD, *v[1:4][::-1])
+        else:
+            f = O.ppm if kind == "ppm" else O.weno
+            a = f(uniform_w(kind), v)
+            b = f(uniform_w(kind), v[::-1])
+        assert a[0] == pytest.approx(b[1], rel=1e-12, abs=1e-14)
+        assert a[1] == pytest.approx(b[0], rel=1e-12, abs=1e-14)
+
+
+def test_ppm_monotone():
+    rng = np.random.default_rng(5)
+    w = uniform_w("ppm")
+    for _ in range(200):
+        v = np.cumsum(np.abs(rng.normal(size=5)))   # monotone increasing data
+        vm, vp = O.ppm(w, v)
+        assert v[1] - 1e-14 <= vm <= v[3] + 1e-14
+        assert v[1] - 1e-14 <= vp <= v[3] + 1e-14
+        assert vm <= v[2] + 1e-14 <= vp + 2e-14
+
+
+def physical_flux(q, g, d):
+    """Ideal-MHD flux along axis d (0,1,2) of primitive state q."""
+    rho, vx, vy, vz, p, bx, by, bz = q
+    v = np.array([vx, vy, vz]); b = np.array([bx, by, bz])
+    e = p / (g - 1) + 0.5 * rho * v @ v + 0.5 * b @ b
+    pt = p + 0.5 * b @ b
+    f = np.zeros(8)
+    f[0] = rho * v[d]
+    f[1:4] = rho * v[d] * v - b[d] * b
+    f[1 + d] += pt
+    f[4] = (e + pt) * v[d] - b[d] * (v @ b)
+    f[5:8] = v[d] * b - v * b[d]
+    return f
+
+
+@pytest.mark.parametrize("solver", ["lf", "hll", "hllc", "hlld"])
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_riemann_consistency(solver, d):
+    """F(U,U) = F(U) for every solver and direction (independent restatement of the
+    MHD flux, Tutorial eqs. for ideal MHD)."""
+    rng = np.random.default_rng(11 + d)
+    g = 5.0 / 3.0
+    for _ in range(20):
+        q = np.array([rng.uniform(0.5, 2), *rng.normal(size=3), rng.uniform(0.2, 3), *rng.normal(size=3)])
+        f = O.riemann_point(solver, d, g, q, q, hlld_bn_mode=1)
+        ref = physical_flux(q, g, d - 1)
+        n = 5 + (d - 1)
+        keep = [i for i in range(8) if i != n]
+        np.testing.assert_allclose(f[keep], ref[keep], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("solver", ["hll", "hllc", "hlld"])
+def test_riemann_supersonic_upwinding(solver):
+    """s_L >= 0: the flux is exactly the left physical flux (rsolver_module.f90:609-612)."""
+    g = 5.0 / 3.0
+    qL = np.array([1.0, 20.0, 0.1, -0.2, 1.0, 0.5, 0.3, 0.1])
+    qR = np.array([0.8, 19.0, 0.0, 0.1, 0.9, 0.5, 0.2, 0.2])
+    f = O.riemann_point(solver, 1, g, qL, qR)
+    np.testing.assert_allclose(f[:5], physical_flux(qL, g, 0)[:5], rtol=1e-14)
+
+
+@pytest.mark.parametrize("solver", ["hllc", "hlld"])
+def test_riemann_resolves_stationary_contact(solver):
+    """A density jump at rest with equal total pressure and normal-only field carries
+    no mass flux through HLLC/HLLD (they resolve the contact); HLL does not."""
+    g = 5.0 / 3.0
+    qL = np.array([1.0, 0, 0, 0, 1.0, 0.7, 0, 0])
+    qR = np.array([0.2, 0, 0, 0, 1.0, 0.7, 0, 0])
+    f = O.riemann_point(solver, 1, g, qL, qR)
+    assert abs(f[0]) < 1e-14
+    assert abs(O.riemann_point("hll", 1, g, qL, qR)[0]) > 1e-3
+
+
+@pytest.mark.parametrize("solver", ["lf", "hll", "hllc", "hlld"])
+def test_riemann_mirror_symmetry(solver):
+    """Reflection x -> -x (vx -> -vx, Bx -> -Bx, states swapped) leaves ideal MHD
+    invariant, so the x-flux transforms with signs (-,+,-,-,-,.,-,-)."""
+    rng = np.random.default_rng(7)
+    g = 1.4
+    M = np.array([1, -1, 1, 1, 1, -1, 1, 1.0])
+    S = np.array([-1, 1, -1, -1, -1, 1, -1, -1.0])
+    keep = [0, 1, 2, 3, 4, 6, 7]
+    for _ in range(20):
+        bn = rng.normal()
+        qL = np.array([rng.uniform(0.5, 2), *rng.normal(size=3), rng.uniform(0.2, 3), bn, *rng.normal(size=2)])
+        qR = np.array([rng.uniform(0.5, 2), *rng.normal(size=3), rng.uniform(0.2, 3), bn, *rng.normal(size=2)])
+        f = O.riemann_point(solver, 1, g, qL, qR, hlld_bn_mode=1)
+        fm = O.riemann_point(solver, 1, g, qR * M, qL * M, hlld_bn_mode=1)
+        np.testing.assert_allclose(fm[keep], (S * f)[keep], rtol=1e-10, atol=1e-12)
+
+
+def test_uniform_state_is_stationary():
+    """A uniform state with uniform field is an exact steady solution of the whole
+    step (all fluxes equal, CT EMF uniform), for every scheme."""
+    for recon in ("tvdmc", "ppm", "weno"):
+        for solver in ("lf", "hllc", "hlld"):
+            cfg = Config(nx=8, ny=8, nz=8, n_dim=3, recon=recon, riemann=solver, boundary_flag=(0,) * 6)
+            st = ic.empty_state(cfg)
+            st["rho"][:] = 1.3
+            st["vx"][:] = 0.2; st["vy"][:] = -0.1; st["vz"][:] = 0.05
+            st["p"][:] = 0.7
+            st["bx"][:] = 0.4; st["by"][:] = -0.3; st["bz"][:] = 0.2
+            st["eps"][:] = st["p"] / st["rho"] / (cfg.ggas - 1)
+            o = O.Oracle(cfg, st)
+            o.run(3)
+            for f in ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz"):
+                a = o.get(f)[3:-3, 3:-3, 3:-3]
+                np.testing.assert_allclose(a, st[f][3:-3, 3:-3, 3:-3], rtol=1e-13, atol=1e-15)
+
+
+def test_mass_and_energy_conservation_periodic_hydro():
+    """Periodic box without field: total mass and energy change only by rounding
+    (the flux divergence telescopes; the cons->prim->cons round trip keeps rho).
+    With a field the lowest face is never refreshed (fromrvetou.f90:201-207), so
+    the x-flux at interface 0 differs from its periodic image at interface nx and
+    the reference algorithm itself loses exact conservation."""
+    case = CASES["ot_hlld_weno"]
+    st = ic.GENERATORS[case.ic](case.cfg)
+    for b in ("bx", "by", "bz"):
+        st[b][:] = 0.0
+    o = O.Oracle(case.cfg, st)
+    I = (slice(3, -3),) * 3
+    m0 = st["rho"][I].sum()
+    o.run(10)
+    m1 = o.get("rho")[I].sum()
+    assert abs(m1 - m0) / m0 < 1e-13
+
+
+def test_divb_zero_away_from_frozen_faces():
+    """CT keeps the discrete div B of every cell whose faces all evolve at round-off.
+    Cells j=1 (k=1, l=1) use the lowest face, which the reference never refreshes
+    (FROMUTORVE copies faces 1..n, fromrvetou.f90:201-207), so div B there is O(1)."""
+    case = CASES["ot_hlld_ppm"]
+    cfg = case.cfg
+    st = ic.GENERATORS[case.ic](cfg)
+    o = O.Oracle(cfg, st)
+    o.run(case.steps)
+    bx, by, bz = o.get("bx"), o.get("by"), o.get("bz")
+    h = 1.0 / cfg.nx
+    sl = slice(4, cfg.nx + 3)     # cells 2..n
+    div = ((bx[sl, sl, sl] - bx[sl, sl, 3:cfg.nx + 2]) + (by[sl, sl, sl] - by[sl, 3:cfg.nx + 2, sl])
+           + (bz[sl, sl, sl] - bz[3:cfg.nx + 2, sl, sl])) / h
+    bmag = np.abs(bx).max()
+    assert np.abs(div).max() * h / bmag < 1e-12
+    assert o.divb() > 1e-3    # the frozen lower faces break it at cells 1
