@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -37,8 +38,9 @@ struct cumc3d_ctx {
     DevScalars *S = nullptr;
     unsigned long long *blockmax = nullptr;
     long nblockmax = 0;
-    double *P[5], *eps, *cs, *Bc[3], *B[2][3], *Uo[8], *F[3][7], *E[3], *floors[3];
+    double *P[5], *eps, *cs, *Bc[3], *B[2][3], *Uo[8], *F[3][7], *E[3], *Ec[3], *floors[3];
     int cur = 0;                  // B[cur] holds the current face field
+    long march_threads = 1L << 19;  // target thread count of the y/z sweeps
     int bfac[6][12];
     Axis ax[3];
     bool uploaded = false, initialized = false, dt_pending = false;
@@ -185,6 +187,7 @@ static Args make_args(cumc3d_ctx *x) {
         a.Bin[d] = x->B[x->cur][d];
         a.Bout[d] = x->B[1 - x->cur][d];
         a.E[d] = x->E[d];
+        a.Ec[d] = x->Ec[d];
         for (int i = 0; i < 7; ++i) a.F[d][i] = x->F[d][i];
     }
     for (int i = 0; i < 8; ++i) a.Uo[i] = x->Uo[i];
@@ -274,9 +277,13 @@ static void weights(int recon, const std::vector<double> &dd, int n, std::vector
     }
 }
 
-static int nseg_for(int nx, int na, int nd) {
+// Segments per column of the y/z sweeps: each thread marches one segment of
+// interfaces along the sweep axis, reusing the reconstruction of the cell it
+// shares with the previous interface.  About march_threads threads in flight
+// (2^19 = 8k waves by default); CUMC3D_MARCH_THREADS overrides it at init.
+static int nseg_for(const cumc3d_ctx *x, int nx, int na, int nd) {
     long cols = (long)nx * na;
-    long want = (1L << 19) / (cols > 0 ? cols : 1);   // about 8k waves in flight
+    long want = x->march_threads / (cols > 0 ? cols : 1);
     if (want < 1) want = 1;
     if (want > nd + 1) want = nd + 1;
     return (int)want;
@@ -294,13 +301,13 @@ static void launch_sweep_rs(cumc3d_ctx *x, const Args &a, int D, hipStream_t st)
         if (S == 3 && a.bn_mode == 2)
             hipLaunchKernelGGL((k_carry<double, R, 0>), dim3(1), dim3(1024), 0, st, a, (int)grid);
     } else if (D == 1) {
-        int ns = nseg_for(x->nx, x->nz, x->ny);
+        int ns = nseg_for(x, x->nx, x->nz, x->ny);
         unsigned grid = nblk((long)x->nx * x->nz * ns);
         hipLaunchKernelGGL((k_flux_yz<double, R, S, 1>), dim3(grid), dim3(256), 0, st, a, ns);
         if (S == 3 && a.bn_mode == 2)
             hipLaunchKernelGGL((k_carry<double, R, 1>), dim3(1), dim3(1024), 0, st, a, (int)grid);
     } else {
-        int ns = nseg_for(x->nx, x->ny, x->nz);
+        int ns = nseg_for(x, x->nx, x->ny, x->nz);
         unsigned grid = nblk((long)x->nx * x->ny * ns);
         hipLaunchKernelGGL((k_flux_yz<double, R, S, 2>), dim3(grid), dim3(256), 0, st, a, ns);
         if (S == 3 && a.bn_mode == 2)
@@ -331,8 +338,8 @@ static void launch_sweep(cumc3d_ctx *x, const Args &a, int D, hipStream_t st) {
 static long blocks_needed(cumc3d_ctx *x) {
     long waves = ((long)(x->nx + 2) * x->ny * x->nz + 62) / 63 + 1;
     long bx = (waves * 64 + 255) / 256;
-    long by = nblk((long)x->nx * x->nz * nseg_for(x->nx, x->nz, x->ny));
-    long bz = nblk((long)x->nx * x->ny * nseg_for(x->nx, x->ny, x->nz));
+    long by = nblk((long)x->nx * x->nz * nseg_for(x, x->nx, x->nz, x->ny));
+    long bz = nblk((long)x->nx * x->ny * nseg_for(x, x->nx, x->ny, x->nz));
     long m = bx > by ? bx : by;
     return m > bz ? m : bz;
 }
@@ -356,6 +363,7 @@ static void run_stage(cumc3d_ctx *x, int stage) {
     tmark(x, 2);
     if (x->d.n_dim > 2) launch_sweep(x, a, 2, st);
     tmark(x, 3);
+    hipLaunchKernelGGL(k_emf_prep, dim3(nblk((long)(x->nx + 2) * (x->ny + 2) * (x->nz + 2))), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_edge, dim3(nblk((long)(x->nx + 1) * (x->ny + 1) * (x->nz + 1))), dim3(256), 0, st, a);
     tmark(x, 4);
     if (stage == 1) hipLaunchKernelGGL(k_update<1>, dim3(nblk(nn)), dim3(256), 0, st, a);
@@ -437,6 +445,8 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
                 return fail(CUMC3D_EINVAL, "periodic or mirror boundaries need at least 3 cells on that axis");
     (void)nmin;
 
+    if ((long)(d.nx + 6) * (d.ny + 6) * (d.nz + 6) * 8 >= (1L << 32))
+        return fail(CUMC3D_EINVAL, "grid too large: each field must stay below 4 GiB (about 800^3 cells)");
     cumc3d_ctx *x = new cumc3d_ctx();
     x->d = d;
     x->nx = d.nx; x->ny = d.ny; x->nz = d.nz;
@@ -481,7 +491,7 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         x->ax[a].w = x->met + off[a][3];
     }
     // per-cell fields
-    const int nfields = 5 + 2 + 3 + 6 + 8 + 21 + 3 + 3;
+    const int nfields = 5 + 2 + 3 + 6 + 8 + 21 + 3 + 3 + 3;
     size_t bytes = (size_t)nfields * x->G * sizeof(double);
     if (hipMalloc(&x->pool, bytes) != hipSuccess) {
         cleanup();
@@ -500,6 +510,7 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
     for (int dd = 0; dd < 3; ++dd)
         for (int i = 0; i < 7; ++i) x->F[dd][i] = take();
     for (int i = 0; i < 3; ++i) x->E[i] = take();
+    for (int i = 0; i < 3; ++i) x->Ec[i] = take();
     for (int i = 0; i < 3; ++i) x->floors[i] = take();
     if (hipMalloc(&x->S, sizeof(DevScalars)) != hipSuccess) { cleanup(); return fail(CUMC3D_ENOMEM, "scalars"); }
     DevScalars s0;
@@ -508,6 +519,10 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
     s0.active = 1;
     s0.dtmin_bits = (unsigned long long)0x7E37E43C8800759CULL;   // bits of 1e300
     if (hipMemcpy(x->S, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess) { cleanup(); return fail(CUMC3D_EHIP, "scalars"); }
+    if (const char *mt = getenv("CUMC3D_MARCH_THREADS")) {
+        long v = atol(mt);
+        if (v >= 1) x->march_threads = v;
+    }
     x->nblockmax = blocks_needed(x) + 16;
     if (hipMalloc(&x->blockmax, x->nblockmax * sizeof(unsigned long long)) != hipSuccess) {
         cleanup();

@@ -52,10 +52,19 @@ struct Args {
     double *Uo[8];          // u_old
     double *F[3][7];        // interface fluxes: 5 hydro + 2 face EMFs
     double *E[3];           // edge EMFs efield_x/y/z
+    double *Ec[3];          // cell EMFs ecell after the ghost fill, cells 0..n+1
     const double *rfl, *pfl, *efl;   // torus floors
     DevScalars *S;
     unsigned long long *blockmax;    // per-block max of double-star interface keys
 };
+
+// Load field[idx] as scalar base + 32-bit byte offset (the global_load saddr
+// form): one 32-bit register per address instead of a 64-bit pointer.  Every
+// field is below 4 GiB (checked in cumc3d_init).
+__device__ __forceinline__ double ldf(const double *__restrict__ base, long idx) {
+    unsigned off = (unsigned)idx << 3;
+    return *(const double *)((const char *)base + off);
+}
 
 __device__ __forceinline__ long cidx(const Args &a, int j, int k, int l) {
     return (long)(j + 2) + a.NX * ((long)(k + 2) + a.NY * (long)(l + 2));
@@ -113,12 +122,12 @@ __device__ __forceinline__ void load_stencil(const Args &a, long c, long st, T s
 #pragma unroll
     for (int v = 0; v < 7; ++v) {
         if constexpr (R >= 3) {
-            s[v][0] = (T)src[v][c - 2 * st];
-            s[v][4] = (T)src[v][c + 2 * st];
+            s[v][0] = (T)ldf(src[v], c - 2 * st);
+            s[v][4] = (T)ldf(src[v], c + 2 * st);
         }
-        s[v][1] = (T)src[v][c - st];
-        s[v][2] = (T)src[v][c];
-        s[v][3] = (T)src[v][c + st];
+        s[v][1] = (T)ldf(src[v], c - st);
+        s[v][2] = (T)ldf(src[v], c);
+        s[v][3] = (T)ldf(src[v], c + st);
     }
 }
 
@@ -243,25 +252,26 @@ __global__ __launch_bounds__(256) void k_flux_yz(Args a, int nseg) {
         int i0 = (int)(((long)s * (nd + 1)) / nseg);
         int i1 = (int)(((long)(s + 1) * (nd + 1)) / nseg);
         const long st = stride<D>(a);
-        Face<T> prev;
-        for (int i = i0; i <= i1; ++i) {
+        Face<T> prev, fm;
+        recon_at<T, R, D>(a, (D == 1) ? j : j, (D == 1) ? i0 : o2, (D == 1) ? o2 : i0, st, fm, prev);
+        const T carry = (T)a.S->carry[D];
+#pragma unroll 1
+        for (int i = i0 + 1; i <= i1; ++i) {
             int k = (D == 1) ? i : o2;
             int l = (D == 1) ? o2 : i;
-            Face<T> fm, fp;
+            Face<T> fp;
             recon_at<T, R, D>(a, j, k, l, st, fm, fp);
-            if (i > i0) {
-                int kk = (D == 1) ? i - 1 : k;
-                int ll = (D == 1) ? l : i - 1;
-                long c = cidx(a, j, kk, ll);
-                T bn = (T)a.Bin[D][c];
-                Q<T> qL = to_q<T, D>(prev, bn), qR = to_q<T, D>(fm, bn);
-                RiemannOut<T> o;
-                riemann<T, D, S>(qL, qR, prev.eps, fm.eps, prev.cs, fm.cs, a.bn_mode, (T)a.S->carry[D], o);
-                store_flux<T, D>(a, c, o);
-                if (o.ds) {
-                    unsigned long long kk2 = serial_key<D>(a, j, kk, ll);
-                    key = kk2 > key ? kk2 : key;
-                }
+            int kk = (D == 1) ? i - 1 : k;
+            int ll = (D == 1) ? l : i - 1;
+            long c = cidx(a, j, kk, ll);
+            T bn = (T)a.Bin[D][c];
+            Q<T> qL = to_q<T, D>(prev, bn), qR = to_q<T, D>(fm, bn);
+            RiemannOut<T> o;
+            riemann<T, D, S>(qL, qR, prev.eps, fm.eps, prev.cs, fm.cs, a.bn_mode, carry, o);
+            store_flux<T, D>(a, c, o);
+            if (o.ds) {
+                unsigned long long kk2 = serial_key<D>(a, j, kk, ll);
+                key = kk2 > key ? kk2 : key;
             }
             prev = fp;
         }
@@ -359,31 +369,62 @@ template <int E, int D> __device__ __forceinline__ double eface_at(const Args &a
     return (gsign(a, 0, sx, 9 + E) * gsign(a, 1, sy, 9 + E) * gsign(a, 2, sz, 9 + E)) * v;
 }
 
+// Cell and face EMFs on the cells 0..n+1 the edge stencil reads (flux_ct,
+// mhd_module.f90:310-360): ecell of every cell (ghosts mapped as in
+// BOUNDARY_EFIELD) and, for face EMFs, the ghost rows along their two
+// cell-centred axes plus the faces of inactive directions (1D/2D fill).
+__global__ __launch_bounds__(256) void k_emf_prep(Args a) {
+    if (!a.S->active) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n0 = a.nx + 2, n1 = a.ny + 2;
+    if (t >= n0 * n1 * (a.nz + 2)) return;
+    const int j = (int)(t % n0), k = (int)((t / n0) % n1), l = (int)(t / (n0 * n1));
+    const long c = cidx(a, j, k, l);
+    a.Ec[0][c] = ecell_at<0>(a, j, k, l);
+    a.Ec[1][c] = ecell_at<1>(a, j, k, l);
+    a.Ec[2][c] = ecell_at<2>(a, j, k, l);
+    const bool gx = j < 1 || j > a.nx, gy = k < 1 || k > a.ny, gz = l < 1 || l > a.nz;
+    // x faces (face index j <= nx), cell axes y, z
+    if (j <= a.nx && (a.ndim < 1 || gy || gz)) {
+        a.F[0][5][c] = eface_at<1, 0>(a, j, k, l);
+        a.F[0][6][c] = eface_at<2, 0>(a, j, k, l);
+    }
+    if (k <= a.ny && (a.ndim < 2 || gx || gz)) {
+        a.F[1][5][c] = eface_at<0, 1>(a, j, k, l);
+        a.F[1][6][c] = eface_at<2, 1>(a, j, k, l);
+    }
+    if (l <= a.nz && (a.ndim < 3 || gx || gy)) {
+        a.F[2][5][c] = eface_at<0, 2>(a, j, k, l);
+        a.F[2][6][c] = eface_at<1, 2>(a, j, k, l);
+    }
+}
+
 // Edge EMFs (mhd_module.f90:414-423) on the edges the CT update reads:
 // efield_x for j=1..nx, efield_y for k=1..ny, efield_z for l=1..nz
-// (0..n along the other axes).
+// (0..n along the other axes), from the arrays k_emf_prep completed.
 __global__ __launch_bounds__(256) void k_edge(Args a) {
     if (!a.S->active) return;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n1 = a.nx + 1, n2 = a.ny + 1;
     if (t >= n1 * n2 * (a.nz + 1)) return;
-    int j = (int)(t % n1), k = (int)((t / n1) % n2), l = (int)(t / (n1 * n2));
-    long c = cidx(a, j, k, l);
-    if (j >= 1)
-        a.E[0][c] = 0.50 * (eface_at<0, 1>(a, j, k, l) + eface_at<0, 1>(a, j, k, l + 1) + eface_at<0, 2>(a, j, k, l) +
-                            eface_at<0, 2>(a, j, k + 1, l)) -
-                    0.25 * (ecell_at<0>(a, j, k, l) + ecell_at<0>(a, j, k, l + 1) + ecell_at<0>(a, j, k + 1, l) +
-                            ecell_at<0>(a, j, k + 1, l + 1));
-    if (k >= 1)
-        a.E[1][c] = 0.50 * (eface_at<1, 0>(a, j, k, l) + eface_at<1, 0>(a, j, k, l + 1) + eface_at<1, 2>(a, j, k, l) +
-                            eface_at<1, 2>(a, j + 1, k, l)) -
-                    0.25 * (ecell_at<1>(a, j, k, l) + ecell_at<1>(a, j, k, l + 1) + ecell_at<1>(a, j + 1, k, l) +
-                            ecell_at<1>(a, j + 1, k, l + 1));
-    if (l >= 1)
-        a.E[2][c] = 0.50 * (eface_at<2, 0>(a, j, k, l) + eface_at<2, 0>(a, j, k + 1, l) + eface_at<2, 1>(a, j, k, l) +
-                            eface_at<2, 1>(a, j + 1, k, l)) -
-                    0.25 * (ecell_at<2>(a, j, k, l) + ecell_at<2>(a, j, k + 1, l) + ecell_at<2>(a, j + 1, k, l) +
-                            ecell_at<2>(a, j + 1, k + 1, l));
+    const int j = (int)(t % n1), k = (int)((t / n1) % n2), l = (int)(t / (n1 * n2));
+    const long c = cidx(a, j, k, l), sy = a.NX, sz = a.NX * a.NY;
+    const double *ex = a.Ec[0], *ey = a.Ec[1], *ez = a.Ec[2];
+    if (j >= 1) {
+        const double *fxy = a.F[1][5], *fxz = a.F[2][5];
+        a.E[0][c] = 0.50 * (fxy[c] + fxy[c + sz] + fxz[c] + fxz[c + sy]) -
+                    0.25 * (ex[c] + ex[c + sz] + ex[c + sy] + ex[c + sy + sz]);
+    }
+    if (k >= 1) {
+        const double *fyx = a.F[0][5], *fyz = a.F[2][6];
+        a.E[1][c] = 0.50 * (fyx[c] + fyx[c + sz] + fyz[c] + fyz[c + 1]) -
+                    0.25 * (ey[c] + ey[c + sz] + ey[c + 1] + ey[c + 1 + sz]);
+    }
+    if (l >= 1) {
+        const double *fzx = a.F[0][6], *fzy = a.F[1][6];
+        a.E[2][c] = 0.50 * (fzx[c] + fzx[c + sy] + fzy[c] + fzy[c + 1]) -
+                    0.25 * (ez[c] + ez[c + sy] + ez[c + 1] + ez[c + 1 + sy]);
+    }
 }
 
 // ---------------------------------------------------------------------------

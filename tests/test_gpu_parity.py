@@ -40,6 +40,25 @@ def test_gpu_matches_reference_fixture(name):
     assert s.find_divb() == float(gold["maxdivb"])
 
 
+@pytest.mark.parametrize("threads", [1, 37])
+@pytest.mark.parametrize("name", ["ot_hlld_weno", "ot_hlld_tvdmc", "ot_hllc_ppm", "ot_bc_mixed",
+                                  "blast_weno", "ot2d_hlld_ppm", "torus_hlld_ppm", "torus_sym_tvdmc"])
+def test_gpu_long_march_segments(name, threads, monkeypatch):
+    """The y/z sweeps with long segments: CUMC3D_MARCH_THREADS=1 gives one
+    segment per column (every thread marches the whole column), 37 an uneven
+    split; the result must not depend on the segmentation."""
+    monkeypatch.setenv("CUMC3D_MARCH_THREADS", str(threads))
+    case = CASES[name]
+    gold = load_golden(name)
+    print(f"case {name} march_threads={threads}", flush=True)
+    s = make_solver(case.cfg, ic.GENERATORS[case.ic](case.cfg))
+    dts = s.run(case.steps)
+    assert np.array_equal(dts, gold["dts"])
+    w = interior_window(case.cfg)
+    st = s.state()
+    assert_same({f: st[f][w] for f in FIELDS}, gold, label=f"{name}/{threads}: ")
+
+
 @pytest.mark.parametrize("cfg,ic_name,steps", [
     (orszag_tang_3d(32, recon="weno"), "orszag_tang", 4),
     (orszag_tang_3d(24, recon="ppm").with_(nx=40, boundary_flag=(1, 2, 0, 0, 2, 1)), "orszag_tang", 4),
