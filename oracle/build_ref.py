@@ -16,6 +16,11 @@ Recipe (test infrastructure only):
   * flang -O2 -ffp-contract=off -fopenmp, as planned for the CPU baseline.
 Outputs: oracle/_ref/<case>/cumc3d_ref.
 
+For the cases in DROPIN_CASES it also links oracle/_ref/dropin/<case>/cumc3d_dropin:
+the same reference program with solver/mhd/rungekutta.f90 replaced by the
+Fortran drop-in of this repository (cumc3d_amd/fortran/*.f90) over
+libcumc3d_hip.so, i.e. the reference's own driver stepping on the GPU.
+
 usage: python oracle/build_ref.py [case ...]     (default: every case)
 """
 from __future__ import annotations
@@ -36,6 +41,10 @@ REF = os.environ.get("CUMC3D_REFERENCE", "/root/reference")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 SCRATCH = os.environ.get("CUMC3D_REF_SCRATCH", "/tmp/cumc3d_ref_build")
 OUT = os.path.join(HERE, "_ref")
+REPO = os.path.dirname(HERE)
+HIPLIB_DIR = os.path.join(REPO, "cumc3d_amd")
+DROPIN_SRC = [os.path.join(REPO, "cumc3d_amd", "fortran", f) for f in ("cumc3d_hip.f90", "rungekutta_hip.f90")]
+DROPIN_CASES = ("c1_briowu", "ot_hlld_weno", "ot_bc_mixed", "torus_hlld_ppm", "ot2d_hlld_ppm")
 FFLAGS = ["-cpp", "-O2", "-ffp-contract=off", "-fopenmp", "-DDIVB", "-DFIXPOLE"]
 
 # solver/mhd/source.list minus main.f90, dataio.f90 (HDF5) and openacc.f90
@@ -155,7 +164,28 @@ def build_case(name: str) -> str:
     libdir = os.path.join(os.path.dirname(os.path.dirname(FLANG)), "lib")
     run([FLANG, "-fopenmp", "-o", os.path.join(outdir, "cumc3d_ref"), *objs,
          f"-Wl,-rpath,{libdir}"], work)
+    if name in DROPIN_CASES and os.path.exists(os.path.join(HIPLIB_DIR, "libcumc3d_hip.so")):
+        build_dropin(name, cfg, work, objs, libdir)
     return outdir
+
+
+def build_dropin(name, cfg, work, objs, libdir):
+    """Link the reference program with the Fortran drop-in instead of rungekutta.f90."""
+    extra = ["-DCUMC3D_TORUS"] if cfg.model == "torus" else []
+    dobjs = [o for o in objs if o not in ("rungekutta.o", "driver.o")]
+    for src in DROPIN_SRC:
+        obj = "dropin_" + os.path.basename(src).replace(".f90", ".o")
+        run([FLANG, *FFLAGS, *extra, "-c", src, "-o", obj], work)
+        dobjs.append(obj)
+    run([FLANG, *FFLAGS, "-DHIPDROP", "-c", os.path.join(HERE, "ref_harness", "driver.f90"),
+         "-o", "driver_hipdrop.o"], work)
+    dobjs.append("driver_hipdrop.o")
+    outdir = os.path.join(OUT, "dropin", name)
+    os.makedirs(outdir, exist_ok=True)
+    # the library is found relative to the binary, so the tree can move
+    run([FLANG, "-fopenmp", "-o", os.path.join(outdir, "cumc3d_dropin"), *dobjs,
+         f"-L{HIPLIB_DIR}", "-lcumc3d_hip", f"-Wl,-rpath,{libdir}",
+         "-Wl,-rpath,$ORIGIN/../../../../cumc3d_amd"], work)
 
 
 def available() -> bool:

@@ -3,11 +3,18 @@
 ! does it (INITIAL_MODEL, INITIAL_UPDATE, then FINDDT / dt clip / RUNGEKUTTA),
 ! without the HDF5 writer, and dumps the final state as raw binary.
 !
+! Built with -DHIPDROP it links the drop-in RUNGEKUTTA/FINDDT of
+! cumc3d_amd/fortran/rungekutta_hip.f90 instead of the reference's and
+! refreshes the host state from the device before FIND_DIVB and the dump.
+!
 ! usage: cumc3d_ref IC_FILE OUT_FILE MAX_STEPS [THREADS]
 PROGRAM REF_HARNESS
 USE DEFINITION
 USE MHD_MODULE
 USE HARNESS_IO
+#ifdef HIPDROP
+USE CUMC3D_HIP_STATE, ONLY : CUMC3D_HIP_SYNC_HOST, CUMC3D_HIP_DETACH
+#endif
 IMPLICIT NONE
 CHARACTER(len=1024) :: out_path, arg
 INTEGER :: max_steps, nthreads, u, cnt, c0, c1, crate
@@ -48,6 +55,9 @@ DO WHILE (global_time < total_time .AND. n_step < max_steps)
 END DO
 CALL SYSTEM_CLOCK(c1)
 loop_seconds = DBLE(c1 - c0)/DBLE(crate)
+#ifdef HIPDROP
+CALL CUMC3D_HIP_SYNC_HOST
+#endif
 
 OPEN(UNIT=999, FILE=TRIM(out_path)//'.divb', STATUS='REPLACE')
 CALL FIND_DIVB
@@ -63,4 +73,7 @@ WRITE(u) cs
 WRITE(u) cons
 WRITE(u) dts(1:cnt)
 CLOSE(u)
+#ifdef HIPDROP
+CALL CUMC3D_HIP_DETACH
+#endif
 END PROGRAM REF_HARNESS

@@ -44,17 +44,20 @@ def read_dump(path: str) -> dict:
     return out
 
 
-def run_ref(case: str, ic_path: str, steps: int, threads: int = 1, timeout: float = 600) -> dict:
+def dropin_binary(case: str) -> str:
+    """The reference program stepping through the Fortran drop-in on the GPU."""
+    return os.path.join(HERE, "_ref", "dropin", case, "cumc3d_dropin")
+
+
+def run_ref(case: str, ic_path: str, steps: int, threads: int = 1, timeout: float = 600,
+            binary: str | None = None) -> dict:
     with tempfile.TemporaryDirectory() as d:
         out = os.path.join(d, "dump.bin")
         env = dict(os.environ)
-        # HLLDNM reads usstar(ibn), which it never assigns
-        # (solver/mhd/rsolver_module.f90:704-741); the value comes from
-        # whatever the OpenMP-private copy's malloc returned.  glibc's
-        # MALLOC_PERTURB_=255 zero-fills every allocation, which pins that
-        # value to 0, the value fresh memory holds.
+        # Fresh heap memory filled with a fixed byte pattern, so that nothing
+        # the program reads before writing can differ between runs.
         env["MALLOC_PERTURB_"] = "255"
-        r = subprocess.run([ref_binary(case), ic_path, out, str(steps), str(threads)],
+        r = subprocess.run([binary or ref_binary(case), ic_path, out, str(steps), str(threads)],
                            capture_output=True, text=True, timeout=timeout, env=env)
         if r.returncode != 0:
             raise RuntimeError(f"reference oracle failed: {r.stderr[-2000:]}")

@@ -1,0 +1,43 @@
+"""The Fortran drop-in: the reference program itself (solver modules, model
+hooks, its time loop) with RUNGEKUTTA/FINDDT replaced by
+cumc3d_amd/fortran/rungekutta_hip.f90 over the C-ABI, against the reference
+fixtures bit for bit.  Needs an MI355X and the prebuilt drop-in binaries
+(oracle/_ref/dropin, linked by oracle/build_ref.py)."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import interior_window, load_golden
+from cumc3d_amd import ic
+from oracle.build_ref import DROPIN_CASES
+from oracle.cases import CASES
+from oracle.refio import dropin_binary, run_ref
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz", "eps", "bcx", "bcy", "bcz", "cs")
+
+
+@pytest.mark.parametrize("name", DROPIN_CASES)
+def test_dropin_matches_reference(name):
+    binary = dropin_binary(name)
+    if not os.path.isfile(binary):
+        pytest.skip(f"{binary} not built (oracle/build_ref.py needs the reference sources)")
+    case = CASES[name]
+    gold = load_golden(name)
+    print(f"case {name} (Fortran drop-in)", flush=True)
+    st = ic.GENERATORS[case.ic](case.cfg)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "ic.bin")
+        ic.write_ic(p, case.cfg, st)
+        got = run_ref(name, p, case.steps, threads=1, timeout=300, binary=binary)
+    assert np.array_equal(got["dts"], gold["dts"])
+    w = interior_window(case.cfg)
+    for f in FIELDS:
+        if f not in gold:
+            continue
+        a, b = got[f][w], gold[f]
+        assert np.array_equal(a, b, equal_nan=True), f"{name}: {f} differs, max {np.nanmax(np.abs(a - b)):.3e}"
+    assert got["maxdivb"] == float(gold["maxdivb"])
