@@ -33,12 +33,16 @@ struct cumc3d_ctx {
     int nx, ny, nz;
     long NX, NY, NZ, G;
     hipStream_t stream = nullptr;
-    double *pool = nullptr;       // all per-cell fields
+    void *pool = nullptr;         // all per-cell fields
+    bool fp32 = false;            // CUMC3D_FP32: working fields are float
     double *met = nullptr;        // grid metrics and weights
     DevScalars *S = nullptr;
     unsigned long long *blockmax = nullptr;
     long nblockmax = 0;
-    double *P[5], *eps, *cs, *Bc[3], *B[2][3], *Uo[8], *F[3][7], *E[3], *Ec[3], *floors[3];
+    // working-type fields (double, or float in the FP32 split)
+    void *P[5], *eps, *cs, *Bc[3], *F[3][7], *E[3], *Ec[3];
+    // always double: face field (ping-pong), u_old, conserved state (FP32 split), floors
+    double *B[2][3], *Uo[8], *U[5], *floors[3];
     int cur = 0;                  // B[cur] holds the current face field
     long march_threads = 1L << 19;  // target thread count of the y/z sweeps
     int bfac[6][12];
@@ -55,47 +59,83 @@ struct cumc3d_ctx {
 // ---------------------------------------------------------------------------
 // layout conversion kernels (upload/download only)
 // ---------------------------------------------------------------------------
-__global__ void k_split_prim(const double *aos, double *rho, double *vx, double *vy, double *vz, double *p,
-                             double *bx0, double *by0, double *bz0, double *bx1, double *by1, double *bz1, long G) {
+template <class W> __global__ void k_split_prim(const double *aos, const double *eps64, ArgsT<W> a) {
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= G) return;
+    if (c >= a.G) return;
     const double *q = aos + 8 * c;
-    rho[c] = q[0]; vx[c] = q[1]; vy[c] = q[2]; vz[c] = q[3]; p[c] = q[4];
-    bx0[c] = q[5]; by0[c] = q[6]; bz0[c] = q[7];
-    bx1[c] = q[5]; by1[c] = q[6]; bz1[c] = q[7];
+    for (int i = 0; i < 5; ++i) a.P[i][c] = (W)q[i];
+    for (int d = 0; d < 3; ++d) {
+        a.Bin[d][c] = q[5 + d];
+        a.Bout[d][c] = q[5 + d];
+    }
+    a.eps[c] = (W)eps64[c];
+    if constexpr (sizeof(W) == 4) {
+        // double copies of rho, v, eps for INITIAL_UPDATE's conserved state
+        for (int i = 0; i < 4; ++i) a.U[i][c] = q[i];
+        a.U[4][c] = eps64[c];
+    }
 }
 
-__global__ void k_join(double *aos, int nv, const double *f0, const double *f1, const double *f2, const double *f3,
-                       const double *f4, const double *f5, const double *f6, const double *f7, long G) {
+// prim AoS (rho, vx, vy, vz, p, Bx, By, Bz per cell) from the device fields
+template <class W> __global__ void k_join_prim(double *aos, ArgsT<W> a) {
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.G) return;
+    double *q = aos + 8 * c;
+    for (int i = 0; i < 5; ++i) q[i] = a.P[i][c];
+    for (int d = 0; d < 3; ++d) q[5 + d] = a.Bin[d][c];
+}
+
+template <class W> __global__ void k_join3(double *aos, const W *f0, const W *f1, const W *f2, long G) {
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= G) return;
-    const double *f[8] = {f0, f1, f2, f3, f4, f5, f6, f7};
-    for (int i = 0; i < nv; ++i) aos[(long)nv * c + i] = f[i][c];
+    aos[3 * c] = f0[c];
+    aos[3 * c + 1] = f1[c];
+    aos[3 * c + 2] = f2[c];
+}
+
+template <class W> __global__ void k_widen(double *out, const W *in, long G) {
+    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < G) out[c] = in[c];
 }
 
 // INITIAL_UPDATE interior part: CUSTOM_CHECKRHO, bcell of FROMRVETOU and
-// FINDPRESSURE on the interior (initial.f90:91-108).
-__global__ void k_init_interior(Args a) {
+// FINDPRESSURE on the interior (initial.f90:91-108).  The FP32 split starts
+// from the double copies k_split_prim left in U and writes the conserved
+// state there (FROMRVETOU, fromrvetou.f90:428-438).
+template <class W> __global__ void k_init_interior(ArgsT<W> a) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)a.nx * a.ny * a.nz) return;
     const int j = 1 + (int)(t % a.nx), k = 1 + (int)((t / a.nx) % a.ny), l = 1 + (int)(t / ((long)a.nx * a.ny));
     const long c = cidx(a, j, k, l);
-    double rho = a.P[0][c], eps = a.eps[c];
-    double v[3] = {a.P[1][c], a.P[2][c], a.P[3][c]};
+    constexpr bool split = sizeof(W) == 4;
+    double rho, eps, v[3];
+    if constexpr (split) {
+        rho = a.U[0][c]; v[0] = a.U[1][c]; v[1] = a.U[2][c]; v[2] = a.U[3][c]; eps = a.U[4][c];
+    } else {
+        rho = a.P[0][c]; v[0] = a.P[1][c]; v[1] = a.P[2][c]; v[2] = a.P[3][c]; eps = a.eps[c];
+    }
     checkrho(a, c, rho, v, eps);
-    a.P[0][c] = rho; a.P[1][c] = v[0]; a.P[2][c] = v[1]; a.P[3][c] = v[2];
-    a.eps[c] = eps;
-    a.Bc[0][c] = bavg<0>(a, j, a.Bin[0][c], a.Bin[0][c - 1]);
-    a.Bc[1][c] = bavg<1>(a, j, a.Bin[1][c], a.Bin[1][c - a.NX]);
-    a.Bc[2][c] = bavg<2>(a, j, a.Bin[2][c], a.Bin[2][c - a.NX * a.NY]);
+    a.P[0][c] = (W)rho; a.P[1][c] = (W)v[0]; a.P[2][c] = (W)v[1]; a.P[3][c] = (W)v[2];
+    a.eps[c] = (W)eps;
+    double b[3] = {bavg<0>(a, j, a.Bin[0][c], a.Bin[0][c - 1]), bavg<1>(a, j, a.Bin[1][c], a.Bin[1][c - a.NX]),
+                   bavg<2>(a, j, a.Bin[2][c], a.Bin[2][c - a.NX * a.NY])};
+    for (int d = 0; d < 3; ++d) a.Bc[d][c] = (W)b[d];
     double p, cs;
     pressure(a, c, rho, eps, p, cs);
-    a.P[4][c] = p;
-    a.cs[c] = cs;
+    a.P[4][c] = (W)p;
+    a.cs[c] = (W)cs;
+    if constexpr (split) {
+        double vsq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2], bsq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+        a.U[0][c] = rho;
+        a.U[1][c] = v[0] * rho;
+        a.U[2][c] = v[1] * rho;
+        a.U[3][c] = v[2] * rho;
+        a.U[4][c] = rho * (eps + 0.5 * vsq) + 0.5 * bsq;
+    }
 }
 
 // standalone FINDDT (rungekutta.f90:256-323)
-__global__ __launch_bounds__(256) void k_finddt(Args a) {
+template <class W> __global__ __launch_bounds__(256) void k_finddt(ArgsT<W> a) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double dtloc = 1.0e5;
     if (t < (long)a.nx * a.ny * a.nz) {
@@ -139,7 +179,7 @@ __global__ void k_set_dt(DevScalars *S, double dt) {
 }
 
 // FIND_DIVB (mhd_module.f90:133-189) as a max reduction on bit patterns
-__global__ __launch_bounds__(256) void k_divb(Args a, unsigned long long *out) {
+template <class W> __global__ __launch_bounds__(256) void k_divb(ArgsT<W> a, unsigned long long *out) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.0;
     if (t < (long)a.nx * a.ny * a.nz) {
@@ -163,8 +203,8 @@ __global__ __launch_bounds__(256) void k_divb(Args a, unsigned long long *out) {
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
-static Args make_args(cumc3d_ctx *x) {
-    Args a;
+template <class W> static ArgsT<W> make_args(cumc3d_ctx *x) {
+    ArgsT<W> a;
     memset(&a, 0, sizeof(a));
     a.nx = x->nx; a.ny = x->ny; a.nz = x->nz;
     a.ndim = x->d.n_dim;
@@ -179,18 +219,19 @@ static Args make_args(cumc3d_ctx *x) {
     a.cfl = x->d.cfl;
     a.r_sh = x->d.r_sh;
     for (int d = 0; d < 3; ++d) a.ax[d] = x->ax[d];
-    for (int i = 0; i < 5; ++i) a.P[i] = x->P[i];
-    a.eps = x->eps;
-    a.cs = x->cs;
+    for (int i = 0; i < 5; ++i) a.P[i] = (W *)x->P[i];
+    a.eps = (W *)x->eps;
+    a.cs = (W *)x->cs;
     for (int d = 0; d < 3; ++d) {
-        a.Bc[d] = x->Bc[d];
+        a.Bc[d] = (W *)x->Bc[d];
         a.Bin[d] = x->B[x->cur][d];
         a.Bout[d] = x->B[1 - x->cur][d];
-        a.E[d] = x->E[d];
-        a.Ec[d] = x->Ec[d];
-        for (int i = 0; i < 7; ++i) a.F[d][i] = x->F[d][i];
+        a.E[d] = (W *)x->E[d];
+        a.Ec[d] = (W *)x->Ec[d];
+        for (int i = 0; i < 7; ++i) a.F[d][i] = (W *)x->F[d][i];
     }
     for (int i = 0; i < 8; ++i) a.Uo[i] = x->Uo[i];
+    for (int i = 0; i < 5; ++i) a.U[i] = x->U[i];
     a.rfl = x->floors[0];
     a.pfl = x->floors[1];
     a.efl = x->floors[2];
@@ -292,46 +333,46 @@ static int nseg_for(const cumc3d_ctx *x, int nx, int na, int nd) {
 // ---------------------------------------------------------------------------
 // flux dispatch over (reconstruction, Riemann solver)
 // ---------------------------------------------------------------------------
-template <int R, int S>
-static void launch_sweep_rs(cumc3d_ctx *x, const Args &a, int D, hipStream_t st) {
+template <class W, int R, int S>
+static void launch_sweep_rs(cumc3d_ctx *x, const ArgsT<W> &a, int D, hipStream_t st) {
     if (D == 0) {
         long waves = ((long)(x->nx + 2) * x->ny * x->nz + 62) / 63 + 1;
         unsigned grid = (unsigned)((waves * 64 + 255) / 256);
-        hipLaunchKernelGGL((k_flux_x<double, R, S>), dim3(grid), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_flux_x<W, R, S>), dim3(grid), dim3(256), 0, st, a);
         if (S == 3 && a.bn_mode == 2)
-            hipLaunchKernelGGL((k_carry<double, R, 0>), dim3(1), dim3(1024), 0, st, a, (int)grid);
+            hipLaunchKernelGGL((k_carry<W, R, 0>), dim3(1), dim3(1024), 0, st, a, (int)grid);
     } else if (D == 1) {
         int ns = nseg_for(x, x->nx, x->nz, x->ny);
         unsigned grid = nblk((long)x->nx * x->nz * ns);
-        hipLaunchKernelGGL((k_flux_yz<double, R, S, 1>), dim3(grid), dim3(256), 0, st, a, ns);
+        hipLaunchKernelGGL((k_flux_yz<W, R, S, 1>), dim3(grid), dim3(256), 0, st, a, ns);
         if (S == 3 && a.bn_mode == 2)
-            hipLaunchKernelGGL((k_carry<double, R, 1>), dim3(1), dim3(1024), 0, st, a, (int)grid);
+            hipLaunchKernelGGL((k_carry<W, R, 1>), dim3(1), dim3(1024), 0, st, a, (int)grid);
     } else {
         int ns = nseg_for(x, x->nx, x->ny, x->nz);
         unsigned grid = nblk((long)x->nx * x->ny * ns);
-        hipLaunchKernelGGL((k_flux_yz<double, R, S, 2>), dim3(grid), dim3(256), 0, st, a, ns);
+        hipLaunchKernelGGL((k_flux_yz<W, R, S, 2>), dim3(grid), dim3(256), 0, st, a, ns);
         if (S == 3 && a.bn_mode == 2)
-            hipLaunchKernelGGL((k_carry<double, R, 2>), dim3(1), dim3(1024), 0, st, a, (int)grid);
+            hipLaunchKernelGGL((k_carry<W, R, 2>), dim3(1), dim3(1024), 0, st, a, (int)grid);
     }
 }
 
-template <int R>
-static void launch_sweep_r(cumc3d_ctx *x, const Args &a, int D, hipStream_t st) {
+template <class W, int R>
+static void launch_sweep_r(cumc3d_ctx *x, const ArgsT<W> &a, int D, hipStream_t st) {
     switch (x->d.riemann) {
-        case 0: launch_sweep_rs<R, 0>(x, a, D, st); break;
-        case 1: launch_sweep_rs<R, 1>(x, a, D, st); break;
-        case 2: launch_sweep_rs<R, 2>(x, a, D, st); break;
-        default: launch_sweep_rs<R, 3>(x, a, D, st); break;
+        case 0: launch_sweep_rs<W, R, 0>(x, a, D, st); break;
+        case 1: launch_sweep_rs<W, R, 1>(x, a, D, st); break;
+        case 2: launch_sweep_rs<W, R, 2>(x, a, D, st); break;
+        default: launch_sweep_rs<W, R, 3>(x, a, D, st); break;
     }
 }
 
-static void launch_sweep(cumc3d_ctx *x, const Args &a, int D, hipStream_t st) {
+template <class W> static void launch_sweep(cumc3d_ctx *x, const ArgsT<W> &a, int D, hipStream_t st) {
     switch (x->d.recon) {
-        case 0: launch_sweep_r<0>(x, a, D, st); break;
-        case 1: launch_sweep_r<1>(x, a, D, st); break;
-        case 2: launch_sweep_r<2>(x, a, D, st); break;
-        case 3: launch_sweep_r<3>(x, a, D, st); break;
-        default: launch_sweep_r<4>(x, a, D, st); break;
+        case 0: launch_sweep_r<W, 0>(x, a, D, st); break;
+        case 1: launch_sweep_r<W, 1>(x, a, D, st); break;
+        case 2: launch_sweep_r<W, 2>(x, a, D, st); break;
+        case 3: launch_sweep_r<W, 3>(x, a, D, st); break;
+        default: launch_sweep_r<W, 4>(x, a, D, st); break;
     }
 }
 
@@ -352,9 +393,9 @@ static inline void tmark(cumc3d_ctx *x, int slot) {
     if (x->timing && x->tstage < TSTAGES) hipEventRecord(x->ev[x->tstage * 7 + slot], x->stream);
 }
 
-static void run_stage(cumc3d_ctx *x, int stage) {
+template <class W> static void run_stage_t(cumc3d_ctx *x, int stage) {
     hipStream_t st = x->stream;
-    Args a = make_args(x);
+    ArgsT<W> a = make_args<W>(x);
     const long nn = (long)x->nx * x->ny * x->nz;
     tmark(x, 0);
     launch_sweep(x, a, 0, st);
@@ -363,18 +404,23 @@ static void run_stage(cumc3d_ctx *x, int stage) {
     tmark(x, 2);
     if (x->d.n_dim > 2) launch_sweep(x, a, 2, st);
     tmark(x, 3);
-    hipLaunchKernelGGL(k_emf_prep, dim3(nblk((long)(x->nx + 2) * (x->ny + 2) * (x->nz + 2))), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_edge, dim3(nblk((long)(x->nx + 1) * (x->ny + 1) * (x->nz + 1))), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_emf_prep<W>, dim3(nblk((long)(x->nx + 2) * (x->ny + 2) * (x->nz + 2))), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_edge<W>, dim3(nblk((long)(x->nx + 1) * (x->ny + 1) * (x->nz + 1))), dim3(256), 0, st, a);
     tmark(x, 4);
-    if (stage == 1) hipLaunchKernelGGL(k_update<1>, dim3(nblk(nn)), dim3(256), 0, st, a);
-    else if (stage == 2) hipLaunchKernelGGL(k_update<2>, dim3(nblk(nn)), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(k_update<3>, dim3(nblk(nn)), dim3(256), 0, st, a);
+    if (stage == 1) hipLaunchKernelGGL((k_update<1, W>), dim3(nblk(nn)), dim3(256), 0, st, a);
+    else if (stage == 2) hipLaunchKernelGGL((k_update<2, W>), dim3(nblk(nn)), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_update<3, W>), dim3(nblk(nn)), dim3(256), 0, st, a);
     tmark(x, 5);
     x->cur = 1 - x->cur;
-    Args g = make_args(x);
-    hipLaunchKernelGGL(k_ghost, dim3(nblk(x->G)), dim3(256), 0, st, g);
+    ArgsT<W> g = make_args<W>(x);
+    hipLaunchKernelGGL(k_ghost<W>, dim3(nblk(x->G)), dim3(256), 0, st, g);
     tmark(x, 6);
     if (x->timing && x->tstage < TSTAGES) x->tstage += 1;
+}
+
+static void run_stage(cumc3d_ctx *x, int stage) {
+    if (x->fp32) run_stage_t<float>(x, stage);
+    else run_stage_t<double>(x, stage);
 }
 
 // per-kernel-class device times from the recorded events (after a sync)
@@ -393,9 +439,9 @@ static void accumulate_timing(cumc3d_ctx *x) {
 }
 
 static int launch_finddt(cumc3d_ctx *x) {
-    Args a = make_args(x);
     const long nn = (long)x->nx * x->ny * x->nz;
-    hipLaunchKernelGGL(k_finddt, dim3(nblk(nn)), dim3(256), 0, x->stream, a);
+    if (x->fp32) hipLaunchKernelGGL(k_finddt<float>, dim3(nblk(nn)), dim3(256), 0, x->stream, make_args<float>(x));
+    else hipLaunchKernelGGL(k_finddt<double>, dim3(nblk(nn)), dim3(256), 0, x->stream, make_args<double>(x));
     return 0;
 }
 
@@ -433,7 +479,7 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
     if (d.recon < 0 || d.recon > 4) return fail(CUMC3D_EINVAL, "unknown reconstruction");
     if (d.model < 0 || d.model > 1) return fail(CUMC3D_EINVAL, "unknown model");
     if (d.hlld_bn_mode < 0 || d.hlld_bn_mode > 2) return fail(CUMC3D_EINVAL, "unknown hlld_bn_mode");
-    if (d.precision != CUMC3D_FP64) return fail(CUMC3D_EINVAL, "only CUMC3D_FP64 is available in this build");
+    if (d.precision != CUMC3D_FP64 && d.precision != CUMC3D_FP32) return fail(CUMC3D_EINVAL, "unknown precision");
     if (!d.xF || !d.yF || !d.zF) return fail(CUMC3D_EINVAL, "face coordinate arrays are required");
     for (int i = 0; i < 6; ++i)
         if (d.boundary_flag[i] < 0 || d.boundary_flag[i] > 4) return fail(CUMC3D_EINVAL, "boundary flag out of range");
@@ -490,28 +536,34 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         x->ax[a].c = x->met + off[a][2];
         x->ax[a].w = x->met + off[a][3];
     }
-    // per-cell fields
-    const int nfields = 5 + 2 + 3 + 6 + 8 + 21 + 3 + 3 + 3;
-    size_t bytes = (size_t)nfields * x->G * sizeof(double);
+    // per-cell fields: the working-type block, then the double block, each
+    // field G entries, fields 256-byte aligned
+    x->fp32 = d.precision == CUMC3D_FP32;
+    const size_t wsz = x->fp32 ? sizeof(float) : sizeof(double);
+    const size_t wfield = (x->G * wsz + 255) / 256 * 256, dfield = (x->G * sizeof(double) + 255) / 256 * 256;
+    const int nw = 5 + 2 + 3 + 21 + 3 + 3, nd = 6 + 8 + (x->fp32 ? 5 : 0) + 3;
+    size_t bytes = nw * wfield + nd * dfield;
     if (hipMalloc(&x->pool, bytes) != hipSuccess) {
         cleanup();
         return fail(CUMC3D_ENOMEM, "state allocation of " + std::to_string(bytes) + " bytes failed");
     }
     if (hipMemset(x->pool, 0, bytes) != hipSuccess) { cleanup(); return fail(CUMC3D_EHIP, "memset"); }
-    double *p = x->pool;
-    auto take = [&]() { double *r = p; p += x->G; return r; };
-    for (int i = 0; i < 5; ++i) x->P[i] = take();
-    x->eps = take();
-    x->cs = take();
-    for (int i = 0; i < 3; ++i) x->Bc[i] = take();
-    for (int b = 0; b < 2; ++b)
-        for (int i = 0; i < 3; ++i) x->B[b][i] = take();
-    for (int i = 0; i < 8; ++i) x->Uo[i] = take();
+    char *p = (char *)x->pool;
+    auto takew = [&]() { void *r = p; p += wfield; return r; };
+    auto taked = [&]() { double *r = (double *)p; p += dfield; return r; };
+    for (int i = 0; i < 5; ++i) x->P[i] = takew();
+    x->eps = takew();
+    x->cs = takew();
+    for (int i = 0; i < 3; ++i) x->Bc[i] = takew();
     for (int dd = 0; dd < 3; ++dd)
-        for (int i = 0; i < 7; ++i) x->F[dd][i] = take();
-    for (int i = 0; i < 3; ++i) x->E[i] = take();
-    for (int i = 0; i < 3; ++i) x->Ec[i] = take();
-    for (int i = 0; i < 3; ++i) x->floors[i] = take();
+        for (int i = 0; i < 7; ++i) x->F[dd][i] = takew();
+    for (int i = 0; i < 3; ++i) x->E[i] = takew();
+    for (int i = 0; i < 3; ++i) x->Ec[i] = takew();
+    for (int b = 0; b < 2; ++b)
+        for (int i = 0; i < 3; ++i) x->B[b][i] = taked();
+    for (int i = 0; i < 8; ++i) x->Uo[i] = taked();
+    for (int i = 0; i < 5; ++i) x->U[i] = x->fp32 ? taked() : nullptr;
+    for (int i = 0; i < 3; ++i) x->floors[i] = taked();
     if (hipMalloc(&x->S, sizeof(DevScalars)) != hipSuccess) { cleanup(); return fail(CUMC3D_ENOMEM, "scalars"); }
     DevScalars s0;
     memset(&s0, 0, sizeof(s0));
@@ -537,18 +589,22 @@ int cumc3d_upload(cumc3d_ctx *x, const double *prim, const double *eps, const do
     if (!x || !prim || !eps) return fail(CUMC3D_EINVAL, "null argument");
     if (x->d.model == CUMC3D_MODEL_TORUS && !floors3) return fail(CUMC3D_EINVAL, "torus model needs floor arrays");
     double *tmp = nullptr;
-    HIPCHK(hipMalloc(&tmp, (size_t)8 * x->G * sizeof(double)));
+    HIPCHK(hipMalloc(&tmp, (size_t)9 * x->G * sizeof(double)));
     HIPCHK(hipMemcpyAsync(tmp, prim, (size_t)8 * x->G * sizeof(double), hipMemcpyHostToDevice, x->stream));
-    hipLaunchKernelGGL(k_split_prim, dim3(nblk(x->G)), dim3(256), 0, x->stream, tmp, x->P[0], x->P[1], x->P[2],
-                       x->P[3], x->P[4], x->B[0][0], x->B[0][1], x->B[0][2], x->B[1][0], x->B[1][1], x->B[1][2], x->G);
-    HIPCHK(hipMemcpyAsync(x->eps, eps, x->G * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(tmp + (size_t)8 * x->G, eps, x->G * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    x->cur = 0;
+    if (x->fp32)
+        hipLaunchKernelGGL(k_split_prim<float>, dim3(nblk(x->G)), dim3(256), 0, x->stream, tmp, tmp + (size_t)8 * x->G,
+                           make_args<float>(x));
+    else
+        hipLaunchKernelGGL(k_split_prim<double>, dim3(nblk(x->G)), dim3(256), 0, x->stream, tmp, tmp + (size_t)8 * x->G,
+                           make_args<double>(x));
     if (floors3)
         for (int i = 0; i < 3; ++i)
             HIPCHK(hipMemcpyAsync(x->floors[i], floors3 + (size_t)i * x->G, x->G * sizeof(double),
                                   hipMemcpyHostToDevice, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     HIPCHK(hipFree(tmp));
-    x->cur = 0;
     x->uploaded = true;
     x->initialized = false;
     return 0;
@@ -557,10 +613,15 @@ int cumc3d_upload(cumc3d_ctx *x, const double *prim, const double *eps, const do
 int cumc3d_initial_update(cumc3d_ctx *x) {
     if (!x) return fail(CUMC3D_EINVAL, "null context");
     if (!x->uploaded) return fail(CUMC3D_ESTATE, "upload an initial model first");
-    Args a = make_args(x);
     hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, x->stream, x->S, 0.0);   // also re-activates the stage kernels
-    hipLaunchKernelGGL(k_init_interior, dim3(nblk((long)x->nx * x->ny * x->nz)), dim3(256), 0, x->stream, a);
-    hipLaunchKernelGGL(k_ghost, dim3(nblk(x->G)), dim3(256), 0, x->stream, a);
+    const unsigned gi = nblk((long)x->nx * x->ny * x->nz), gg = nblk(x->G);
+    if (x->fp32) {
+        hipLaunchKernelGGL(k_init_interior<float>, dim3(gi), dim3(256), 0, x->stream, make_args<float>(x));
+        hipLaunchKernelGGL(k_ghost<float>, dim3(gg), dim3(256), 0, x->stream, make_args<float>(x));
+    } else {
+        hipLaunchKernelGGL(k_init_interior<double>, dim3(gi), dim3(256), 0, x->stream, make_args<double>(x));
+        hipLaunchKernelGGL(k_ghost<double>, dim3(gg), dim3(256), 0, x->stream, make_args<double>(x));
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(x->stream));
     x->initialized = true;
@@ -675,22 +736,36 @@ int cumc3d_download(cumc3d_ctx *x, double *prim, double *eps, double *bcell, dou
     if (!x) return fail(CUMC3D_EINVAL, "null context");
     double *tmp = nullptr;
     if (prim || bcell) HIPCHK(hipMalloc(&tmp, (size_t)8 * x->G * sizeof(double)));
-    double **B = x->B[x->cur];
+    if (!tmp && x->fp32 && (eps || cs)) HIPCHK(hipMalloc(&tmp, (size_t)x->G * sizeof(double)));
+    const unsigned gg = nblk(x->G);
     if (prim) {
-        hipLaunchKernelGGL(k_join, dim3(nblk(x->G)), dim3(256), 0, x->stream, tmp, 8, x->P[0], x->P[1], x->P[2],
-                           x->P[3], x->P[4], B[0], B[1], B[2], x->G);
+        if (x->fp32) hipLaunchKernelGGL(k_join_prim<float>, dim3(gg), dim3(256), 0, x->stream, tmp, make_args<float>(x));
+        else hipLaunchKernelGGL(k_join_prim<double>, dim3(gg), dim3(256), 0, x->stream, tmp, make_args<double>(x));
         HIPCHK(hipMemcpyAsync(prim, tmp, (size_t)8 * x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
         HIPCHK(hipStreamSynchronize(x->stream));
     }
     if (bcell) {
-        hipLaunchKernelGGL(k_join, dim3(nblk(x->G)), dim3(256), 0, x->stream, tmp, 3, x->Bc[0], x->Bc[1], x->Bc[2],
-                           x->Bc[2], x->Bc[2], x->Bc[2], x->Bc[2], x->Bc[2], x->G);
+        if (x->fp32)
+            hipLaunchKernelGGL(k_join3<float>, dim3(gg), dim3(256), 0, x->stream, tmp, (const float *)x->Bc[0],
+                               (const float *)x->Bc[1], (const float *)x->Bc[2], x->G);
+        else
+            hipLaunchKernelGGL(k_join3<double>, dim3(gg), dim3(256), 0, x->stream, tmp, (const double *)x->Bc[0],
+                               (const double *)x->Bc[1], (const double *)x->Bc[2], x->G);
         HIPCHK(hipMemcpyAsync(bcell, tmp, (size_t)3 * x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
         HIPCHK(hipStreamSynchronize(x->stream));
     }
-    if (eps) HIPCHK(hipMemcpyAsync(eps, x->eps, x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
-    if (cs) HIPCHK(hipMemcpyAsync(cs, x->cs, x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipStreamSynchronize(x->stream));
+    void *scal[2] = {x->eps, x->cs};
+    double *outs[2] = {eps, cs};
+    for (int i = 0; i < 2; ++i) {
+        if (!outs[i]) continue;
+        if (x->fp32) {
+            hipLaunchKernelGGL(k_widen<float>, dim3(gg), dim3(256), 0, x->stream, tmp, (const float *)scal[i], x->G);
+            HIPCHK(hipMemcpyAsync(outs[i], tmp, x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+        } else {
+            HIPCHK(hipMemcpyAsync(outs[i], scal[i], x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+        }
+        HIPCHK(hipStreamSynchronize(x->stream));
+    }
     if (tmp) HIPCHK(hipFree(tmp));
     return 0;
 }
@@ -700,8 +775,8 @@ int cumc3d_divb(cumc3d_ctx *x, double *maxdivb) {
     unsigned long long *d = nullptr, h = 0;
     HIPCHK(hipMalloc(&d, sizeof(*d)));
     HIPCHK(hipMemsetAsync(d, 0, sizeof(*d), x->stream));
-    Args a = make_args(x);
-    hipLaunchKernelGGL(k_divb, dim3(nblk((long)x->nx * x->ny * x->nz)), dim3(256), 0, x->stream, a, d);
+    hipLaunchKernelGGL(k_divb<double>, dim3(nblk((long)x->nx * x->ny * x->nz)), dim3(256), 0, x->stream,
+                       make_args<double>(x), d);
     HIPCHK(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     HIPCHK(hipFree(d));

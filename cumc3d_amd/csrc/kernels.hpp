@@ -37,22 +37,27 @@ struct Axis {
     const double *w;    // reconstruction weights, (j+3)*NW
 };
 
-struct Args {
+// Field pointers of one context.  W is the working type of the spatial
+// operator: double in the FP64 mode; float in the FP32 split, where
+// primitives, interface fluxes and EMFs are single precision and the face
+// field, u_old and the conserved state U (the RK accumulators) stay double.
+template <class W> struct ArgsT {
     int nx, ny, nz, ndim, coord, model, rt_gravity, bn_mode;
     int bc[6];
     int bfac[6][12];        // [2*axis+side][var 1..11], SETUP_EQNS parity factors
     long NX, NY, G;
     double g1, cfl, r_sh;
     Axis ax[3];
-    double *P[5];           // rho, vx, vy, vz, p
-    double *eps, *cs;
-    double *Bc[3];          // cell-centred field (bcell)
+    W *P[5];                // rho, vx, vy, vz, p
+    W *eps, *cs;
+    W *Bc[3];               // cell-centred field (bcell)
     double *Bin[3];         // face field at stage start (prim(ibx..ibz))
     double *Bout[3];        // face field written by the update
     double *Uo[8];          // u_old
-    double *F[3][7];        // interface fluxes: 5 hydro + 2 face EMFs
-    double *E[3];           // edge EMFs efield_x/y/z
-    double *Ec[3];          // cell EMFs ecell after the ghost fill, cells 0..n+1
+    double *U[5];           // conserved hydro state (FP32 split only)
+    W *F[3][7];             // interface fluxes: 5 hydro + 2 face EMFs
+    W *E[3];                // edge EMFs efield_x/y/z
+    W *Ec[3];               // cell EMFs ecell after the ghost fill, cells 0..n+1
     const double *rfl, *pfl, *efl;   // torus floors
     DevScalars *S;
     unsigned long long *blockmax;    // per-block max of double-star interface keys
@@ -65,8 +70,12 @@ __device__ __forceinline__ double ldf(const double *__restrict__ base, long idx)
     unsigned off = (unsigned)idx << 3;
     return *(const double *)((const char *)base + off);
 }
+__device__ __forceinline__ float ldf(const float *__restrict__ base, long idx) {
+    unsigned off = (unsigned)idx << 2;
+    return *(const float *)((const char *)base + off);
+}
 
-__device__ __forceinline__ long cidx(const Args &a, int j, int k, int l) {
+template <class A> __device__ __forceinline__ long cidx(const A &a, int j, int k, int l) {
     return (long)(j + 2) + a.NX * ((long)(k + 2) + a.NY * (long)(l + 2));
 }
 
@@ -89,7 +98,7 @@ __device__ __forceinline__ int gmap(int i, int n, int fin, int fout, int &side) 
 
 // parity factor of variable var for a ghost on (axis, side); constant array
 // indices only, so the kernel argument block is never copied to scratch
-__device__ __forceinline__ double gsign(const Args &a, int axis, int side, int var) {
+template <class A> __device__ __forceinline__ double gsign(const A &a, int axis, int side, int var) {
     if (side < 0) return 1.0;
     int f, b;
     if (axis == 0) { f = side ? a.bc[1] : a.bc[0]; b = side ? a.bfac[1][var] : a.bfac[0][var]; }
@@ -105,8 +114,8 @@ __device__ __forceinline__ double gsign(const Args &a, int axis, int side, int v
 // Serial position of interface (j,k,l) in the reference's Riemann loop
 // (rsolver_module.f90:596-598: l outer, j inner; the sweep axis starts at 0,
 // the others at 1), plus one.
-template <int D>
-__device__ __forceinline__ unsigned long long serial_key(const Args &a, int j, int k, int l) {
+template <int D, class A>
+__device__ __forceinline__ unsigned long long serial_key(const A &a, int j, int k, int l) {
     if constexpr (D == 0) return ((unsigned long long)(l - 1) * a.ny + (k - 1)) * (a.nx + 1) + j + 1;
     else if constexpr (D == 1) return ((unsigned long long)(l - 1) * (a.ny + 1) + k) * a.nx + (j - 1) + 1;
     else return ((unsigned long long)l * a.ny + (k - 1)) * a.nx + (j - 1) + 1;
@@ -115,10 +124,10 @@ __device__ __forceinline__ unsigned long long serial_key(const Args &a, int j, i
 // 7-variable stencil of cell c along D: rho, vx, vy, vz, p and the two
 // transverse cell-centred field components.
 template <class T, int R, int D>
-__device__ __forceinline__ void load_stencil(const Args &a, long c, long st, T s[7][5]) {
+__device__ __forceinline__ void load_stencil(const ArgsT<T> &a, long c, long st, T s[7][5]) {
     constexpr int t1 = (D == 0) ? 1 : 0;
     constexpr int t2 = (D == 2) ? 1 : 2;
-    const double *src[7] = {a.P[0], a.P[1], a.P[2], a.P[3], a.P[4], a.Bc[t1], a.Bc[t2]};
+    const T *src[7] = {a.P[0], a.P[1], a.P[2], a.P[3], a.P[4], a.Bc[t1], a.Bc[t2]};
 #pragma unroll
     for (int v = 0; v < 7; ++v) {
         if constexpr (R >= 3) {
@@ -140,7 +149,7 @@ template <class T> __device__ __forceinline__ void load_w(const double *w, T out
 template <int R> constexpr int nweights() { return R <= 2 ? 2 : (R == 3 ? 14 : 10); }
 
 template <class T, int R, int D>
-__device__ __forceinline__ void recon_at(const Args &a, int j, int k, int l, long st, Face<T> &fm, Face<T> &fp) {
+__device__ __forceinline__ void recon_at(const ArgsT<T> &a, int j, int k, int l, long st, Face<T> &fm, Face<T> &fp) {
     long c = cidx(a, j, k, l);
     T s[7][5];
     load_stencil<T, R, D>(a, c, st, s);
@@ -150,7 +159,7 @@ __device__ __forceinline__ void recon_at(const Args &a, int j, int k, int l, lon
     recon_cell<T, R>(w, s, (T)a.g1, fm, fp);
 }
 
-template <int D> __device__ __forceinline__ long stride(const Args &a) {
+template <int D, class A> __device__ __forceinline__ long stride(const A &a) {
     return D == 0 ? 1L : (D == 1 ? a.NX : a.NX * a.NY);
 }
 
@@ -158,12 +167,12 @@ template <int D> __device__ __forceinline__ long stride(const Args &a) {
 // EMFs of MHD_FLUX (mhd_module.f90:229-271).  Slot 5 holds the face EMF with
 // the lower field index, slot 6 the other.
 template <class T, int D>
-__device__ __forceinline__ void store_flux(const Args &a, long c, const RiemannOut<T> &o) {
+__device__ __forceinline__ void store_flux(const ArgsT<T> &a, long c, const RiemannOut<T> &o) {
 #pragma unroll
-    for (int i = 0; i < 5; ++i) a.F[D][i][c] = (double)o.f[i];
-    if constexpr (D == 0) { a.F[0][5][c] = (double)o.f[7]; a.F[0][6][c] = -(double)o.f[6]; }
-    else if constexpr (D == 1) { a.F[1][5][c] = -(double)o.f[7]; a.F[1][6][c] = (double)o.f[5]; }
-    else { a.F[2][5][c] = (double)o.f[6]; a.F[2][6][c] = -(double)o.f[5]; }
+    for (int i = 0; i < 5; ++i) a.F[D][i][c] = o.f[i];
+    if constexpr (D == 0) { a.F[0][5][c] = o.f[7]; a.F[0][6][c] = -o.f[6]; }
+    else if constexpr (D == 1) { a.F[1][5][c] = -o.f[7]; a.F[1][6][c] = o.f[5]; }
+    else { a.F[2][5][c] = o.f[6]; a.F[2][6][c] = -o.f[5]; }
 }
 
 __device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v) {
@@ -188,7 +197,7 @@ __device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v
 // between them, taking the right neighbour's left face with a lane shuffle.
 // Waves overlap by one cell, so every cell is reconstructed about 64/63 times.
 template <class T, int R, int S>
-__global__ __launch_bounds__(256) void k_flux_x(Args a) {
+__global__ __launch_bounds__(256) void k_flux_x(ArgsT<T> a) {
     if (!a.S->active) return;
     const int lane = threadIdx.x & 63;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -237,7 +246,7 @@ __global__ __launch_bounds__(256) void k_flux_x(Args a) {
 // the sweep axis over a segment of interfaces [i0,i1), reconstructing cells
 // i0..i1 once and keeping the previous cell's right face in registers.
 template <class T, int R, int S, int D>
-__global__ __launch_bounds__(256) void k_flux_yz(Args a, int nseg) {
+__global__ __launch_bounds__(256) void k_flux_yz(ArgsT<T> a, int nseg) {
     if (!a.S->active) return;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int na = (D == 1) ? a.nz : a.ny;    // the other transverse axis
@@ -289,7 +298,7 @@ __global__ __launch_bounds__(256) void k_flux_yz(Args a, int nseg) {
 // reference leaves behind in usstarL/usstarR and reads back as usstar(ibn) in
 // later sweeps (rsolver_module.f90:704-741).
 template <class T, int R, int D>
-__global__ __launch_bounds__(1024) void k_carry(Args a, int nblocks) {
+__global__ __launch_bounds__(1024) void k_carry(ArgsT<T> a, int nblocks) {
     if (!a.S->active) return;
     unsigned long long key = 0;
     for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
@@ -327,9 +336,9 @@ __global__ __launch_bounds__(1024) void k_carry(Args a, int nblocks) {
 // ---------------------------------------------------------------------------
 
 // cell-centred EMF -v x B of an interior cell (mhd_module.f90:316-318)
-template <int E> __device__ __forceinline__ double ecell_raw(const Args &a, long c) {
-    double vx = a.P[1][c], vy = a.P[2][c], vz = a.P[3][c];
-    double bx = a.Bc[0][c], by = a.Bc[1][c], bz = a.Bc[2][c];
+template <int E, class W> __device__ __forceinline__ W ecell_raw(const ArgsT<W> &a, long c) {
+    W vx = a.P[1][c], vy = a.P[2][c], vz = a.P[3][c];
+    W bx = a.Bc[0][c], by = a.Bc[1][c], bz = a.Bc[2][c];
     if constexpr (E == 0) return vz * by - vy * bz;
     else if constexpr (E == 1) return vx * bz - vz * bx;
     else return vy * bx - vx * by;
@@ -337,25 +346,25 @@ template <int E> __device__ __forceinline__ double ecell_raw(const Args &a, long
 
 // ecell after BOUNDARY_EFIELD (mhd_module.f90:478-840): ghost cells take the
 // mapped interior value times the parity factors of component E.
-template <int E> __device__ __forceinline__ double ecell_at(const Args &a, int j, int k, int l) {
+template <int E, class W> __device__ __forceinline__ W ecell_at(const ArgsT<W> &a, int j, int k, int l) {
     int sx, sy, sz;
     int jj = gmap(j, a.nx, a.bc[0], a.bc[1], sx);
     int kk = gmap(k, a.ny, a.bc[2], a.bc[3], sy);
     int ll = gmap(l, a.nz, a.bc[4], a.bc[5], sz);
-    double s = gsign(a, 0, sx, 9 + E) * gsign(a, 1, sy, 9 + E) * gsign(a, 2, sz, 9 + E);
-    double v = ecell_raw<E>(a, cidx(a, jj, kk, ll));
+    W s = (W)(gsign(a, 0, sx, 9 + E) * gsign(a, 1, sy, 9 + E) * gsign(a, 2, sz, 9 + E));
+    W v = ecell_raw<E>(a, cidx(a, jj, kk, ll));
     return (sx < 0 && sy < 0 && sz < 0) ? v : s * v;
 }
 
 // Face EMF eface of component E at faces normal to D, after the 1D/2D fill
 // (mhd_module.f90:328-356) and BOUNDARY_EFIELD: face index along D is real,
 // the two cell-centred axes are ghost-mapped.
-template <int E, int D> __device__ __forceinline__ double eface_at(const Args &a, int j, int k, int l) {
+template <int E, int D, class W> __device__ __forceinline__ W eface_at(const ArgsT<W> &a, int j, int k, int l) {
     int sx = -1, sy = -1, sz = -1;
     if constexpr (D != 0) j = gmap(j, a.nx, a.bc[0], a.bc[1], sx);
     if constexpr (D != 1) k = gmap(k, a.ny, a.bc[2], a.bc[3], sy);
     if constexpr (D != 2) l = gmap(l, a.nz, a.bc[4], a.bc[5], sz);
-    double v;
+    W v;
     if (D < a.ndim) {
         constexpr int slot = (D == 0) ? (E == 1 ? 5 : 6) : (D == 1 ? (E == 0 ? 5 : 6) : (E == 0 ? 5 : 6));
         v = a.F[D][slot][cidx(a, j, k, l)];
@@ -366,14 +375,14 @@ template <int E, int D> __device__ __forceinline__ double eface_at(const Args &a
         v = ecell_raw<E>(a, cidx(a, j, k, l));
     }
     if (sx < 0 && sy < 0 && sz < 0) return v;
-    return (gsign(a, 0, sx, 9 + E) * gsign(a, 1, sy, 9 + E) * gsign(a, 2, sz, 9 + E)) * v;
+    return (W)(gsign(a, 0, sx, 9 + E) * gsign(a, 1, sy, 9 + E) * gsign(a, 2, sz, 9 + E)) * v;
 }
 
 // Cell and face EMFs on the cells 0..n+1 the edge stencil reads (flux_ct,
 // mhd_module.f90:310-360): ecell of every cell (ghosts mapped as in
 // BOUNDARY_EFIELD) and, for face EMFs, the ghost rows along their two
 // cell-centred axes plus the faces of inactive directions (1D/2D fill).
-__global__ __launch_bounds__(256) void k_emf_prep(Args a) {
+template <class W> __global__ __launch_bounds__(256) void k_emf_prep(ArgsT<W> a) {
     if (!a.S->active) return;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n0 = a.nx + 2, n1 = a.ny + 2;
@@ -402,28 +411,26 @@ __global__ __launch_bounds__(256) void k_emf_prep(Args a) {
 // Edge EMFs (mhd_module.f90:414-423) on the edges the CT update reads:
 // efield_x for j=1..nx, efield_y for k=1..ny, efield_z for l=1..nz
 // (0..n along the other axes), from the arrays k_emf_prep completed.
-__global__ __launch_bounds__(256) void k_edge(Args a) {
+template <class W> __global__ __launch_bounds__(256) void k_edge(ArgsT<W> a) {
     if (!a.S->active) return;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n1 = a.nx + 1, n2 = a.ny + 1;
     if (t >= n1 * n2 * (a.nz + 1)) return;
     const int j = (int)(t % n1), k = (int)((t / n1) % n2), l = (int)(t / (n1 * n2));
     const long c = cidx(a, j, k, l), sy = a.NX, sz = a.NX * a.NY;
-    const double *ex = a.Ec[0], *ey = a.Ec[1], *ez = a.Ec[2];
+    const W *ex = a.Ec[0], *ey = a.Ec[1], *ez = a.Ec[2];
+    const W h = W(0.50), q = W(0.25);
     if (j >= 1) {
-        const double *fxy = a.F[1][5], *fxz = a.F[2][5];
-        a.E[0][c] = 0.50 * (fxy[c] + fxy[c + sz] + fxz[c] + fxz[c + sy]) -
-                    0.25 * (ex[c] + ex[c + sz] + ex[c + sy] + ex[c + sy + sz]);
+        const W *fxy = a.F[1][5], *fxz = a.F[2][5];
+        a.E[0][c] = h * (fxy[c] + fxy[c + sz] + fxz[c] + fxz[c + sy]) - q * (ex[c] + ex[c + sz] + ex[c + sy] + ex[c + sy + sz]);
     }
     if (k >= 1) {
-        const double *fyx = a.F[0][5], *fyz = a.F[2][6];
-        a.E[1][c] = 0.50 * (fyx[c] + fyx[c + sz] + fyz[c] + fyz[c + 1]) -
-                    0.25 * (ey[c] + ey[c + sz] + ey[c + 1] + ey[c + 1 + sz]);
+        const W *fyx = a.F[0][5], *fyz = a.F[2][6];
+        a.E[1][c] = h * (fyx[c] + fyx[c + sz] + fyz[c] + fyz[c + 1]) - q * (ey[c] + ey[c + sz] + ey[c + 1] + ey[c + 1 + sz]);
     }
     if (l >= 1) {
-        const double *fzx = a.F[0][6], *fzy = a.F[1][6];
-        a.E[2][c] = 0.50 * (fzx[c] + fzx[c + sy] + fzy[c] + fzy[c + 1]) -
-                    0.25 * (ez[c] + ez[c + sy] + ez[c + 1] + ez[c + 1 + sy]);
+        const W *fzx = a.F[0][6], *fzy = a.F[1][6];
+        a.E[2][c] = h * (fzx[c] + fzx[c + sy] + fzy[c] + fzy[c + 1]) - q * (ez[c] + ez[c + sy] + ez[c + 1] + ez[c + 1 + sy]);
     }
 }
 
@@ -432,29 +439,30 @@ __global__ __launch_bounds__(256) void k_edge(Args a) {
 // CUSTOM_CHECKRHO + FROMRVETOU + FINDPRESSURE (+ FINDDT) for one interior cell
 // ---------------------------------------------------------------------------
 
-// l_rk of the face fields (l_operator, spatial.f90:639-680)
-__device__ __forceinline__ double lB_x(const Args &a, int jf, int k, int l) {
-    const double *ey = a.E[1], *ez = a.E[2];
+// l_rk of the face fields (l_operator, spatial.f90:639-680); the edge EMFs
+// are read in the working type and differenced in double
+template <class A> __device__ __forceinline__ double lB_x(const A &a, int jf, int k, int l) {
     long c = cidx(a, jf, k, l), sy = a.NX, sz = a.NX * a.NY;
+    const double ez0 = a.E[2][c], ez1 = a.E[2][c - sy], ey0 = a.E[1][c], ey1 = a.E[1][c - sz];
     double dy = a.ax[1].d[k + 3], dz = a.ax[2].d[l + 3];
-    if (a.coord == 0) return -((ez[c] - ez[c - sy]) / (dy) - (ey[c] - ey[c - sz]) / (dz));
+    if (a.coord == 0) return -((ez0 - ez1) / (dy) - (ey0 - ey1) / (dz));
     double xF = a.ax[0].xF[jf + 3];
-    return -(ez[c] - ez[c - sy]) / (xF * dy) + (ey[c] - ey[c - sz]) / (dz);
+    return -(ez0 - ez1) / (xF * dy) + (ey0 - ey1) / (dz);
 }
-__device__ __forceinline__ double lB_y(const Args &a, int j, int kf, int l) {
-    const double *ex = a.E[0], *ez = a.E[2];
+template <class A> __device__ __forceinline__ double lB_y(const A &a, int j, int kf, int l) {
     long c = cidx(a, j, kf, l), sz = a.NX * a.NY;
+    const double ex0 = a.E[0][c], ex1 = a.E[0][c - sz], ez0 = a.E[2][c], ez1 = a.E[2][c - 1];
     double dx = a.ax[0].d[j + 3], dz = a.ax[2].d[l + 3];
-    if (a.coord == 0) return -((ex[c] - ex[c - sz]) / (dz) - (ez[c] - ez[c - 1]) / (dx));
-    return -(ex[c] - ex[c - sz]) / (dz) + (ez[c] - ez[c - 1]) / (dx);
+    if (a.coord == 0) return -((ex0 - ex1) / (dz) - (ez0 - ez1) / (dx));
+    return -(ex0 - ex1) / (dz) + (ez0 - ez1) / (dx);
 }
-__device__ __forceinline__ double lB_z(const Args &a, int j, int k, int lf) {
-    const double *ex = a.E[0], *ey = a.E[1];
+template <class A> __device__ __forceinline__ double lB_z(const A &a, int j, int k, int lf) {
     long c = cidx(a, j, k, lf), sy = a.NX;
+    const double ey0 = a.E[1][c], ey1 = a.E[1][c - 1], ex0 = a.E[0][c], ex1 = a.E[0][c - sy];
     double dx = a.ax[0].d[j + 3], dy = a.ax[1].d[k + 3];
-    if (a.coord == 0) return -((ey[c] - ey[c - 1]) / (dx) - (ex[c] - ex[c - sy]) / (dy));
+    if (a.coord == 0) return -((ey0 - ey1) / (dx) - (ex0 - ex1) / (dy));
     double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2], x = a.ax[0].c[j + 3];
-    return -(xF * ey[c] - xFm * ey[c - 1]) / (x * dx) + (ex[c] - ex[c - sy]) / (x * dy);
+    return -(xF * ey0 - xFm * ey1) / (x * dx) + (ex0 - ex1) / (x * dy);
 }
 
 template <int STAGE>
@@ -466,7 +474,7 @@ __device__ __forceinline__ double rk(double uo, double u, double dt, double L) {
 
 // CUSTOM_CHECKRHO of the model (model/riemann/src/custom.f90:115-121,
 // model/torus-cyn/src/custom.f90:254-271)
-__device__ __forceinline__ void checkrho(const Args &a, long c, double &rho, double v[3], double &eps) {
+template <class A> __device__ __forceinline__ void checkrho(const A &a, long c, double &rho, double v[3], double &eps) {
     if (a.model == 0) {
         eps = fmax2(eps, 0.0);
     } else {
@@ -481,7 +489,7 @@ __device__ __forceinline__ void checkrho(const Args &a, long c, double &rho, dou
 
 // FINDPRESSURE of the model (model/riemann/src/findpressure.f90:17-28,
 // model/torus-cyn/src/findpressure.f90:97-112)
-__device__ __forceinline__ void pressure(const Args &a, long c, double rho, double eps, double &p, double &cs) {
+template <class A> __device__ __forceinline__ void pressure(const A &a, long c, double rho, double eps, double &p, double &cs) {
     double dpdrho, dpdeps;
     if (a.model == 0 || rho > a.rfl[c]) {
         p = rho * eps * a.g1;
@@ -496,7 +504,7 @@ __device__ __forceinline__ void pressure(const Args &a, long c, double rho, doub
 }
 
 // FINDDT per cell (rungekutta.f90:272-311)
-__device__ __forceinline__ double cfl_dt(const Args &a, int j, int k, int l, double cs, double rho, const double v[3],
+template <class A> __device__ __forceinline__ double cfl_dt(const A &a, int j, int k, int l, double cs, double rho, const double v[3],
                                          const double b[3]) {
     double a2 = cs * cs, a4 = a2 * a2;
     double b2x = (b[0] * b[0] / rho), b2y = (b[1] * b[1] / rho), b2z = (b[2] * b[2] / rho);
@@ -540,8 +548,8 @@ __device__ __forceinline__ double block_min_f64(double v, DevScalars *S) {
 }
 
 // geometry-weighted cell average of face field component D (fromrvetou.f90:153-195)
-template <int D>
-__device__ __forceinline__ double bavg(const Args &a, int j, double up, double lo) {
+template <int D, class A>
+__device__ __forceinline__ double bavg(const A &a, int j, double up, double lo) {
     if (D == 0 && a.coord == 1) {
         double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2];
         return (xF * up + xFm * lo) / (xF + xFm);
@@ -549,8 +557,9 @@ __device__ __forceinline__ double bavg(const Args &a, int j, double up, double l
     return 0.5 * (up + lo);
 }
 
-template <int STAGE>
-__global__ __launch_bounds__(256) void k_update(Args a) {
+template <int STAGE, class W>
+__global__ __launch_bounds__(256) void k_update(ArgsT<W> a) {
+    constexpr bool split = sizeof(W) == 4;   // FP32 split: U holds the conserved state
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nn = (long)a.nx * a.ny * a.nz;
     if (!a.S->active) {
@@ -571,9 +580,11 @@ __global__ __launch_bounds__(256) void k_update(Args a) {
         // stage-start primitives and conserved variables (FROMRVETOU, fromrvetou.f90:428-438)
         double rho = a.P[0][c], vx = a.P[1][c], vy = a.P[2][c], vz = a.P[3][c];
         double bx = a.Bc[0][c], by = a.Bc[1][c], bz = a.Bc[2][c];
-        double eps = a.eps[c];
         double u[5], uo[5];
-        {
+        if constexpr (split) {
+            for (int i = 0; i < 5; ++i) u[i] = a.U[i][c];
+        } else {
+            double eps = a.eps[c];
             double vsq = vx * vx + vy * vy + vz * vz, bsq = bx * bx + by * by + bz * bz;
             u[0] = rho;
             u[1] = vx * rho;
@@ -610,27 +621,28 @@ __global__ __launch_bounds__(256) void k_update(Args a) {
         double L[5];
         {
             double dx = a.ax[0].d[j + 3];
+            // fluxes are read in the working type and differenced in double
             for (int i = 0; i < 5; ++i) {
-                const double *f = a.F[0][i];
+                const double f1 = a.F[0][i][c], f0 = a.F[0][i][c - 1];
                 double df;
                 if (a.coord == 1)
-                    df = (a.ax[0].xF[j + 3] * f[c] - a.ax[0].xF[j + 2] * f[c - 1]) / (a.ax[0].c[j + 3] * dx);
-                else df = (f[c] - f[c - 1]) / (dx);
+                    df = (a.ax[0].xF[j + 3] * f1 - a.ax[0].xF[j + 2] * f0) / (a.ax[0].c[j + 3] * dx);
+                else df = (f1 - f0) / (dx);
                 L[i] = -df;
             }
             if (a.ndim > 1) {
                 double dy = a.ax[1].d[k + 3];
                 for (int i = 0; i < 5; ++i) {
-                    const double *f = a.F[1][i];
-                    double df = (a.coord == 1) ? (f[c] - f[c - sy]) / (a.ax[0].c[j + 3] * dy) : (f[c] - f[c - sy]) / (dy);
+                    const double f1 = a.F[1][i][c], f0 = a.F[1][i][c - sy];
+                    double df = (a.coord == 1) ? (f1 - f0) / (a.ax[0].c[j + 3] * dy) : (f1 - f0) / (dy);
                     L[i] = L[i] - df;
                 }
             }
             if (a.ndim > 2) {
                 double dz = a.ax[2].d[l + 3];
                 for (int i = 0; i < 5; ++i) {
-                    const double *f = a.F[2][i];
-                    L[i] = L[i] - (f[c] - f[c - sz]) / dz;
+                    const double f1 = a.F[2][i][c], f0 = a.F[2][i][c - sz];
+                    L[i] = L[i] - (f1 - f0) / dz;
                 }
             }
             for (int i = 0; i < 5; ++i) L[i] = L[i] + sc[i];
@@ -678,16 +690,26 @@ __global__ __launch_bounds__(256) void k_update(Args a) {
                          bavg<2>(a, j, bn_up[2], bprim_lo[2])};
         double p, cs;
         pressure(a, c, rn, en, p, cs);
-        a.P[0][c] = rn;
-        a.P[1][c] = v[0];
-        a.P[2][c] = v[1];
-        a.P[3][c] = v[2];
-        a.P[4][c] = p;
-        a.eps[c] = en;
-        a.cs[c] = cs;
+        a.P[0][c] = (W)rn;
+        a.P[1][c] = (W)v[0];
+        a.P[2][c] = (W)v[1];
+        a.P[3][c] = (W)v[2];
+        a.P[4][c] = (W)p;
+        a.eps[c] = (W)en;
+        a.cs[c] = (W)cs;
         for (int d = 0; d < 3; ++d) {
-            a.Bc[d][c] = bcf[d];
+            a.Bc[d][c] = (W)bcf[d];
             a.Bout[d][c] = bn_up[d];
+        }
+        if constexpr (split) {
+            // FROMRVETOU of the double-precision primitives: the next stage's U
+            double vsq2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            double bsq2 = bcf[0] * bcf[0] + bcf[1] * bcf[1] + bcf[2] * bcf[2];
+            a.U[0][c] = rn;
+            a.U[1][c] = v[0] * rn;
+            a.U[2][c] = v[1] * rn;
+            a.U[3][c] = v[2] * rn;
+            a.U[4][c] = rn * (en + 0.5 * vsq2) + 0.5 * bsq2;
         }
         if (STAGE == 3) dtloc = cfl_dt(a, j, k, l, cs, rn, v, bcf);
     }
@@ -701,7 +723,7 @@ __global__ __launch_bounds__(256) void k_update(Args a) {
 // ghost fill: BOUNDARYP_NM + BOUNDARY1D_NM(epsilon) + custom BCs, then
 // FINDPRESSURE on the ghost cells
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_ghost(Args a) {
+template <class W> __global__ __launch_bounds__(256) void k_ghost(ArgsT<W> a) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.G) return;
     const int j = (int)(t % a.NX) - 2, k = (int)((t / a.NX) % a.NY) - 2, l = (int)(t / (a.NX * a.NY)) - 2;
@@ -725,7 +747,7 @@ __global__ __launch_bounds__(256) void k_ghost(Args a) {
     for (int d = 0; d < 3; ++d)
         v[d] = a.P[1 + d][cs_] * (gsign(a, 0, sx, 2 + d) * gsign(a, 1, sy, 2 + d) * gsign(a, 2, sz, 2 + d));
     for (int d = 0; d < 3; ++d)
-        a.Bc[d][c] = a.Bc[d][cs_] * (gsign(a, 0, sx, 6 + d) * gsign(a, 1, sy, 6 + d) * gsign(a, 2, sz, 6 + d));
+        a.Bc[d][c] = (W)(a.Bc[d][cs_] * (gsign(a, 0, sx, 6 + d) * gsign(a, 1, sy, 6 + d) * gsign(a, 2, sz, 6 + d)));
     // epsilon: BOUNDARY1D_NM takes the y-outer branch from boundary_flag(3) (boundary.f90:243)
     {
         int ey = kk;
@@ -751,14 +773,14 @@ __global__ __launch_bounds__(256) void k_ghost(Args a) {
         if (sz == 0) v[2] = fmin2(v[2], 0.0);
         if (sz == 1) v[2] = fmax2(v[2], 0.0);
     }
-    a.P[0][c] = rho;
-    a.P[1][c] = v[0];
-    a.P[2][c] = v[1];
-    a.P[3][c] = v[2];
+    a.P[0][c] = (W)rho;
+    a.P[1][c] = (W)v[0];
+    a.P[2][c] = (W)v[1];
+    a.P[3][c] = (W)v[2];
     double p, cs;
-    pressure(a, c, rho, a.eps[c], p, cs);
-    a.P[4][c] = p;
-    a.cs[c] = cs;
+    pressure(a, c, rho, (double)a.eps[c], p, cs);
+    a.P[4][c] = (W)p;
+    a.cs[c] = (W)cs;
 }
 
 }  // namespace cumc3d

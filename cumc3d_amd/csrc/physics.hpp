@@ -111,16 +111,25 @@ __device__ __forceinline__ void recon_ppm(const T *__restrict__ w, T vm2, T vm1,
 }
 
 // WENO-Z, q=2, eps=1e-40 (weno_module.f90:315-408); w = 10 weights of
-// WENO_INTERPOLANT.
+// WENO_INTERPOLANT.  In single precision eps is 1e-30 (1e-40 is below the
+// normal range) and tau5/(eps+beta) is capped at 1e18 so that its square
+// stays finite; both change nothing unless a smoothness indicator is within
+// 18 decades of zero while tau5 is not.
+template <class T> __device__ __forceinline__ T weno_ratio(T tau5, T sp, T b) {
+    T q = tau5 / (sp + b);
+    if constexpr (sizeof(T) == 4) q = fmin2(q, T(1.0e18));
+    return q;
+}
 template <class T>
 __device__ __forceinline__ T weno_face(T P1, T P2, T P3, T B1, T B2, T B3) {
-    const T sp = T(1.0e-40), g1 = T(1.0 / 10.0), g2 = T(6.0 / 10.0), g3 = T(3.0 / 10.0);
+    const T sp = sizeof(T) == 4 ? T(1.0e-30) : T(1.0e-40);
+    const T g1 = T(1.0 / 10.0), g2 = T(6.0 / 10.0), g3 = T(3.0 / 10.0);
     T tau5 = fabsT(B1 - B3);
-    T q = tau5 / (sp + B1);
+    T q = weno_ratio(tau5, sp, B1);
     T w1 = g1 * (T(1) + q * q);
-    q = tau5 / (sp + B2);
+    q = weno_ratio(tau5, sp, B2);
     T w2 = g2 * (T(1) + q * q);
-    q = tau5 / (sp + B3);
+    q = weno_ratio(tau5, sp, B3);
     T w3 = g3 * (T(1) + q * q);
     return (w1 * P1 + w2 * P2 + w3 * P3) / (w1 + w2 + w3);
 }
