@@ -11,6 +11,7 @@
 
 #include "../../include/cumc3d_hip.h"
 #include "kernels.hpp"
+#include "selector.hpp"
 
 using namespace cumc3d;
 
@@ -45,6 +46,17 @@ struct cumc3d_ctx {
     double *B[2][3], *Uo[8], *U[5], *floors[3];
     int cur = 0;                  // B[cur] holds the current face field
     long march_threads = 1L << 19;  // target thread count of the y/z sweeps
+    // learned reconstruction selector
+    bool sel_on = false;
+    int sel_every = 1, sel_tick = 0;
+    float *sel_w = nullptr, *sel_m = nullptr, *sel_v = nullptr;   // FP32 parameters + Adam moments
+    unsigned short *sel_frag = nullptr;                            // BF16 MFMA fragments
+    unsigned char *mask = nullptr;                                 // 3 x G class mask
+    float *samp_feat = nullptr;
+    unsigned char *samp_label = nullptr;
+    int samp_cap = 0, *samp_counts = nullptr, adam_step = 0;
+    float *grad_part = nullptr, *loss_part = nullptr, *loss_dev = nullptr;
+    int grad_blocks = 0;
     int bfac[6][12];
     Axis ax[3];
     bool uploaded = false, initialized = false, dt_pending = false;
@@ -237,6 +249,7 @@ template <class W> static ArgsT<W> make_args(cumc3d_ctx *x) {
     a.efl = x->floors[2];
     a.S = x->S;
     a.blockmax = x->blockmax;
+    for (int d = 0; d < 3; ++d) a.mask[d] = x->mask ? x->mask + (size_t)d * x->G : nullptr;
     return a;
 }
 
@@ -367,6 +380,10 @@ static void launch_sweep_r(cumc3d_ctx *x, const ArgsT<W> &a, int D, hipStream_t 
 }
 
 template <class W> static void launch_sweep(cumc3d_ctx *x, const ArgsT<W> &a, int D, hipStream_t st) {
+    if (x->sel_on) {
+        launch_sweep_r<W, RSEL>(x, a, D, st);
+        return;
+    }
     switch (x->d.recon) {
         case 0: launch_sweep_r<W, 0>(x, a, D, st); break;
         case 1: launch_sweep_r<W, 1>(x, a, D, st); break;
@@ -418,7 +435,26 @@ template <class W> static void run_stage_t(cumc3d_ctx *x, int stage) {
     if (x->timing && x->tstage < TSTAGES) x->tstage += 1;
 }
 
+template <class W> static void sel_infer_t(cumc3d_ctx *x) {
+    ArgsT<W> a = make_args<W>(x);
+    for (int D = 0; D < x->d.n_dim; ++D) {
+        unsigned char *m = x->mask + (size_t)D * x->G;
+        if (D == 0) hipLaunchKernelGGL((k_sel_infer<W, 0>), dim3(2048), dim3(256), 0, x->stream, a, x->sel_frag, x->sel_w, m);
+        else if (D == 1) hipLaunchKernelGGL((k_sel_infer<W, 1>), dim3(2048), dim3(256), 0, x->stream, a, x->sel_frag, x->sel_w, m);
+        else hipLaunchKernelGGL((k_sel_infer<W, 2>), dim3(2048), dim3(256), 0, x->stream, a, x->sel_frag, x->sel_w, m);
+    }
+}
+
+// class masks from the stage-1 state of a step (kept for stages 2 and 3)
+static void sel_refresh(cumc3d_ctx *x) {
+    if (!x->sel_on) return;
+    if (x->sel_tick++ % x->sel_every != 0) return;
+    if (x->fp32) sel_infer_t<float>(x);
+    else sel_infer_t<double>(x);
+}
+
 static void run_stage(cumc3d_ctx *x, int stage) {
+    if (stage == 1) sel_refresh(x);
     if (x->fp32) run_stage_t<float>(x, stage);
     else run_stage_t<double>(x, stage);
 }
@@ -505,7 +541,7 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
     // grid metrics: GetGrid (getgrid.f90:43-63) from the face coordinates
     const double *Fc[3] = {d.xF, d.yF, d.zF};
     std::vector<double> hm;
-    long off[3][4];
+    long off[3][7];
     for (int a = 0; a < 3; ++a) {
         int n = nd3[a];
         std::vector<double> f(Fc[a], Fc[a] + n + 7), dd(n + 7, 0.0), cc(n + 7, 0.0), w;
@@ -518,6 +554,11 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         off[a][1] = (long)hm.size(); hm.insert(hm.end(), dd.begin(), dd.end());
         off[a][2] = (long)hm.size(); hm.insert(hm.end(), cc.begin(), cc.end());
         off[a][3] = (long)hm.size(); hm.insert(hm.end(), w.begin(), w.end());
+        // the selector's three schemes: TVD-MC, PPM, WENO-Z
+        for (int sch = 0; sch < 3; ++sch) {
+            weights(sch == 0 ? 2 : (sch == 1 ? 3 : 4), dd, n, w);
+            off[a][4 + sch] = (long)hm.size(); hm.insert(hm.end(), w.begin(), w.end());
+        }
     }
     int rc = 0;
     auto cleanup = [&]() { cumc3d_finalize(x); };
@@ -535,6 +576,7 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         x->ax[a].d = x->met + off[a][1];
         x->ax[a].c = x->met + off[a][2];
         x->ax[a].w = x->met + off[a][3];
+        for (int sch = 0; sch < 3; ++sch) x->ax[a].wsel[sch] = x->met + off[a][4 + sch];
     }
     // per-cell fields: the working-type block, then the double block, each
     // field G entries, fields 256-byte aligned
@@ -792,8 +834,198 @@ int cumc3d_finalize(cumc3d_ctx *x) {
     if (x->met) hipFree(x->met);
     if (x->S) hipFree(x->S);
     if (x->blockmax) hipFree(x->blockmax);
+    void *sel[] = {x->sel_w, x->sel_m, x->sel_v, x->sel_frag, x->mask, x->samp_feat, x->samp_label,
+                   x->samp_counts, x->grad_part, x->loss_part, x->loss_dev};
+    for (void *q : sel)
+        if (q) hipFree(q);
     if (x->stream) hipStreamDestroy(x->stream);
     delete x;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// learned reconstruction selector
+// ---------------------------------------------------------------------------
+static int sel_pack(cumc3d_ctx *x) {
+    hipLaunchKernelGGL(k_sel_pack, dim3(1), dim3(64 * SEL_NFRAG), 0, x->stream, x->sel_w, x->sel_frag);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int cumc3d_selector_init(cumc3d_ctx *x, int capacity, unsigned seed) {
+    if (!x) return fail(CUMC3D_EINVAL, "null context");
+    if (capacity < SEL_TB) return fail(CUMC3D_EINVAL, "sample capacity must be at least 128");
+    if (!x->sel_w) {
+        HIPCHK(hipMalloc(&x->sel_w, SEL_NPARAM * sizeof(float)));
+        HIPCHK(hipMalloc(&x->sel_m, SEL_NPARAM * sizeof(float)));
+        HIPCHK(hipMalloc(&x->sel_v, SEL_NPARAM * sizeof(float)));
+        HIPCHK(hipMalloc(&x->sel_frag, (size_t)64 * SEL_NFRAG * 4 * sizeof(unsigned short)));
+        HIPCHK(hipMalloc(&x->mask, (size_t)3 * x->G));
+        HIPCHK(hipMalloc(&x->samp_counts, 4 * sizeof(int)));
+        HIPCHK(hipMalloc(&x->loss_dev, sizeof(float)));
+    }
+    if (x->samp_cap != capacity) {
+        if (x->samp_feat) HIPCHK(hipFree(x->samp_feat));
+        if (x->samp_label) HIPCHK(hipFree(x->samp_label));
+        HIPCHK(hipMalloc(&x->samp_feat, (size_t)capacity * SEL_IN * sizeof(float)));
+        HIPCHK(hipMalloc(&x->samp_label, (size_t)capacity));
+        x->samp_cap = capacity;
+    }
+    // He-uniform initial weights from a fixed-seed generator, zero biases
+    std::vector<float> w(SEL_NPARAM, 0.0f);
+    unsigned long long st = 0x9E3779B97F4A7C15ULL ^ seed;
+    auto uni = [&]() {
+        st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+        return (float)((st >> 11) * (1.0 / 9007199254740992.0)) * 2.0f - 1.0f;
+    };
+    for (int i = 0; i < SEL_IN * SEL_H; ++i) w[SEL_W1 + i] = uni() * sqrtf(6.0f / 15.0f);
+    for (int i = 0; i < SEL_H * SEL_H; ++i) w[SEL_W2 + i] = uni() * sqrtf(6.0f / SEL_H);
+    for (int i = 0; i < SEL_H * SEL_OUT; ++i) w[SEL_W3 + i] = uni() * sqrtf(6.0f / SEL_H);
+    std::vector<float> zero(SEL_NPARAM, 0.0f);
+    HIPCHK(hipMemcpy(x->sel_w, w.data(), SEL_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(x->sel_m, zero.data(), SEL_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(x->sel_v, zero.data(), SEL_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(x->samp_counts, 0, 4 * sizeof(int)));
+    HIPCHK(hipMemset(x->mask, 2, (size_t)3 * x->G));   // WENO-Z until the first inference
+    x->adam_step = 0;
+    if (int rc = sel_pack(x)) return rc;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+int cumc3d_selector_set_params(cumc3d_ctx *x, const float *w, int n) {
+    if (!x || !w) return fail(CUMC3D_EINVAL, "null argument");
+    if (!x->sel_w) return fail(CUMC3D_ESTATE, "call cumc3d_selector_init first");
+    if (n != SEL_NPARAM) return fail(CUMC3D_EINVAL, "the selector has " + std::to_string(SEL_NPARAM) + " parameters");
+    HIPCHK(hipMemcpy(x->sel_w, w, SEL_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    if (int rc = sel_pack(x)) return rc;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+int cumc3d_selector_get_params(cumc3d_ctx *x, float *w, int n) {
+    if (!x || !w) return fail(CUMC3D_EINVAL, "null argument");
+    if (!x->sel_w) return fail(CUMC3D_ESTATE, "call cumc3d_selector_init first");
+    if (n != SEL_NPARAM) return fail(CUMC3D_EINVAL, "the selector has " + std::to_string(SEL_NPARAM) + " parameters");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(w, x->sel_w, SEL_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // extern "C"
+
+template <class W> static void sel_collect_t(cumc3d_ctx *x, int stride, unsigned seed, float t1, float t2) {
+    ArgsT<W> a = make_args<W>(x);
+    for (int D = 0; D < x->d.n_dim; ++D) {
+        long nq = (long)(x->nx + (D == 0 ? 2 : 0)) * (x->ny + (D == 1 ? 2 : 0)) * (x->nz + (D == 2 ? 2 : 0));
+        unsigned g = nblk((nq + stride - 1) / stride);
+        unsigned sd = seed * 3u + (unsigned)D;
+        if (D == 0)
+            hipLaunchKernelGGL((k_sel_collect<W, 0>), dim3(g), dim3(256), 0, x->stream, a, stride, sd, t1, t2,
+                               x->samp_feat, x->samp_label, x->samp_cap, x->samp_counts);
+        else if (D == 1)
+            hipLaunchKernelGGL((k_sel_collect<W, 1>), dim3(g), dim3(256), 0, x->stream, a, stride, sd, t1, t2,
+                               x->samp_feat, x->samp_label, x->samp_cap, x->samp_counts);
+        else
+            hipLaunchKernelGGL((k_sel_collect<W, 2>), dim3(g), dim3(256), 0, x->stream, a, stride, sd, t1, t2,
+                               x->samp_feat, x->samp_label, x->samp_cap, x->samp_counts);
+    }
+}
+
+extern "C" {
+
+int cumc3d_selector_collect(cumc3d_ctx *x, int stride, double thr_ppm, double thr_weno, unsigned seed, int *counts) {
+    if (!x) return fail(CUMC3D_EINVAL, "null context");
+    if (!x->sel_w) return fail(CUMC3D_ESTATE, "call cumc3d_selector_init first");
+    if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
+    if (stride < 1) return fail(CUMC3D_EINVAL, "stride must be >= 1");
+    if (x->fp32) sel_collect_t<float>(x, stride, seed, (float)thr_ppm, (float)thr_weno);
+    else sel_collect_t<double>(x, stride, seed, (float)thr_ppm, (float)thr_weno);
+    HIPCHK(hipGetLastError());
+    int h[4];
+    HIPCHK(hipMemcpyAsync(h, x->samp_counts, sizeof(h), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (h[0] > x->samp_cap) {
+        // the buffer is full: later slots were dropped, the histogram counted them
+        h[0] = x->samp_cap;
+    }
+    if (counts) memcpy(counts, h, sizeof(h));
+    return 0;
+}
+
+int cumc3d_selector_train(cumc3d_ctx *x, int iters, int blocks, double lr, const double *class_weight, unsigned seed,
+                          float *loss) {
+    if (!x) return fail(CUMC3D_EINVAL, "null context");
+    if (!x->sel_w) return fail(CUMC3D_ESTATE, "call cumc3d_selector_init first");
+    if (iters < 0 || blocks < 1 || blocks > 65536) return fail(CUMC3D_EINVAL, "bad iteration or block count");
+    int h[4];
+    HIPCHK(hipMemcpy(h, x->samp_counts, sizeof(h), hipMemcpyDeviceToHost));
+    int n = h[0] < x->samp_cap ? h[0] : x->samp_cap;
+    if (n < 1) return fail(CUMC3D_ESTATE, "no training samples: call cumc3d_selector_collect first");
+    if (x->grad_blocks < blocks) {
+        if (x->grad_part) HIPCHK(hipFree(x->grad_part));
+        if (x->loss_part) HIPCHK(hipFree(x->loss_part));
+        HIPCHK(hipMalloc(&x->grad_part, (size_t)blocks * SEL_NPARAM * sizeof(float)));
+        HIPCHK(hipMalloc(&x->loss_part, (size_t)blocks * sizeof(float)));
+        x->grad_blocks = blocks;
+    }
+    float cw[3] = {1.0f, 1.0f, 1.0f};
+    if (class_weight)
+        for (int i = 0; i < 3; ++i) cw[i] = (float)class_weight[i];
+    for (int it = 0; it < iters; ++it) {
+        hipLaunchKernelGGL(k_sel_grad, dim3(blocks), dim3(SEL_TB), 0, x->stream, x->sel_w, x->samp_feat,
+                           x->samp_label, n, seed + 7919u * (unsigned)(x->adam_step + 1), cw[0], cw[1], cw[2],
+                           x->grad_part, x->loss_part);
+        x->adam_step += 1;
+        hipLaunchKernelGGL(k_sel_adam, dim3((SEL_NPARAM + 255) / 256), dim3(256), 0, x->stream, x->sel_w, x->sel_m,
+                           x->sel_v, x->grad_part, x->loss_part, blocks, (float)lr, x->adam_step, x->loss_dev);
+        if (loss) HIPCHK(hipMemcpyAsync(&loss[it], x->loss_dev, sizeof(float), hipMemcpyDeviceToHost, x->stream));
+    }
+    if (int rc = sel_pack(x)) return rc;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+int cumc3d_selector_enable(cumc3d_ctx *x, int enable, int every) {
+    if (!x) return fail(CUMC3D_EINVAL, "null context");
+    if (enable && !x->sel_w) return fail(CUMC3D_ESTATE, "call cumc3d_selector_init first");
+    if (every < 1) return fail(CUMC3D_EINVAL, "every must be >= 1");
+    x->sel_on = enable != 0;
+    x->sel_every = every;
+    x->sel_tick = 0;
+    return 0;
+}
+
+int cumc3d_selector_mlp(cumc3d_ctx *x, const float *feat, int n, float *logits) {
+    if (!x || !feat || !logits) return fail(CUMC3D_EINVAL, "null argument");
+    if (!x->sel_w) return fail(CUMC3D_ESTATE, "call cumc3d_selector_init first");
+    if (n < 1) return fail(CUMC3D_EINVAL, "n must be >= 1");
+    float *df = nullptr, *dl = nullptr;
+    HIPCHK(hipMalloc(&df, (size_t)n * SEL_IN * sizeof(float)));
+    HIPCHK(hipMalloc(&dl, (size_t)n * 3 * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(df, feat, (size_t)n * SEL_IN * sizeof(float), hipMemcpyHostToDevice, x->stream));
+    unsigned g = (unsigned)((n + 15) / 16);
+    hipLaunchKernelGGL(k_sel_mlp_rows, dim3(g < 4096 ? g : 4096), dim3(64), 0, x->stream, x->sel_frag, x->sel_w, df,
+                       (long)n, dl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(logits, dl, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipFree(df));
+    HIPCHK(hipFree(dl));
+    return 0;
+}
+
+int cumc3d_selector_mask(cumc3d_ctx *x, int dir, int refresh, unsigned char *out) {
+    if (!x || !out) return fail(CUMC3D_EINVAL, "null argument");
+    if (!x->sel_w) return fail(CUMC3D_ESTATE, "call cumc3d_selector_init first");
+    if (dir < 0 || dir >= x->d.n_dim) return fail(CUMC3D_EINVAL, "direction out of range");
+    if (refresh) {
+        if (x->fp32) sel_infer_t<float>(x);
+        else sel_infer_t<double>(x);
+    }
+    HIPCHK(hipMemcpyAsync(out, x->mask + (size_t)dir * x->G, x->G, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
     return 0;
 }
 

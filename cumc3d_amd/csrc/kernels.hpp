@@ -34,8 +34,13 @@ struct Axis {
     const double *xF;   // face coordinate, index j+3 (j=-3..n+3)
     const double *d;    // width, index j+3
     const double *c;    // centre, index j+3
-    const double *w;    // reconstruction weights, (j+3)*NW
+    const double *w;    // reconstruction weights of the configured scheme, (j+3)*NW
+    const double *wsel[3];   // TVD-MC, PPM and WENO-Z weights for the learned selector
 };
+
+// Reconstruction ids: 0 minmod, 1 van Leer, 2 MC, 3 PPM, 4 WENO-Z, and
+// RSEL: per cell from the learned selector's class mask (0 MC, 1 PPM, 2 WENO-Z).
+constexpr int RSEL = 5;
 
 // Field pointers of one context.  W is the working type of the spatial
 // operator: double in the FP64 mode; float in the FP32 split, where
@@ -61,6 +66,7 @@ template <class W> struct ArgsT {
     const double *rfl, *pfl, *efl;   // torus floors
     DevScalars *S;
     unsigned long long *blockmax;    // per-block max of double-star interface keys
+    const unsigned char *mask[3];    // selector class per cell and sweep direction
 };
 
 // Load field[idx] as scalar base + 32-bit byte offset (the global_load saddr
@@ -151,12 +157,33 @@ template <int R> constexpr int nweights() { return R <= 2 ? 2 : (R == 3 ? 14 : 1
 template <class T, int R, int D>
 __device__ __forceinline__ void recon_at(const ArgsT<T> &a, int j, int k, int l, long st, Face<T> &fm, Face<T> &fp) {
     long c = cidx(a, j, k, l);
-    T s[7][5];
-    load_stencil<T, R, D>(a, c, st, s);
     int i = (D == 0) ? j : (D == 1 ? k : l);
-    T w[NW];
-    load_w<T>(a.ax[D].w + (long)(i + 3) * NW, w, nweights<R>());
-    recon_cell<T, R>(w, s, (T)a.g1, fm, fp);
+    if constexpr (R == RSEL) {
+        // the highest class among the wave's active lanes, so the wave runs
+        // one reconstruction (ballots see active lanes only)
+        const int own = a.mask[D][c];
+        const unsigned long long b2 = __ballot(own == 2), b1 = __ballot(own >= 1);
+        const int cls = b2 ? 2 : (b1 ? 1 : 0);
+        T s[7][5];
+        load_stencil<T, 3, D>(a, c, st, s);
+        T w[NW];
+        if (cls == 0) {
+            load_w<T>(a.ax[D].wsel[0] + (long)(i + 3) * NW, w, 2);
+            recon_cell<T, 2>(w, s, (T)a.g1, fm, fp);
+        } else if (cls == 1) {
+            load_w<T>(a.ax[D].wsel[1] + (long)(i + 3) * NW, w, 14);
+            recon_cell<T, 3>(w, s, (T)a.g1, fm, fp);
+        } else {
+            load_w<T>(a.ax[D].wsel[2] + (long)(i + 3) * NW, w, 10);
+            recon_cell<T, 4>(w, s, (T)a.g1, fm, fp);
+        }
+    } else {
+        T s[7][5];
+        load_stencil<T, R, D>(a, c, st, s);
+        T w[NW];
+        load_w<T>(a.ax[D].w + (long)(i + 3) * NW, w, nweights<R>());
+        recon_cell<T, R>(w, s, (T)a.g1, fm, fp);
+    }
 }
 
 template <int D, class A> __device__ __forceinline__ long stride(const A &a) {

@@ -28,7 +28,12 @@ PD = C.POINTER(C.c_double)
 EXPORTS = ("cumc3d_init", "cumc3d_upload", "cumc3d_initial_update", "cumc3d_finddt",
            "cumc3d_rk3_step", "cumc3d_run", "cumc3d_download", "cumc3d_divb",
            "cumc3d_finalize", "cumc3d_error_string", "cumc3d_last_error",
-           "cumc3d_set_timing", "cumc3d_get_timing")
+           "cumc3d_set_timing", "cumc3d_get_timing",
+           "cumc3d_selector_init", "cumc3d_selector_set_params", "cumc3d_selector_get_params",
+           "cumc3d_selector_collect", "cumc3d_selector_train", "cumc3d_selector_enable",
+           "cumc3d_selector_mlp", "cumc3d_selector_mask")
+
+SEL_NPARAM = 1699
 
 BN_MODES = {"zero": 0, "field": 1, "carry": 2}
 
@@ -72,12 +77,25 @@ def load_library(path: str = LIB_PATH):
     L.cumc3d_last_error.restype = C.c_char_p
     L.cumc3d_set_timing.argtypes = [vp, C.c_int]
     L.cumc3d_get_timing.argtypes = [vp, PD, C.c_int]
+    PF = C.POINTER(C.c_float)
+    L.cumc3d_selector_init.argtypes = [vp, C.c_int, C.c_uint]
+    L.cumc3d_selector_set_params.argtypes = [vp, PF, C.c_int]
+    L.cumc3d_selector_get_params.argtypes = [vp, PF, C.c_int]
+    L.cumc3d_selector_collect.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_uint, C.POINTER(C.c_int)]
+    L.cumc3d_selector_train.argtypes = [vp, C.c_int, C.c_int, C.c_double, PD, C.c_uint, PF]
+    L.cumc3d_selector_enable.argtypes = [vp, C.c_int, C.c_int]
+    L.cumc3d_selector_mlp.argtypes = [vp, PF, C.c_int, PF]
+    L.cumc3d_selector_mask.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_ubyte)]
     _lib = L
     return L
 
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(PD)
+
+
+def _pf(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 def _check(rc: int):
@@ -193,6 +211,49 @@ class Solver:
         _check(load_library().cumc3d_get_timing(self._h, _p(ms), 7))
         keys = ("flux_x", "flux_y", "flux_z", "edge", "update", "ghost", "step")
         return dict(zip(keys, ms.tolist()))
+
+    # -- learned reconstruction selector ---------------------------------------
+    def selector_init(self, capacity: int = 1 << 20, seed: int = 1):
+        _check(load_library().cumc3d_selector_init(self._h, capacity, seed))
+
+    def selector_params(self, w: np.ndarray | None = None) -> np.ndarray:
+        """Set (if w is given) and return the FP32 parameter vector."""
+        L = load_library()
+        if w is not None:
+            w = np.ascontiguousarray(w, dtype=np.float32)
+            _check(L.cumc3d_selector_set_params(self._h, _pf(w), w.size))
+        out = np.zeros(SEL_NPARAM, dtype=np.float32)
+        _check(L.cumc3d_selector_get_params(self._h, _pf(out), SEL_NPARAM))
+        return out
+
+    def selector_collect(self, stride: int = 4, thr_ppm: float = 1e-3, thr_weno: float = 1e-2,
+                         seed: int = 1) -> np.ndarray:
+        """Append samples from the current state; returns (held, n0, n1, n2)."""
+        c = (C.c_int * 4)()
+        _check(load_library().cumc3d_selector_collect(self._h, stride, thr_ppm, thr_weno, seed, c))
+        return np.array(list(c))
+
+    def selector_train(self, iters: int, blocks: int = 64, lr: float = 3e-3, class_weight=(1.0, 1.0, 1.0),
+                       seed: int = 1) -> np.ndarray:
+        cw = np.ascontiguousarray(class_weight, dtype=np.float64)
+        loss = np.zeros(max(iters, 1), dtype=np.float32)
+        _check(load_library().cumc3d_selector_train(self._h, iters, blocks, lr, _p(cw), seed, _pf(loss)))
+        return loss[:iters]
+
+    def selector_enable(self, on: bool = True, every: int = 1):
+        _check(load_library().cumc3d_selector_enable(self._h, int(on), every))
+
+    def selector_mlp(self, feat: np.ndarray) -> np.ndarray:
+        feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1, 16)
+        out = np.zeros((feat.shape[0], 3), dtype=np.float32)
+        _check(load_library().cumc3d_selector_mlp(self._h, _pf(feat), feat.shape[0], _pf(out)))
+        return out
+
+    def selector_mask(self, direction: int, refresh: bool = True) -> np.ndarray:
+        out = np.zeros(self.cfg.shape, dtype=np.uint8)
+        _check(load_library().cumc3d_selector_mask(self._h, direction, int(refresh),
+                                                   out.ctypes.data_as(C.POINTER(C.c_ubyte))))
+        return out
 
 
 def make_solver(cfg: Config, state: dict, **kw) -> Solver:

@@ -113,6 +113,37 @@ const char *cumc3d_last_error(void);
 int cumc3d_set_timing(cumc3d_ctx *ctx, int enable);
 int cumc3d_get_timing(cumc3d_ctx *ctx, double *ms_per_launch, int n);
 
+/* Learned reconstruction selector (no reference counterpart: the reference's
+ * reconstruction is one compile-time choice, spatial.f90:33-43).  A 15->32->32->3
+ * MLP (BF16 weights on the matrix cores, FP32 accumulation) picks TVD-MC, PPM
+ * or WENO-Z per cell and sweep direction; the flux kernels run the highest
+ * class among a wave's cells.  Parameters: CUMC3D_SEL_NPARAM floats, layout
+ * W1[16][32], b1[32], W2[32][32], b2[32], W3[32][3], b3[3] (input 16 is a zero pad). */
+enum { CUMC3D_SEL_NPARAM = 1699 };
+/* Allocate the selector (sample buffer of `capacity` samples) and set fixed-seed
+ * He-uniform initial weights.  Until the first inference every cell is WENO-Z. */
+int cumc3d_selector_init(cumc3d_ctx *ctx, int capacity, unsigned seed);
+int cumc3d_selector_set_params(cumc3d_ctx *ctx, const float *w, int n);
+int cumc3d_selector_get_params(cumc3d_ctx *ctx, float *w, int n);
+/* Append training samples from the current state: every stride-th cell of each
+ * sweep region, labelled 2 (WENO-Z) where the relative gap between TVD-MC and
+ * WENO-Z faces of rho or p exceeds thr_weno, 1 (PPM) above thr_ppm, else 0.
+ * counts (may be NULL) receives {samples held, class-0, class-1, class-2 seen}. */
+int cumc3d_selector_collect(cumc3d_ctx *ctx, int stride, double thr_ppm, double thr_weno, unsigned seed,
+                            int *counts);
+/* iters Adam steps on minibatches of blocks*128 samples, class-weighted
+ * cross-entropy; loss (may be NULL) receives the mean loss of each step. */
+int cumc3d_selector_train(cumc3d_ctx *ctx, int iters, int blocks, double lr, const double *class_weight,
+                          unsigned seed, float *loss);
+/* Use the selector in the flux sweeps (refreshing the class masks every `every`
+ * steps, at stage 1), or the descriptor's reconstruction (enable = 0). */
+int cumc3d_selector_enable(cumc3d_ctx *ctx, int enable, int every);
+/* Logits of n feature rows (16 floats each) through the MFMA inference path. */
+int cumc3d_selector_mlp(cumc3d_ctx *ctx, const float *features, int n, float *logits);
+/* Class mask (one byte per ghosted-box cell) of sweep direction dir, after a
+ * fresh inference on the current state if refresh != 0. */
+int cumc3d_selector_mask(cumc3d_ctx *ctx, int dir, int refresh, unsigned char *out);
+
 #ifdef __cplusplus
 }
 #endif

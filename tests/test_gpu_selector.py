@@ -1,0 +1,125 @@
+"""The learned reconstruction selector on the GPU.  Needs an MI355X.
+
+* The MFMA inference path against a host model of the same network (BF16
+  weights, inputs and hidden activations, FP32 accumulation): logits within
+  1e-3 relative.  A wrong lane layout of the matrix-core fragments gives
+  unrelated logits, so this pins the layout.
+* Dispatch: parameters that force one class everywhere make the selector run
+  bit-identical to the plain run of that scheme (TVD-MC, PPM, WENO-Z), in the
+  FP32 split and in FP64.
+* Training on samples the solver generates lowers the loss, and the trained
+  selector's run stays close to the all-WENO FP64 run (tolerance below)."""
+import numpy as np
+import pytest
+
+from cumc3d_amd import ic
+from cumc3d_amd.config import orszag_tang_3d
+from cumc3d_amd.solver import SEL_NPARAM, make_solver
+
+pytestmark = pytest.mark.gpu
+
+W1, B1, W2, B2, W3, B3 = 0, 512, 544, 1568, 1600, 1696
+
+
+def bf16(x):
+    """Round float32 to bfloat16 (nearest even) and back."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+def host_mlp(w, feat):
+    w1 = bf16(w[W1:B1].reshape(16, 32)).astype(np.float64)
+    w2 = bf16(w[W2:B2].reshape(32, 32)).astype(np.float64)
+    w3 = bf16(w[W3:B3].reshape(32, 3)).astype(np.float64)
+    x = bf16(feat).astype(np.float64)
+    h1 = bf16(np.maximum(x @ w1 + w[B1:W2], 0.0)).astype(np.float64)
+    h2 = bf16(np.maximum(h1 @ w2 + w[B2:W3], 0.0)).astype(np.float64)
+    return h2 @ w3 + w[B3:B3 + 3]
+
+
+def forcing_params(cls):
+    w = np.zeros(SEL_NPARAM, dtype=np.float32)
+    w[B3 + cls] = 1.0
+    return w
+
+
+def small_ot(precision, recon="weno"):
+    cfg = orszag_tang_3d(24, recon=recon).with_(precision=precision)
+    return cfg, ic.orszag_tang(cfg)
+
+
+def test_selector_mlp_matches_host_model():
+    cfg, st = small_ot("fp32")
+    s = make_solver(cfg, st)
+    s.selector_init(capacity=1024, seed=3)
+    rng = np.random.default_rng(7)
+    w = (rng.standard_normal(SEL_NPARAM) * 0.4).astype(np.float32)
+    s.selector_params(w)
+    feat = rng.uniform(-1, 1, size=(1000, 16)).astype(np.float32)
+    feat[:, 15] = 0.0
+    got = s.selector_mlp(feat)
+    want = host_mlp(w, feat)
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    print(f"max relative logit error {err:.2e}", flush=True)
+    assert err < 1e-3
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("cls,recon", [(0, "tvdmc"), (1, "ppm"), (2, "weno")])
+def test_selector_forced_class_equals_plain_scheme(precision, cls, recon):
+    print(f"case class {cls} ({recon}) {precision}", flush=True)
+    cfg, st = small_ot(precision, recon=recon)
+    plain = make_solver(cfg, st)
+    sel = make_solver(cfg.with_(recon="weno"), st)
+    sel.selector_init(capacity=1024)
+    sel.selector_params(forcing_params(cls))
+    sel.selector_enable(True)
+    dp = plain.run(4)
+    ds = sel.run(4)
+    assert np.array_equal(dp, ds)
+    a, b = plain.state(), sel.state()
+    for f in ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz", "eps"):
+        assert np.array_equal(a[f], b[f]), f
+    for d in range(3):
+        m = sel.selector_mask(d, refresh=True)
+        assert np.all(m[3:-3, 3:-3, 3:-3] == cls)
+
+
+def test_selector_training_and_accuracy():
+    cfg, st = small_ot("fp32")
+    ref = make_solver(cfg.with_(precision="fp64"), st)
+    s = make_solver(cfg, st)
+    s.selector_init(capacity=1 << 18, seed=5)
+    counts = None
+    for k in range(4):
+        s.run(3)
+        counts = s.selector_collect(stride=2, thr_ppm=1e-3, thr_weno=1e-2, seed=k + 1)
+    print("samples held / class counts", counts.tolist(), flush=True)
+    assert counts[0] > 1000 and counts[1:].sum() >= counts[0]
+    loss = s.selector_train(400, blocks=64, lr=3e-3, class_weight=(1.0, 1.0, 2.0))
+    print(f"loss {loss[:10].mean():.4f} -> {loss[-10:].mean():.4f}", flush=True)
+    assert np.all(np.isfinite(loss))
+    assert loss[-10:].mean() < 0.8 * loss[:10].mean()
+    w = s.selector_params()
+
+    # the trained selector on a fresh FP32 run against the all-WENO FP64 run
+    t = make_solver(cfg, st)
+    t.selector_init(capacity=1024)
+    t.selector_params(w)
+    t.selector_enable(True)
+    dts = []
+    for _ in range(12):
+        dt = ref.finddt()
+        dts.append(dt)
+        ref.rungekutta(dt)
+        t.finddt()
+        t.rungekutta(dt)
+    fr = [np.bincount(t.selector_mask(d, refresh=False)[3:-3, 3:-3, 3:-3].ravel(), minlength=3) for d in range(3)]
+    frac = np.sum(fr, axis=0) / np.sum(fr)
+    x = ref.state()["rho"][3:-3, 3:-3, 3:-3]
+    y = t.state()["rho"][3:-3, 3:-3, 3:-3]
+    l1 = np.mean(np.abs(x - y)) / np.mean(np.abs(x))
+    print(f"class fractions {np.round(frac, 3).tolist()}, rho rel L1 vs FP64 WENO {l1:.2e}", flush=True)
+    assert np.all(np.isfinite(y))
+    assert l1 < 5e-3
