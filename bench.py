@@ -10,17 +10,22 @@ is available with --workload blast, but the reference scheme produces NaN in
 it after 7 steps (the reference Fortran does the same at 128^3 after 8), so a
 20-step timing of it would time NaN arithmetic.
 
-The same steps are timed twice from the same initial state: the FP64 mode
-(bit-exact with the reference) and the FP32 split (single-precision
-reconstruction, Riemann solver and EMFs; double-precision RK accumulators and
-energy-to-pressure recovery).  value = zone-updates/s of the FP32 split;
-fp64 and the ratio are reported beside it, with the FP32 split's density L1
-error against FP64 after the same steps with the same dt sequence.
+The same steps are timed three times from the same initial state:
+  fp64      the FP64 mode, bit-exact with the reference (WENO-Z everywhere);
+  fp32      the FP32 split: single-precision reconstruction, Riemann solver and
+            EMFs; double-precision RK accumulators and energy-to-pressure
+            recovery (WENO-Z everywhere);
+  fp32_sel  the FP32 split with the learned selector choosing TVD-MC, PPM or
+            WENO-Z per cell and direction, retrained here on samples the solver
+            generates from a 64^3 run of the same physics.
+value = zone-updates/s of fp32_sel; the others and the ratio are reported
+beside it, with the density L1 error of both FP32 variants against fp64 after
+the same steps with the same dt sequence.
 
-The roofline object is for the dominant kernel of the FP32 run: HIP-event
-durations on the library's stream, algorithmic bytes per launch.  The CPU
-baseline is the reference Fortran (oracle/_ref, built from its own sources)
-on a bounded sample of the same physics.
+The roofline object is for the dominant kernel of the fp32_sel run:
+HIP-event durations on the library's stream, algorithmic bytes per launch.
+The CPU baseline is the reference Fortran (oracle/_ref, built from its own
+sources) on a bounded sample of the same physics.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
                        [--workload ot|blast|turbulence] [--no-cpu]
@@ -85,9 +90,43 @@ def make_workload(name: str, n: int):
     raise SystemExit(f"unknown workload {name}")
 
 
-def time_precision(cfg, st, precision: str, warmup: int, steps: int):
+def train_selector(n: int, workload: str, thr_ppm: float, thr_weno: float, seed: int = 1):
+    """Train the selector on samples the FP32 solver generates from a small run
+    of the same physics (n^3, snapshots every 3 steps); returns its parameters
+    and the training summary."""
     from cumc3d_amd.solver import make_solver
-    s = make_solver(cfg.with_(precision=precision), st)
+    cfg, st, _ = make_workload(workload, n)
+    s = make_solver(cfg.with_(precision="fp32"), st)
+    s.selector_init(capacity=1 << 20, seed=seed)
+    counts = None
+    for k in range(6):
+        s.run(3, total_time=1e300)
+        counts = s.selector_collect(stride=2, thr_ppm=thr_ppm, thr_weno=thr_weno, seed=k + 1)
+    loss = s.selector_train(600, blocks=128, lr=3e-3, class_weight=(1.0, 1.0, 2.0), seed=seed)
+    w = s.selector_params()
+    del s
+    info = {"train_grid": f"{n}^3", "samples": int(counts[0]), "label_counts": counts[1:].tolist(),
+            "thresholds": [thr_ppm, thr_weno], "loss_first": float(loss[:10].mean()),
+            "loss_last": float(loss[-10:].mean()), "iters": 600}
+    return w, info
+
+
+SEL_EVERY = 1   # selector refresh interval in steps (set from --sel-every)
+
+
+def make_variant(cfg, st, variant: str, sel_w=None):
+    from cumc3d_amd.solver import make_solver
+    prec = "fp64" if variant == "fp64" else "fp32"
+    s = make_solver(cfg.with_(precision=prec), st)
+    if variant == "fp32_sel":
+        s.selector_init(capacity=1024)
+        s.selector_params(sel_w)
+        s.selector_enable(True, every=SEL_EVERY)
+    return s
+
+
+def time_variant(cfg, st, variant: str, warmup: int, steps: int, sel_w=None):
+    s = make_variant(cfg, st, variant, sel_w)
     s.run(warmup, total_time=1e300, sync_each_step=False)
     s.set_timing(True)
     s.run(2, total_time=1e300, sync_each_step=False)
@@ -96,27 +135,42 @@ def time_precision(cfg, st, precision: str, warmup: int, steps: int):
     t0 = time.perf_counter()
     s.run(steps, total_time=1e300, sync_each_step=False)   # returns after a stream sync
     wall = time.perf_counter() - t0
+    frac = None
+    if variant == "fp32_sel":
+        import numpy as np
+        cnt = sum(np.bincount(s.selector_mask(d, refresh=False)[3:-3, 3:-3, 3:-3].ravel(), minlength=3)
+                  for d in range(cfg.n_dim))
+        frac = (cnt / cnt.sum()).tolist()
     del s
-    return wall, kt
+    return wall, kt, frac
 
 
-def l1_error(cfg, st, steps: int):
-    """Density L1 (absolute, relative) and relative Linf of the FP32 split
-    against FP64 after `steps` steps driven with FP64's dt sequence."""
+def l1_errors(cfg, st, steps: int, sel_w=None):
+    """Density L1 (absolute, relative) and relative Linf of the FP32 split and
+    of FP32 + selector against FP64 after `steps` steps driven with FP64's dt
+    sequence."""
     import numpy as np
-    from cumc3d_amd.solver import make_solver
-    a = make_solver(cfg, st)
-    b = make_solver(cfg.with_(precision="fp32"), st)
+    a = make_variant(cfg, st, "fp64")
+    others = {"fp32": make_variant(cfg, st, "fp32")}
+    if sel_w is not None:
+        others["fp32_sel"] = make_variant(cfg, st, "fp32_sel", sel_w)
     for _ in range(steps):
         dt = a.finddt()
         a.rungekutta(dt)
-        b.finddt()
-        b.rungekutta(dt)
+        for b in others.values():
+            b.finddt()
+            b.rungekutta(dt)
     w = (slice(3, -3),) * 3
     x = a.state()["rho"][w]
-    y = b.state()["rho"][w]
-    d = np.abs(x - y)
-    return float(d.mean()), float(d.mean() / np.abs(x).mean()), float(d.max() / np.abs(x).max()), a.t
+    out = {}
+    for k, b in others.items():
+        y = b.state()["rho"][w]
+        d = np.abs(x - y)
+        out[k] = {"rho_l1": float(d.mean()), "rho_l1_rel": float(d.mean() / np.abs(x).mean()),
+                  "rho_linf_rel": float(d.max() / np.abs(x).max())}
+    out["steps"] = steps
+    out["t"] = a.t
+    return out
 
 
 def cpu_baseline(max_threads: int = 16):
@@ -152,6 +206,10 @@ def main(argv=None):
     ap.add_argument("--workload", default="ot", choices=["ot", "blast", "turbulence"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-l1", action="store_true")
+    ap.add_argument("--train-n", type=int, default=64)
+    ap.add_argument("--sel-every", type=int, default=1)
+    ap.add_argument("--thr-ppm", type=float, default=1e-3)
+    ap.add_argument("--thr-weno", type=float, default=1e-2)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
@@ -160,30 +218,36 @@ def main(argv=None):
 
     cfg, st, label = make_workload(args.workload, args.n)
     zones = cfg.zones
+    print(f"[bench] training the selector on a {args.train_n}^3 run", file=sys.stderr, flush=True)
+    global SEL_EVERY
+    SEL_EVERY = args.sel_every
+    sel_w, sel_info = train_selector(args.train_n, args.workload, args.thr_ppm, args.thr_weno)
+    sel_info["refresh_every"] = SEL_EVERY
+    print(f"[bench] selector: {sel_info}", file=sys.stderr, flush=True)
     res = {}
-    for prec in ("fp64", "fp32"):
-        print(f"[bench] {label}: {prec} warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
-        wall, kt = time_precision(cfg, st, prec, args.warmup, args.steps)
-        res[prec] = (wall, kt)
-        print(f"[bench] {prec}: {1e3 * wall / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
+    for var in ("fp64", "fp32", "fp32_sel"):
+        print(f"[bench] {label}: {var} warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
+        res[var] = time_variant(cfg, st, var, args.warmup, args.steps, sel_w)
+        print(f"[bench] {var}: {1e3 * res[var][0] / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
 
-    wall, kt = res["fp32"]
-    value = zones * args.steps / wall
-    fp64_value = zones * args.steps / res["fp64"][0]
+    def rate(v):
+        return zones * args.steps / res[v][0]
+
+    wall, kt, frac = res["fp32_sel"]
+    value = rate("fp32_sel")
     dom = max(KERNELS, key=lambda k: kt[k])
     achieved = kernel_bytes(cfg, dom, 4) / (kt[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                "kernel_ms": kt[dom], "kernel_ms_all": {k: kt[k] for k in KERNELS},
-                "kernel_ms_all_fp64": {k: res["fp64"][1][k] for k in KERNELS},
+                "kernel_ms": kt[dom],
+                "kernel_ms_all": {v: {k: res[v][1][k] for k in KERNELS + ["selector"]} for v in res},
                 "step_bytes_frac": 512.0 * value / (HBM_PEAK_GBS * 1e9)}
 
     err = None
     if not args.no_l1:
         nacc = args.warmup + 2 + args.steps
-        print(f"[bench] L1 of fp32 vs fp64 after {nacc} steps", file=sys.stderr, flush=True)
-        l1, l1rel, linf, t = l1_error(cfg, st, nacc)
-        err = {"rho_l1": l1, "rho_l1_rel": l1rel, "rho_linf_rel": linf, "steps": nacc, "t": t}
+        print(f"[bench] L1 against FP64 after {nacc} steps", file=sys.stderr, flush=True)
+        err = l1_errors(cfg, st, nacc, sel_w)
 
     cpu = None
     if not args.no_cpu:
@@ -196,12 +260,13 @@ def main(argv=None):
     line = {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64 (bf16 selector)",
         "data": f"synthetic (seeded IC, cumc3d_amd.ic.{ {'ot': 'orszag_tang'}.get(args.workload, args.workload) })",
-        "config": {"workload": label + ", FP32 split (FP64 accumulators)", "zones": zones,
-                   "riemann": cfg.riemann, "recon": cfg.recon, "selector": None},
-        "fp64": {"value": fp64_value, "ms_per_step": 1e3 * res["fp64"][0] / args.steps, "dtype": "f64"},
-        "speedup_vs_fp64": value / fp64_value,
+        "config": {"workload": label + ", FP32 split + learned selector (FP64 accumulators)", "zones": zones,
+                   "riemann": cfg.riemann, "recon": "selector: TVD-MC / PPM / WENO-Z per cell"},
+        "variants": {v: {"value": rate(v), "ms_per_step": 1e3 * res[v][0] / args.steps} for v in res},
+        "speedup_vs_fp64": value / rate("fp64"),
+        "selector": dict(sel_info, class_fractions=frac),
         "l1_vs_fp64": err,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -62,10 +62,11 @@ struct cumc3d_ctx {
     bool uploaded = false, initialized = false, dt_pending = false;
     // timing
     bool timing = false;
-    std::vector<hipEvent_t> ev;   // 7 marks per stage, TSTAGES stages
+    std::vector<hipEvent_t> ev;   // TMARKS marks per stage, TSTAGES stages
+    std::vector<char> sel_timed;  // stage refreshed the selector masks
     int tstage = 0;               // stages recorded in the current run
-    double tsum[7] = {0};
-    long tcnt[7] = {0};
+    double tsum[8] = {0};
+    long tcnt[8] = {0};
 };
 
 // ---------------------------------------------------------------------------
@@ -406,8 +407,9 @@ static long blocks_needed(cumc3d_ctx *x) {
 // stage / step
 // ---------------------------------------------------------------------------
 static constexpr int TSTAGES = 192;
+static constexpr int TMARKS = 9;   // 0..6 stage kernels, 7..8 selector inference
 static inline void tmark(cumc3d_ctx *x, int slot) {
-    if (x->timing && x->tstage < TSTAGES) hipEventRecord(x->ev[x->tstage * 7 + slot], x->stream);
+    if (x->timing && x->tstage < TSTAGES) hipEventRecord(x->ev[x->tstage * TMARKS + slot], x->stream);
 }
 
 template <class W> static void run_stage_t(cumc3d_ctx *x, int stage) {
@@ -449,8 +451,11 @@ template <class W> static void sel_infer_t(cumc3d_ctx *x) {
 static void sel_refresh(cumc3d_ctx *x) {
     if (!x->sel_on) return;
     if (x->sel_tick++ % x->sel_every != 0) return;
+    tmark(x, 7);
     if (x->fp32) sel_infer_t<float>(x);
     else sel_infer_t<double>(x);
+    tmark(x, 8);
+    if (x->timing && x->tstage < TSTAGES) x->sel_timed[x->tstage] = 1;
 }
 
 static void run_stage(cumc3d_ctx *x, int stage) {
@@ -463,13 +468,18 @@ static void run_stage(cumc3d_ctx *x, int stage) {
 static void accumulate_timing(cumc3d_ctx *x) {
     int dims = x->d.n_dim;
     for (int st = 0; st < x->tstage; ++st) {
-        hipEvent_t *e = &x->ev[st * 7];
+        hipEvent_t *e = &x->ev[st * TMARKS];
         for (int s = 0; s < 6; ++s) {
             if ((s == 1 && dims < 2) || (s == 2 && dims < 3)) continue;
             float ms = 0;
             if (hipEventElapsedTime(&ms, e[s], e[s + 1]) != hipSuccess) continue;
             x->tsum[s] += ms;
             x->tcnt[s] += 1;
+        }
+        float ms = 0;
+        if (x->sel_timed[st] && hipEventElapsedTime(&ms, e[7], e[8]) == hipSuccess) {
+            x->tsum[7] += ms;
+            x->tcnt[7] += 1;
         }
     }
 }
@@ -716,10 +726,12 @@ int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_st
     HIPCHK(hipMemcpy(x->S, &h, sizeof(h), hipMemcpyHostToDevice));
     if (x->timing) {
         if (x->ev.empty()) {
-            x->ev.resize((size_t)TSTAGES * 7);
+            x->ev.resize((size_t)TSTAGES * TMARKS);
+            x->sel_timed.assign(TSTAGES, 0);
             for (auto &e : x->ev) HIPCHK(hipEventCreate(&e));
         }
-        for (int i = 0; i < 7; ++i) { x->tsum[i] = 0; x->tcnt[i] = 0; }
+        for (int i = 0; i < 8; ++i) { x->tsum[i] = 0; x->tcnt[i] = 0; }
+        x->sel_timed.assign(TSTAGES, 0);
         x->tstage = 0;
     }
     hipEvent_t s0 = nullptr, s1 = nullptr;
@@ -1037,7 +1049,7 @@ int cumc3d_set_timing(cumc3d_ctx *x, int enable) {
 
 int cumc3d_get_timing(cumc3d_ctx *x, double *ms, int n) {
     if (!x || !ms) return fail(CUMC3D_EINVAL, "null argument");
-    for (int i = 0; i < n && i < 7; ++i) ms[i] = x->tcnt[i] ? x->tsum[i] / x->tcnt[i] : 0.0;
+    for (int i = 0; i < n && i < 8; ++i) ms[i] = x->tcnt[i] ? x->tsum[i] / x->tcnt[i] : 0.0;
     return 0;
 }
 

@@ -605,8 +605,13 @@ __global__ __launch_bounds__(256) void k_update(ArgsT<W> a) {
         const long c = cidx(a, j, k, l), sy = a.NX, sz = a.NX * a.NY;
         const double dt = a.S->dt;
         // stage-start primitives and conserved variables (FROMRVETOU, fromrvetou.f90:428-438)
-        double rho = a.P[0][c], vx = a.P[1][c], vy = a.P[2][c], vz = a.P[3][c];
-        double bx = a.Bc[0][c], by = a.Bc[1][c], bz = a.Bc[2][c];
+        // stage-start primitives: FROMRVETOU input in the FP64 mode, source
+        // terms only in the FP32 split
+        double rho = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
+        if (!split || a.coord == 1 || a.model == 1 || a.rt_gravity) {
+            rho = a.P[0][c]; vx = a.P[1][c]; vy = a.P[2][c]; vz = a.P[3][c];
+            bx = a.Bc[0][c]; by = a.Bc[1][c]; bz = a.Bc[2][c];
+        }
         double u[5], uo[5];
         if constexpr (split) {
             for (int i = 0; i < 5; ++i) u[i] = a.U[i][c];
@@ -776,6 +781,7 @@ template <class W> __global__ __launch_bounds__(256) void k_ghost(ArgsT<W> a) {
     for (int d = 0; d < 3; ++d)
         a.Bc[d][c] = (W)(a.Bc[d][cs_] * (gsign(a, 0, sx, 6 + d) * gsign(a, 1, sy, 6 + d) * gsign(a, 2, sz, 6 + d)));
     // epsilon: BOUNDARY1D_NM takes the y-outer branch from boundary_flag(3) (boundary.f90:243)
+    bool eps_same_cell = true;
     {
         int ey = kk;
         bool eps_y = gy;
@@ -787,6 +793,7 @@ template <class W> __global__ __launch_bounds__(256) void k_ghost(ArgsT<W> a) {
             else if (f4 >= 2) ey = a.ny + 1 - m;
             else { ey = k; eps_y = false; }
         }
+        eps_same_cell = (gx || eps_y || gz) && ey == kk;
         if (gx || eps_y || gz) a.eps[c] = a.eps[cidx(a, jj, ey, ll)];
     }
     // transverse face fields: Bx is filled in y/z ghosts, By in x/z, Bz in x/y
@@ -804,6 +811,14 @@ template <class W> __global__ __launch_bounds__(256) void k_ghost(ArgsT<W> a) {
     a.P[1][c] = (W)v[0];
     a.P[2][c] = (W)v[1];
     a.P[3][c] = (W)v[2];
+    if (sizeof(W) == 4 && a.model == 0 && eps_same_cell) {
+        // FP32 split: the source cell's p and c_s were computed from its
+        // double-precision rho and eps; the same FINDPRESSURE of the
+        // single-precision copies could differ in the last bit
+        a.P[4][c] = a.P[4][cs_];
+        a.cs[c] = a.cs[cs_];
+        return;
+    }
     double p, cs;
     pressure(a, c, rho, (double)a.eps[c], p, cs);
     a.P[4][c] = (W)p;

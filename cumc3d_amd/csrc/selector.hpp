@@ -42,24 +42,29 @@ __device__ __forceinline__ float sel_from_bf16(unsigned short h) { return __uint
 // features and labels
 // ---------------------------------------------------------------------------
 
-// Feature f (0..15) of cell c along a sweep with stride st: variable v = f/5
-// (rho, p, |v|) at stencil offset s = f%5 - 2, as a bounded relative
-// difference from the cell itself; feature 15 is the zero pad.
+// The 16 inputs of cell c along a sweep with stride st: feature 5v+s is
+// variable v (rho, p, |v|) at stencil offset s-2 as a bounded relative
+// difference from the cell itself (|v| is scaled by |v| + c_s); input 15 is a
+// zero pad.  Training samples and inference share this function.
 template <class W>
-__device__ __forceinline__ float sel_feature(const ArgsT<W> &a, long c, long st, int f) {
-    if (f >= 15) return 0.0f;
-    const int v = f / 5;
-    const long q = c + (long)(f % 5 - 2) * st;
-    if (v < 2) {
-        const W *x = v == 0 ? a.P[0] : a.P[4];
-        float xs = (float)x[q], x0 = (float)x[c];
-        return (xs - x0) / (fabsf(xs) + fabsf(x0) + 1.0e-30f);
+__device__ __forceinline__ void sel_features(const ArgsT<W> &a, long c, long st, float f[SEL_IN]) {
+    float rho[5], pr[5], u[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const long q = c + (long)(m - 2) * st;
+        rho[m] = (float)a.P[0][q];
+        pr[m] = (float)a.P[4][q];
+        const float vx = (float)a.P[1][q], vy = (float)a.P[2][q], vz = (float)a.P[3][q];
+        u[m] = sqrtf(vx * vx + vy * vy + vz * vz);
     }
-    float us = sqrtf((float)a.P[1][q] * (float)a.P[1][q] + (float)a.P[2][q] * (float)a.P[2][q] +
-                     (float)a.P[3][q] * (float)a.P[3][q]);
-    float u0 = sqrtf((float)a.P[1][c] * (float)a.P[1][c] + (float)a.P[2][c] * (float)a.P[2][c] +
-                     (float)a.P[3][c] * (float)a.P[3][c]);
-    return (us - u0) / (us + u0 + (float)a.cs[c] + 1.0e-30f);
+    const float c0 = (float)a.cs[c];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        f[m] = (rho[m] - rho[2]) / (fabsf(rho[m]) + fabsf(rho[2]) + 1.0e-30f);
+        f[5 + m] = (pr[m] - pr[2]) / (fabsf(pr[m]) + fabsf(pr[2]) + 1.0e-30f);
+        f[10 + m] = (u[m] - u[2]) / (u[m] + u[2] + c0 + 1.0e-30f);
+    }
+    f[15] = 0.0f;
 }
 
 // Training label of cell c along D: the largest relative gap between the
@@ -172,8 +177,7 @@ __device__ __forceinline__ sel_s4 sel_act(sel_f4 d, const float b[4]) {
 
 // Logits of 16 cells: x = this lane's 4 features (inputs 4g..4g+3 of cell r).
 // Returns D3: lanes 0..15 hold logits 0..3 (3 used) of cell r.
-__device__ __forceinline__ sel_f4 sel_mlp16(const SelNet &n, const float x[4]) {
-    sel_s4 bx = {(short)sel_bf16(x[0]), (short)sel_bf16(x[1]), (short)sel_bf16(x[2]), (short)sel_bf16(x[3])};
+__device__ __forceinline__ sel_f4 sel_mlp16_b(const SelNet &n, sel_s4 bx) {
     const sel_f4 z = {0.0f, 0.0f, 0.0f, 0.0f};
     sel_f4 d1a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(n.A[0], bx, z, 0, 0, 0);
     sel_f4 d1b = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(n.A[1], bx, z, 0, 0, 0);
@@ -192,6 +196,11 @@ __device__ __forceinline__ sel_f4 sel_mlp16(const SelNet &n, const float x[4]) {
     return d3;
 }
 
+__device__ __forceinline__ sel_f4 sel_mlp16(const SelNet &n, const float x[4]) {
+    sel_s4 bx = {(short)sel_bf16(x[0]), (short)sel_bf16(x[1]), (short)sel_bf16(x[2]), (short)sel_bf16(x[3])};
+    return sel_mlp16_b(n, bx);
+}
+
 __device__ __forceinline__ int sel_argmax(sel_f4 d) {
     // ties go to the higher-order class
     int c = 2;
@@ -201,29 +210,52 @@ __device__ __forceinline__ int sel_argmax(sel_f4 d) {
     return c;
 }
 
-// Class mask of every cell the sweep along D reconstructs.  Each wave walks
-// groups of 16 cells (a grid-stride loop every wave leaves after the last group).
+// Class mask of every cell the sweep along D reconstructs.  Each wave takes
+// 64 consecutive cells (one per lane, coalesced stencil loads), stages their
+// BF16 features in its own LDS tile, and runs four 16-cell MFMA groups from
+// it.  Waves step through the cells grid-stride and never wait on each other,
+// so only wavefront-scope ordering is needed around the tile.
 template <class W, int D>
 __global__ __launch_bounds__(256) void k_sel_infer(ArgsT<W> a, const unsigned short *frag, const float *w,
                                                    unsigned char *mask) {
+    __shared__ unsigned short tile[4][64][SEL_IN + 4];
     SelNet net;
     sel_load_net(frag, w, net);
-    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     const long nq = sel_count<D>(a);
-    const long ngroups = (nq + 15) / 16;
+    const long nchunk = (nq + 63) / 64;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
     const long st = D == 0 ? 1L : (D == 1 ? a.NX : a.NX * a.NY);
-    for (long grp = wave; grp < ngroups; grp += nwaves) {
-        const long q = grp * 16 + r;
+    for (long ch = wave; ch < nchunk; ch += nwaves) {
+        const long q = ch * 64 + lane;
         const bool ok = q < nq;
         int i = 0;
         const long c = ok ? sel_cell<D>(a, q, i) : 0;
-        float x[4];
+        float f[SEL_IN];
+        if (ok) sel_features(a, c, st, f);
+        else
+            for (int e = 0; e < SEL_IN; ++e) f[e] = 0.0f;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) x[e] = ok ? sel_feature(a, c, st, 4 * g + e) : 0.0f;
-        sel_f4 d = sel_mlp16(net, x);
-        if (ok && g == 0) mask[c] = (unsigned char)sel_argmax(d);
+        for (int e = 0; e < SEL_IN; ++e) tile[wv][lane][e] = sel_bf16(f[e]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        unsigned char cls_out = 0;
+#pragma unroll
+        for (int sub = 0; sub < 4; ++sub) {
+            const unsigned short *src = &tile[wv][sub * 16 + r][4 * g];
+            sel_s4 bx = {(short)src[0], (short)src[1], (short)src[2], (short)src[3]};
+            sel_f4 d = sel_mlp16_b(net, bx);
+            // lanes 0..15 hold the logits of cell sub*16 + r; hand them to that lane
+            const int cls = sel_argmax(d);
+            const int want = __shfl(cls, lane & 15, 64);
+            if ((lane >> 4) == sub) cls_out = (unsigned char)want;
+        }
+        if (ok) mask[c] = cls_out;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -277,7 +309,9 @@ __global__ __launch_bounds__(256) void k_sel_collect(ArgsT<W> a, int stride, uns
     const int lab = sel_label<W, D>(a, c, i, st, thr_ppm, thr_weno);
     const int slot = atomicAdd(&counts[0], 1);
     if (slot >= cap) return;
-    for (int f = 0; f < SEL_IN; ++f) feat[(long)slot * SEL_IN + f] = sel_feature(a, c, st, f);
+    float f[SEL_IN];
+    sel_features(a, c, st, f);
+    for (int e = 0; e < SEL_IN; ++e) feat[(long)slot * SEL_IN + e] = f[e];
     label[slot] = (unsigned char)lab;
     atomicAdd(&counts[1 + lab], 1);
 }

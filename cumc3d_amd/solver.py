@@ -207,9 +207,9 @@ class Solver:
         _check(load_library().cumc3d_set_timing(self._h, int(on)))
 
     def timing(self) -> dict:
-        ms = np.zeros(7)
-        _check(load_library().cumc3d_get_timing(self._h, _p(ms), 7))
-        keys = ("flux_x", "flux_y", "flux_z", "edge", "update", "ghost", "step")
+        ms = np.zeros(8)
+        _check(load_library().cumc3d_get_timing(self._h, _p(ms), 8))
+        keys = ("flux_x", "flux_y", "flux_z", "edge", "update", "ghost", "step", "selector")
         return dict(zip(keys, ms.tolist()))
 
     # -- learned reconstruction selector ---------------------------------------
