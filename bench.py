@@ -67,7 +67,11 @@ def kernel_bytes(cfg, name: str, w: int) -> float:
     if name == "update":
         # stage 2/3 of the fused update
         if split:
-            return n * (8 * (5 + 8 + 3 + 3 + 5) + w * (8 + 15 + 3 + 5 + 1 + 1 + 3))
+            # U, u_old, face B (+ lower faces) in double; fluxes, edge EMFs
+            # and the primitives written back in float (the stage-start
+            # primitives are read only for the cylindrical/torus/RT sources)
+            prim_in = 8 if (cfg.coordinate_flag == 1 or cfg.model == "torus" or cfg.rt_gravity) else 0
+            return n * (8 * (5 + 8 + 3 + 3 + 5) + w * (prim_in + 15 + 3 + 5 + 1 + 1 + 3))
         return n * 8 * (5 + 3 + 1 + 8 + 3 + 15 + 3 + 5 + 1 + 1 + 3 + 3)
     if name == "ghost":
         g = (nx + 6) * (ny + 6) * (nz + 6) - n
@@ -197,7 +201,54 @@ def cpu_baseline(max_threads: int = 16):
             "sample": sample + f", {ref['seconds']:.1f} s loop time"}
 
 
+def trace_summary(path: str, n: int) -> str:
+    """Markdown table of a rocprofv3 --kernel-trace CSV: per kernel, the
+    dispatches at its largest grid (the n^3 workload; the selector's training
+    run is smaller), with calls, mean/min/max duration and VGPRs, plus the
+    algorithmic bytes and GB/s of the stage kernels."""
+    import csv
+    from collections import defaultdict
+    from cumc3d_amd.config import orszag_tang_3d
+    cfg = orszag_tang_3d(n)
+    groups = defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+        groups[r["Kernel_Name"]].append((grid, dur, int(r["VGPR_Count"]), int(r["Accum_VGPR_Count"])))
+    rows = []
+    for name, ds in groups.items():
+        gmax = max(d[0] for d in ds)
+        sel = [d for d in ds if d[0] == gmax]
+        durs = [d[1] for d in sel]
+        rows.append((sum(durs), name, len(sel), gmax, sum(durs) / len(durs), min(durs), max(durs), sel[0][2], sel[0][3]))
+    rows.sort(reverse=True)
+    cls = [("k_flux_x<", "flux_x"), ("k_flux_yz<", None), ("k_update<", "update")]
+    out = ["| kernel | calls | grid (threads) | mean us | min us | max us | VGPR | AGPR | algorithmic MB | GB/s |",
+           "|---|---|---|---|---|---|---|---|---|---|"]
+    for tot, name, calls, grid, mean, mn, mx, vg, ag in rows:
+        if calls < 2 and tot < 100:
+            continue
+        w = 4 if "<float" in name else 8
+        kb = None
+        if "k_flux_x<" in name:
+            kb = kernel_bytes(cfg, "flux_x", w)
+        elif "k_flux_yz<" in name:
+            kb = kernel_bytes(cfg, "flux_y" if name.rstrip(")").split(",")[3].strip().startswith("1") else "flux_z", w)
+        elif "k_update<" in name:
+            kb = kernel_bytes(cfg, "update", w)
+        mb = f"{kb / 1e6:.0f}" if kb else ""
+        gbs = f"{kb / (mean * 1e-6) / 1e9:.0f}" if kb else ""
+        short = name.replace("cumc3d::", "").replace("void ", "")
+        short = short.split("(")[0]
+        out.append(f"| `{short}` | {calls} | {grid} | {mean:.1f} | {mn:.1f} | {mx:.1f} | {vg} | {ag} | {mb} | {gbs} |")
+    return "\n".join(out)
+
+
 def main(argv=None):
+    if argv is None and len(sys.argv) > 2 and sys.argv[1] == "--trace-summary":
+        n = int(sys.argv[3]) if len(sys.argv) > 3 else 256
+        print(trace_summary(sys.argv[2], n))
+        return 0
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)

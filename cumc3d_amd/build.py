@@ -12,27 +12,42 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libcumc3d_hip.so")
-SOURCES = [os.path.join(HERE, "csrc", "capi.hip")]
-DEPS = [os.path.join(HERE, "csrc", f) for f in ("kernels.hpp", "physics.hpp")] + \
+# (source, extra flags): the FP32 split's unit may contract and use the short
+# single-precision divide and square root; the FP64 unit may not.
+SOURCES = [(os.path.join(HERE, "csrc", "capi.hip"), []),
+           (os.path.join(HERE, "csrc", "fp32.hip"), ["-ffp-contract=fast",
+                                                     "-fno-hip-fp32-correctly-rounded-divide-sqrt"])]
+DEPS = [os.path.join(HERE, "csrc", f) for f in ("kernels.hpp", "physics.hpp", "selector.hpp", "context.hpp")] + \
        [os.path.join(ROOT, "include", "cumc3d_hip.h")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CUMC3D_ARCH", "gfx950")
 
 # -ffp-contract=off: no fused multiply-add, so the FP64 mode reproduces the
-# reference's arithmetic exactly.  Packed FP32 instructions are disabled.
-FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+# reference's arithmetic exactly (a later -ffp-contract=fast on the FP32 unit
+# overrides it there).  Packed FP32 instructions are disabled.
+FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-ffp-contract=off", "-fno-fast-math", "-fno-gpu-rdc",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result"]
 
 
-def build(verbose: bool = True) -> str:
-    cmd = [HIPCC, *FLAGS, "-o", LIB + ".tmp", *SOURCES]
-    t0 = time.time()
+def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed")
+        raise RuntimeError("hipcc failed: " + " ".join(cmd[-3:]))
+
+
+def build(verbose: bool = True) -> str:
+    from concurrent.futures import ThreadPoolExecutor
+    t0 = time.time()
+    objs = [os.path.join(HERE, "csrc", os.path.basename(src) + ".o") for src, _ in SOURCES]
+    with ThreadPoolExecutor(len(SOURCES)) as ex:
+        list(ex.map(_run, [[HIPCC, *FLAGS, *extra, "-c", src, "-o", obj]
+                           for (src, extra), obj in zip(SOURCES, objs)]))
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-fno-gpu-rdc", "-o", LIB + ".tmp", *objs])
+    for o in objs:
+        os.remove(o)
     os.replace(LIB + ".tmp", LIB)
     if verbose:
         print(f"built {LIB} in {time.time() - t0:.1f}s")

@@ -1,5 +1,5 @@
-// Host side of libcumc3d_hip.so: context, device state, launch sequencing
-// and the C-ABI declared in include/cumc3d_hip.h.
+// Host side of libcumc3d_hip.so: the C-ABI declared in include/cumc3d_hip.h,
+// step sequencing and the FP64 mode's kernels (the FP32 split's are in fp32.hip).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -10,8 +10,7 @@
 #include <vector>
 
 #include "../../include/cumc3d_hip.h"
-#include "kernels.hpp"
-#include "selector.hpp"
+#include "context.hpp"
 
 using namespace cumc3d;
 
@@ -28,139 +27,6 @@ static int fail(int code, const std::string &msg) {
         if (e_ != hipSuccess)                                                             \
             return fail(CUMC3D_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));     \
     } while (0)
-
-struct cumc3d_ctx {
-    cumc3d_desc d;
-    int nx, ny, nz;
-    long NX, NY, NZ, G;
-    hipStream_t stream = nullptr;
-    void *pool = nullptr;         // all per-cell fields
-    bool fp32 = false;            // CUMC3D_FP32: working fields are float
-    double *met = nullptr;        // grid metrics and weights
-    DevScalars *S = nullptr;
-    unsigned long long *blockmax = nullptr;
-    long nblockmax = 0;
-    // working-type fields (double, or float in the FP32 split)
-    void *P[5], *eps, *cs, *Bc[3], *F[3][7], *E[3], *Ec[3];
-    // always double: face field (ping-pong), u_old, conserved state (FP32 split), floors
-    double *B[2][3], *Uo[8], *U[5], *floors[3];
-    int cur = 0;                  // B[cur] holds the current face field
-    long march_threads = 1L << 19;  // target thread count of the y/z sweeps
-    // learned reconstruction selector
-    bool sel_on = false;
-    int sel_every = 1, sel_tick = 0;
-    float *sel_w = nullptr, *sel_m = nullptr, *sel_v = nullptr;   // FP32 parameters + Adam moments
-    unsigned short *sel_frag = nullptr;                            // BF16 MFMA fragments
-    unsigned char *mask = nullptr;                                 // 3 x G class mask
-    float *samp_feat = nullptr;
-    unsigned char *samp_label = nullptr;
-    int samp_cap = 0, *samp_counts = nullptr, adam_step = 0;
-    float *grad_part = nullptr, *loss_part = nullptr, *loss_dev = nullptr;
-    int grad_blocks = 0;
-    int bfac[6][12];
-    Axis ax[3];
-    bool uploaded = false, initialized = false, dt_pending = false;
-    // timing
-    bool timing = false;
-    std::vector<hipEvent_t> ev;   // TMARKS marks per stage, TSTAGES stages
-    std::vector<char> sel_timed;  // stage refreshed the selector masks
-    int tstage = 0;               // stages recorded in the current run
-    double tsum[8] = {0};
-    long tcnt[8] = {0};
-};
-
-// ---------------------------------------------------------------------------
-// layout conversion kernels (upload/download only)
-// ---------------------------------------------------------------------------
-template <class W> __global__ void k_split_prim(const double *aos, const double *eps64, ArgsT<W> a) {
-    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.G) return;
-    const double *q = aos + 8 * c;
-    for (int i = 0; i < 5; ++i) a.P[i][c] = (W)q[i];
-    for (int d = 0; d < 3; ++d) {
-        a.Bin[d][c] = q[5 + d];
-        a.Bout[d][c] = q[5 + d];
-    }
-    a.eps[c] = (W)eps64[c];
-    if constexpr (sizeof(W) == 4) {
-        // double copies of rho, v, eps for INITIAL_UPDATE's conserved state
-        for (int i = 0; i < 4; ++i) a.U[i][c] = q[i];
-        a.U[4][c] = eps64[c];
-    }
-}
-
-// prim AoS (rho, vx, vy, vz, p, Bx, By, Bz per cell) from the device fields
-template <class W> __global__ void k_join_prim(double *aos, ArgsT<W> a) {
-    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.G) return;
-    double *q = aos + 8 * c;
-    for (int i = 0; i < 5; ++i) q[i] = a.P[i][c];
-    for (int d = 0; d < 3; ++d) q[5 + d] = a.Bin[d][c];
-}
-
-template <class W> __global__ void k_join3(double *aos, const W *f0, const W *f1, const W *f2, long G) {
-    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= G) return;
-    aos[3 * c] = f0[c];
-    aos[3 * c + 1] = f1[c];
-    aos[3 * c + 2] = f2[c];
-}
-
-template <class W> __global__ void k_widen(double *out, const W *in, long G) {
-    long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < G) out[c] = in[c];
-}
-
-// INITIAL_UPDATE interior part: CUSTOM_CHECKRHO, bcell of FROMRVETOU and
-// FINDPRESSURE on the interior (initial.f90:91-108).  The FP32 split starts
-// from the double copies k_split_prim left in U and writes the conserved
-// state there (FROMRVETOU, fromrvetou.f90:428-438).
-template <class W> __global__ void k_init_interior(ArgsT<W> a) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)a.nx * a.ny * a.nz) return;
-    const int j = 1 + (int)(t % a.nx), k = 1 + (int)((t / a.nx) % a.ny), l = 1 + (int)(t / ((long)a.nx * a.ny));
-    const long c = cidx(a, j, k, l);
-    constexpr bool split = sizeof(W) == 4;
-    double rho, eps, v[3];
-    if constexpr (split) {
-        rho = a.U[0][c]; v[0] = a.U[1][c]; v[1] = a.U[2][c]; v[2] = a.U[3][c]; eps = a.U[4][c];
-    } else {
-        rho = a.P[0][c]; v[0] = a.P[1][c]; v[1] = a.P[2][c]; v[2] = a.P[3][c]; eps = a.eps[c];
-    }
-    checkrho(a, c, rho, v, eps);
-    a.P[0][c] = (W)rho; a.P[1][c] = (W)v[0]; a.P[2][c] = (W)v[1]; a.P[3][c] = (W)v[2];
-    a.eps[c] = (W)eps;
-    double b[3] = {bavg<0>(a, j, a.Bin[0][c], a.Bin[0][c - 1]), bavg<1>(a, j, a.Bin[1][c], a.Bin[1][c - a.NX]),
-                   bavg<2>(a, j, a.Bin[2][c], a.Bin[2][c - a.NX * a.NY])};
-    for (int d = 0; d < 3; ++d) a.Bc[d][c] = (W)b[d];
-    double p, cs;
-    pressure(a, c, rho, eps, p, cs);
-    a.P[4][c] = (W)p;
-    a.cs[c] = (W)cs;
-    if constexpr (split) {
-        double vsq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2], bsq = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
-        a.U[0][c] = rho;
-        a.U[1][c] = v[0] * rho;
-        a.U[2][c] = v[1] * rho;
-        a.U[3][c] = v[2] * rho;
-        a.U[4][c] = rho * (eps + 0.5 * vsq) + 0.5 * bsq;
-    }
-}
-
-// standalone FINDDT (rungekutta.f90:256-323)
-template <class W> __global__ __launch_bounds__(256) void k_finddt(ArgsT<W> a) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double dtloc = 1.0e5;
-    if (t < (long)a.nx * a.ny * a.nz) {
-        const int j = 1 + (int)(t % a.nx), k = 1 + (int)((t / a.nx) % a.ny), l = 1 + (int)(t / ((long)a.nx * a.ny));
-        const long c = cidx(a, j, k, l);
-        double v[3] = {a.P[1][c], a.P[2][c], a.P[3][c]};
-        double b[3] = {a.Bc[0][c], a.Bc[1][c], a.Bc[2][c]};
-        dtloc = cfl_dt(a, j, k, l, a.cs[c], a.P[0][c], v, b);
-    }
-    dtloc = block_min_f64(dtloc, a.S);
-    if (threadIdx.x == 0) atomicMin(&a.S->dtmin_bits, (unsigned long long)__double_as_longlong(dtloc));
-}
 
 // dt = MIN(1e5, cell minimum), clipped to total_time (main.f90:384-389); a
 // step that starts at or past total_time gets dt = 0 and is a no-op in time.
@@ -191,71 +57,9 @@ __global__ void k_set_dt(DevScalars *S, double dt) {
     S->active = 1;
 }
 
-// FIND_DIVB (mhd_module.f90:133-189) as a max reduction on bit patterns
-template <class W> __global__ __launch_bounds__(256) void k_divb(ArgsT<W> a, unsigned long long *out) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double d = 0.0;
-    if (t < (long)a.nx * a.ny * a.nz) {
-        const int j = 1 + (int)(t % a.nx), k = 1 + (int)((t / a.nx) % a.ny), l = 1 + (int)(t / ((long)a.nx * a.ny));
-        const long c = cidx(a, j, k, l), sy = a.NX, sz = a.NX * a.NY;
-        const double *bx = a.Bin[0], *by = a.Bin[1], *bz = a.Bin[2];
-        double dx = a.ax[0].d[j + 3], dy = a.ax[1].d[k + 3], dz = a.ax[2].d[l + 3];
-        if (a.coord == 0)
-            d = (bx[c] - bx[c - 1]) / (dx) + (by[c] - by[c - sy]) / (dy) + (bz[c] - bz[c - sz]) / (dz);
-        else
-            d = (a.ax[0].xF[j + 3] * bx[c] - a.ax[0].xF[j + 2] * bx[c - 1]) / (a.ax[0].c[j + 3] * dx) +
-                (by[c] - by[c - sy]) / (a.ax[0].c[j + 3] * dy) + (bz[c] - bz[c - sz]) / (dz);
-        d = fabsT(d);
-    }
-    // max of non-negative doubles == max of their bit patterns
-    unsigned long long bits = (unsigned long long)__double_as_longlong(d);
-    bits = block_max_u64(bits);
-    if (threadIdx.x == 0) atomicMax(out, bits);
-}
-
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
-template <class W> static ArgsT<W> make_args(cumc3d_ctx *x) {
-    ArgsT<W> a;
-    memset(&a, 0, sizeof(a));
-    a.nx = x->nx; a.ny = x->ny; a.nz = x->nz;
-    a.ndim = x->d.n_dim;
-    a.coord = x->d.coordinate_flag;
-    a.model = x->d.model;
-    a.rt_gravity = x->d.rt_gravity;
-    a.bn_mode = x->d.hlld_bn_mode;
-    for (int i = 0; i < 6; ++i) a.bc[i] = x->d.boundary_flag[i];
-    memcpy(a.bfac, x->bfac, sizeof(a.bfac));
-    a.NX = x->NX; a.NY = x->NY; a.G = x->G;
-    a.g1 = x->d.ggas - 1.0;
-    a.cfl = x->d.cfl;
-    a.r_sh = x->d.r_sh;
-    for (int d = 0; d < 3; ++d) a.ax[d] = x->ax[d];
-    for (int i = 0; i < 5; ++i) a.P[i] = (W *)x->P[i];
-    a.eps = (W *)x->eps;
-    a.cs = (W *)x->cs;
-    for (int d = 0; d < 3; ++d) {
-        a.Bc[d] = (W *)x->Bc[d];
-        a.Bin[d] = x->B[x->cur][d];
-        a.Bout[d] = x->B[1 - x->cur][d];
-        a.E[d] = (W *)x->E[d];
-        a.Ec[d] = (W *)x->Ec[d];
-        for (int i = 0; i < 7; ++i) a.F[d][i] = (W *)x->F[d][i];
-    }
-    for (int i = 0; i < 8; ++i) a.Uo[i] = x->Uo[i];
-    for (int i = 0; i < 5; ++i) a.U[i] = x->U[i];
-    a.rfl = x->floors[0];
-    a.pfl = x->floors[1];
-    a.efl = x->floors[2];
-    a.S = x->S;
-    a.blockmax = x->blockmax;
-    for (int d = 0; d < 3; ++d) a.mask[d] = x->mask ? x->mask + (size_t)d * x->G : nullptr;
-    return a;
-}
-
-static inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
-
 // SETUP_EQNS parity factors (setup.f90:102-326); returns false on the
 // combinations the reference rejects with STOP.
 static bool setup_bfac(const int *bf, int bfac[6][12]) {
@@ -332,127 +136,12 @@ static void weights(int recon, const std::vector<double> &dd, int n, std::vector
     }
 }
 
-// Segments per column of the y/z sweeps: each thread marches one segment of
-// interfaces along the sweep axis, reusing the reconstruction of the cell it
-// shares with the previous interface.  About march_threads threads in flight
-// (2^19 = 8k waves by default); CUMC3D_MARCH_THREADS overrides it at init.
-static int nseg_for(const cumc3d_ctx *x, int nx, int na, int nd) {
-    long cols = (long)nx * na;
-    long want = x->march_threads / (cols > 0 ? cols : 1);
-    if (want < 1) want = 1;
-    if (want > nd + 1) want = nd + 1;
-    return (int)want;
-}
-
-// ---------------------------------------------------------------------------
-// flux dispatch over (reconstruction, Riemann solver)
-// ---------------------------------------------------------------------------
-template <class W, int R, int S>
-static void launch_sweep_rs(cumc3d_ctx *x, const ArgsT<W> &a, int D, hipStream_t st) {
-    if (D == 0) {
-        long waves = ((long)(x->nx + 2) * x->ny * x->nz + 62) / 63 + 1;
-        unsigned grid = (unsigned)((waves * 64 + 255) / 256);
-        hipLaunchKernelGGL((k_flux_x<W, R, S>), dim3(grid), dim3(256), 0, st, a);
-        if (S == 3 && a.bn_mode == 2)
-            hipLaunchKernelGGL((k_carry<W, R, 0>), dim3(1), dim3(1024), 0, st, a, (int)grid);
-    } else if (D == 1) {
-        int ns = nseg_for(x, x->nx, x->nz, x->ny);
-        unsigned grid = nblk((long)x->nx * x->nz * ns);
-        hipLaunchKernelGGL((k_flux_yz<W, R, S, 1>), dim3(grid), dim3(256), 0, st, a, ns);
-        if (S == 3 && a.bn_mode == 2)
-            hipLaunchKernelGGL((k_carry<W, R, 1>), dim3(1), dim3(1024), 0, st, a, (int)grid);
-    } else {
-        int ns = nseg_for(x, x->nx, x->ny, x->nz);
-        unsigned grid = nblk((long)x->nx * x->ny * ns);
-        hipLaunchKernelGGL((k_flux_yz<W, R, S, 2>), dim3(grid), dim3(256), 0, st, a, ns);
-        if (S == 3 && a.bn_mode == 2)
-            hipLaunchKernelGGL((k_carry<W, R, 2>), dim3(1), dim3(1024), 0, st, a, (int)grid);
-    }
-}
-
-template <class W, int R>
-static void launch_sweep_r(cumc3d_ctx *x, const ArgsT<W> &a, int D, hipStream_t st) {
-    switch (x->d.riemann) {
-        case 0: launch_sweep_rs<W, R, 0>(x, a, D, st); break;
-        case 1: launch_sweep_rs<W, R, 1>(x, a, D, st); break;
-        case 2: launch_sweep_rs<W, R, 2>(x, a, D, st); break;
-        default: launch_sweep_rs<W, R, 3>(x, a, D, st); break;
-    }
-}
-
-template <class W> static void launch_sweep(cumc3d_ctx *x, const ArgsT<W> &a, int D, hipStream_t st) {
-    if (x->sel_on) {
-        launch_sweep_r<W, RSEL>(x, a, D, st);
-        return;
-    }
-    switch (x->d.recon) {
-        case 0: launch_sweep_r<W, 0>(x, a, D, st); break;
-        case 1: launch_sweep_r<W, 1>(x, a, D, st); break;
-        case 2: launch_sweep_r<W, 2>(x, a, D, st); break;
-        case 3: launch_sweep_r<W, 3>(x, a, D, st); break;
-        default: launch_sweep_r<W, 4>(x, a, D, st); break;
-    }
-}
-
-static long blocks_needed(cumc3d_ctx *x) {
-    long waves = ((long)(x->nx + 2) * x->ny * x->nz + 62) / 63 + 1;
-    long bx = (waves * 64 + 255) / 256;
-    long by = nblk((long)x->nx * x->nz * nseg_for(x, x->nx, x->nz, x->ny));
-    long bz = nblk((long)x->nx * x->ny * nseg_for(x, x->nx, x->ny, x->nz));
-    long m = bx > by ? bx : by;
-    return m > bz ? m : bz;
-}
-
-// ---------------------------------------------------------------------------
-// stage / step
-// ---------------------------------------------------------------------------
-static constexpr int TSTAGES = 192;
-static constexpr int TMARKS = 9;   // 0..6 stage kernels, 7..8 selector inference
-static inline void tmark(cumc3d_ctx *x, int slot) {
-    if (x->timing && x->tstage < TSTAGES) hipEventRecord(x->ev[x->tstage * TMARKS + slot], x->stream);
-}
-
-template <class W> static void run_stage_t(cumc3d_ctx *x, int stage) {
-    hipStream_t st = x->stream;
-    ArgsT<W> a = make_args<W>(x);
-    const long nn = (long)x->nx * x->ny * x->nz;
-    tmark(x, 0);
-    launch_sweep(x, a, 0, st);
-    tmark(x, 1);
-    if (x->d.n_dim > 1) launch_sweep(x, a, 1, st);
-    tmark(x, 2);
-    if (x->d.n_dim > 2) launch_sweep(x, a, 2, st);
-    tmark(x, 3);
-    hipLaunchKernelGGL(k_emf_prep<W>, dim3(nblk((long)(x->nx + 2) * (x->ny + 2) * (x->nz + 2))), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_edge<W>, dim3(nblk((long)(x->nx + 1) * (x->ny + 1) * (x->nz + 1))), dim3(256), 0, st, a);
-    tmark(x, 4);
-    if (stage == 1) hipLaunchKernelGGL((k_update<1, W>), dim3(nblk(nn)), dim3(256), 0, st, a);
-    else if (stage == 2) hipLaunchKernelGGL((k_update<2, W>), dim3(nblk(nn)), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_update<3, W>), dim3(nblk(nn)), dim3(256), 0, st, a);
-    tmark(x, 5);
-    x->cur = 1 - x->cur;
-    ArgsT<W> g = make_args<W>(x);
-    hipLaunchKernelGGL(k_ghost<W>, dim3(nblk(x->G)), dim3(256), 0, st, g);
-    tmark(x, 6);
-    if (x->timing && x->tstage < TSTAGES) x->tstage += 1;
-}
-
-template <class W> static void sel_infer_t(cumc3d_ctx *x) {
-    ArgsT<W> a = make_args<W>(x);
-    for (int D = 0; D < x->d.n_dim; ++D) {
-        unsigned char *m = x->mask + (size_t)D * x->G;
-        if (D == 0) hipLaunchKernelGGL((k_sel_infer<W, 0>), dim3(2048), dim3(256), 0, x->stream, a, x->sel_frag, x->sel_w, m);
-        else if (D == 1) hipLaunchKernelGGL((k_sel_infer<W, 1>), dim3(2048), dim3(256), 0, x->stream, a, x->sel_frag, x->sel_w, m);
-        else hipLaunchKernelGGL((k_sel_infer<W, 2>), dim3(2048), dim3(256), 0, x->stream, a, x->sel_frag, x->sel_w, m);
-    }
-}
-
 // class masks from the stage-1 state of a step (kept for stages 2 and 3)
 static void sel_refresh(cumc3d_ctx *x) {
     if (!x->sel_on) return;
     if (x->sel_tick++ % x->sel_every != 0) return;
     tmark(x, 7);
-    if (x->fp32) sel_infer_t<float>(x);
+    if (x->fp32) f32::sel_infer(x);
     else sel_infer_t<double>(x);
     tmark(x, 8);
     if (x->timing && x->tstage < TSTAGES) x->sel_timed[x->tstage] = 1;
@@ -460,7 +149,7 @@ static void sel_refresh(cumc3d_ctx *x) {
 
 static void run_stage(cumc3d_ctx *x, int stage) {
     if (stage == 1) sel_refresh(x);
-    if (x->fp32) run_stage_t<float>(x, stage);
+    if (x->fp32) f32::run_stage(x, stage);
     else run_stage_t<double>(x, stage);
 }
 
@@ -486,7 +175,7 @@ static void accumulate_timing(cumc3d_ctx *x) {
 
 static int launch_finddt(cumc3d_ctx *x) {
     const long nn = (long)x->nx * x->ny * x->nz;
-    if (x->fp32) hipLaunchKernelGGL(k_finddt<float>, dim3(nblk(nn)), dim3(256), 0, x->stream, make_args<float>(x));
+    if (x->fp32) f32::finddt(x);
     else hipLaunchKernelGGL(k_finddt<double>, dim3(nblk(nn)), dim3(256), 0, x->stream, make_args<double>(x));
     return 0;
 }
@@ -646,8 +335,7 @@ int cumc3d_upload(cumc3d_ctx *x, const double *prim, const double *eps, const do
     HIPCHK(hipMemcpyAsync(tmp + (size_t)8 * x->G, eps, x->G * sizeof(double), hipMemcpyHostToDevice, x->stream));
     x->cur = 0;
     if (x->fp32)
-        hipLaunchKernelGGL(k_split_prim<float>, dim3(nblk(x->G)), dim3(256), 0, x->stream, tmp, tmp + (size_t)8 * x->G,
-                           make_args<float>(x));
+        f32::split_prim(x, tmp, tmp + (size_t)8 * x->G);
     else
         hipLaunchKernelGGL(k_split_prim<double>, dim3(nblk(x->G)), dim3(256), 0, x->stream, tmp, tmp + (size_t)8 * x->G,
                            make_args<double>(x));
@@ -668,8 +356,7 @@ int cumc3d_initial_update(cumc3d_ctx *x) {
     hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, x->stream, x->S, 0.0);   // also re-activates the stage kernels
     const unsigned gi = nblk((long)x->nx * x->ny * x->nz), gg = nblk(x->G);
     if (x->fp32) {
-        hipLaunchKernelGGL(k_init_interior<float>, dim3(gi), dim3(256), 0, x->stream, make_args<float>(x));
-        hipLaunchKernelGGL(k_ghost<float>, dim3(gg), dim3(256), 0, x->stream, make_args<float>(x));
+        f32::initial_update(x);
     } else {
         hipLaunchKernelGGL(k_init_interior<double>, dim3(gi), dim3(256), 0, x->stream, make_args<double>(x));
         hipLaunchKernelGGL(k_ghost<double>, dim3(gg), dim3(256), 0, x->stream, make_args<double>(x));
@@ -793,15 +480,14 @@ int cumc3d_download(cumc3d_ctx *x, double *prim, double *eps, double *bcell, dou
     if (!tmp && x->fp32 && (eps || cs)) HIPCHK(hipMalloc(&tmp, (size_t)x->G * sizeof(double)));
     const unsigned gg = nblk(x->G);
     if (prim) {
-        if (x->fp32) hipLaunchKernelGGL(k_join_prim<float>, dim3(gg), dim3(256), 0, x->stream, tmp, make_args<float>(x));
+        if (x->fp32) f32::join_prim(x, tmp);
         else hipLaunchKernelGGL(k_join_prim<double>, dim3(gg), dim3(256), 0, x->stream, tmp, make_args<double>(x));
         HIPCHK(hipMemcpyAsync(prim, tmp, (size_t)8 * x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
         HIPCHK(hipStreamSynchronize(x->stream));
     }
     if (bcell) {
         if (x->fp32)
-            hipLaunchKernelGGL(k_join3<float>, dim3(gg), dim3(256), 0, x->stream, tmp, (const float *)x->Bc[0],
-                               (const float *)x->Bc[1], (const float *)x->Bc[2], x->G);
+            f32::join_bcell(x, tmp);
         else
             hipLaunchKernelGGL(k_join3<double>, dim3(gg), dim3(256), 0, x->stream, tmp, (const double *)x->Bc[0],
                                (const double *)x->Bc[1], (const double *)x->Bc[2], x->G);
@@ -813,7 +499,7 @@ int cumc3d_download(cumc3d_ctx *x, double *prim, double *eps, double *bcell, dou
     for (int i = 0; i < 2; ++i) {
         if (!outs[i]) continue;
         if (x->fp32) {
-            hipLaunchKernelGGL(k_widen<float>, dim3(gg), dim3(256), 0, x->stream, tmp, (const float *)scal[i], x->G);
+            f32::widen(x, tmp, scal[i]);
             HIPCHK(hipMemcpyAsync(outs[i], tmp, x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
         } else {
             HIPCHK(hipMemcpyAsync(outs[i], scal[i], x->G * sizeof(double), hipMemcpyDeviceToHost, x->stream));
@@ -926,24 +612,6 @@ int cumc3d_selector_get_params(cumc3d_ctx *x, float *w, int n) {
 
 }  // extern "C"
 
-template <class W> static void sel_collect_t(cumc3d_ctx *x, int stride, unsigned seed, float t1, float t2) {
-    ArgsT<W> a = make_args<W>(x);
-    for (int D = 0; D < x->d.n_dim; ++D) {
-        long nq = (long)(x->nx + (D == 0 ? 2 : 0)) * (x->ny + (D == 1 ? 2 : 0)) * (x->nz + (D == 2 ? 2 : 0));
-        unsigned g = nblk((nq + stride - 1) / stride);
-        unsigned sd = seed * 3u + (unsigned)D;
-        if (D == 0)
-            hipLaunchKernelGGL((k_sel_collect<W, 0>), dim3(g), dim3(256), 0, x->stream, a, stride, sd, t1, t2,
-                               x->samp_feat, x->samp_label, x->samp_cap, x->samp_counts);
-        else if (D == 1)
-            hipLaunchKernelGGL((k_sel_collect<W, 1>), dim3(g), dim3(256), 0, x->stream, a, stride, sd, t1, t2,
-                               x->samp_feat, x->samp_label, x->samp_cap, x->samp_counts);
-        else
-            hipLaunchKernelGGL((k_sel_collect<W, 2>), dim3(g), dim3(256), 0, x->stream, a, stride, sd, t1, t2,
-                               x->samp_feat, x->samp_label, x->samp_cap, x->samp_counts);
-    }
-}
-
 extern "C" {
 
 int cumc3d_selector_collect(cumc3d_ctx *x, int stride, double thr_ppm, double thr_weno, unsigned seed, int *counts) {
@@ -951,7 +619,7 @@ int cumc3d_selector_collect(cumc3d_ctx *x, int stride, double thr_ppm, double th
     if (!x->sel_w) return fail(CUMC3D_ESTATE, "call cumc3d_selector_init first");
     if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
     if (stride < 1) return fail(CUMC3D_EINVAL, "stride must be >= 1");
-    if (x->fp32) sel_collect_t<float>(x, stride, seed, (float)thr_ppm, (float)thr_weno);
+    if (x->fp32) f32::sel_collect(x, stride, seed, (float)thr_ppm, (float)thr_weno);
     else sel_collect_t<double>(x, stride, seed, (float)thr_ppm, (float)thr_weno);
     HIPCHK(hipGetLastError());
     int h[4];
@@ -1033,7 +701,7 @@ int cumc3d_selector_mask(cumc3d_ctx *x, int dir, int refresh, unsigned char *out
     if (!x->sel_w) return fail(CUMC3D_ESTATE, "call cumc3d_selector_init first");
     if (dir < 0 || dir >= x->d.n_dim) return fail(CUMC3D_EINVAL, "direction out of range");
     if (refresh) {
-        if (x->fp32) sel_infer_t<float>(x);
+        if (x->fp32) f32::sel_infer(x);
         else sel_infer_t<double>(x);
     }
     HIPCHK(hipMemcpyAsync(out, x->mask + (size_t)dir * x->G, x->G, hipMemcpyDeviceToHost, x->stream));

@@ -123,7 +123,7 @@ template <int D, class A> __device__ __forceinline__ long sel_count(const A &a) 
 //   = (0,0),(0,1),(1,0),(1,1);  6,7: layer 3, input tile 0/1 (outputs 0..2).
 
 // Pack the FP32 parameters into the BF16 A fragments (64 lanes x 8 fragments x 4).
-__global__ void k_sel_pack(const float *__restrict__ w, unsigned short *__restrict__ frag) {
+static __global__ void k_sel_pack(const float *__restrict__ w, unsigned short *__restrict__ frag) {
     const int t = threadIdx.x;   // 0..511: lane + 64*fragment
     if (t >= 64 * SEL_NFRAG) return;
     const int lane = t & 63, fr = t >> 6, r = lane & 15, g = lane >> 4;
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(256) void k_sel_infer(ArgsT<W> a, const unsigned sh
 }
 
 // Logits of given feature rows (testing the MFMA path against a host model).
-__global__ __launch_bounds__(64) void k_sel_mlp_rows(const unsigned short *frag, const float *w, const float *feat,
+static __global__ __launch_bounds__(64) void k_sel_mlp_rows(const unsigned short *frag, const float *w, const float *feat,
                                                      long n, float *logits) {
     SelNet net;
     sel_load_net(frag, w, net);
@@ -322,7 +322,7 @@ constexpr int SEL_TB = 128;   // samples (threads) per training block
 // runs forward and backward in FP32 (weights rounded to BF16 as inference
 // sees them), and writes its summed gradient and loss.  Parameter gradients
 // are batch dot products over the block's activations held in LDS.
-__global__ __launch_bounds__(SEL_TB) void k_sel_grad(const float *__restrict__ w, const float *__restrict__ feat,
+static __global__ __launch_bounds__(SEL_TB) void k_sel_grad(const float *__restrict__ w, const float *__restrict__ feat,
                                                      const unsigned char *__restrict__ label, int n, unsigned seed,
                                                      float cw0, float cw1, float cw2, float *part, float *lossp) {
     __shared__ float sw[SEL_NPARAM];
@@ -421,7 +421,7 @@ __global__ __launch_bounds__(SEL_TB) void k_sel_grad(const float *__restrict__ w
 }
 
 // Adam step on the mean gradient of nblk blocks; loss_out = mean loss.
-__global__ __launch_bounds__(256) void k_sel_adam(float *w, float *m, float *v, const float *part, const float *lossp,
+static __global__ __launch_bounds__(256) void k_sel_adam(float *w, float *m, float *v, const float *part, const float *lossp,
                                                   int nblk, float lr, int step, float *loss_out) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const float inv = 1.0f / (float)(nblk * SEL_TB);
