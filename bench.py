@@ -94,7 +94,7 @@ def make_workload(name: str, n: int):
     raise SystemExit(f"unknown workload {name}")
 
 
-def train_selector(n: int, workload: str, thr_ppm: float, thr_weno: float, seed: int = 1):
+def train_selector(n: int, workload: str, thr_ppm: float, thr_weno: float, seed: int = 1, iters: int = 600):
     """Train the selector on samples the FP32 solver generates from a small run
     of the same physics (n^3, snapshots every 3 steps); returns its parameters
     and the training summary."""
@@ -106,16 +106,31 @@ def train_selector(n: int, workload: str, thr_ppm: float, thr_weno: float, seed:
     for k in range(6):
         s.run(3, total_time=1e300)
         counts = s.selector_collect(stride=2, thr_ppm=thr_ppm, thr_weno=thr_weno, seed=k + 1)
-    loss = s.selector_train(600, blocks=128, lr=3e-3, class_weight=(1.0, 1.0, 2.0), seed=seed)
+    loss = s.selector_train(iters, blocks=128, lr=3e-3, class_weight=(1.0, 1.0, 2.0), seed=seed)
     w = s.selector_params()
     del s
     info = {"train_grid": f"{n}^3", "samples": int(counts[0]), "label_counts": counts[1:].tolist(),
             "thresholds": [thr_ppm, thr_weno], "loss_first": float(loss[:10].mean()),
-            "loss_last": float(loss[-10:].mean()), "iters": 600}
+            "loss_last": float(loss[-10:].mean()), "iters": iters}
     return w, info
 
 
 SEL_EVERY = 1   # selector refresh interval in steps (set from --sel-every)
+
+
+def calibration_copy(nbytes: int = 1 << 30):
+    """One device-to-device copy of nbytes through the HIP runtime: rocprofv3
+    counters of its copy kernel give the factor between FETCH_SIZE /
+    WRITE_SIZE and bytes moved."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def make_variant(cfg, st, variant: str, sel_w=None):
@@ -258,7 +273,11 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-l1", action="store_true")
     ap.add_argument("--train-n", type=int, default=64)
-    ap.add_argument("--sel-every", type=int, default=1)
+    ap.add_argument("--train-iters", type=int, default=600)
+    ap.add_argument("--variants", default="fp64,fp32,fp32_sel")
+    ap.add_argument("--pmc-calibrate", action="store_true",
+                    help="copy 1 GiB device to device first (the known-size kernel that scales the HBM counters)")
+    ap.add_argument("--sel-every", type=int, default=4)
     ap.add_argument("--thr-ppm", type=float, default=1e-3)
     ap.add_argument("--thr-weno", type=float, default=1e-2)
     args = ap.parse_args(argv)
@@ -272,11 +291,14 @@ def main(argv=None):
     print(f"[bench] training the selector on a {args.train_n}^3 run", file=sys.stderr, flush=True)
     global SEL_EVERY
     SEL_EVERY = args.sel_every
-    sel_w, sel_info = train_selector(args.train_n, args.workload, args.thr_ppm, args.thr_weno)
+    if args.pmc_calibrate:
+        calibration_copy()
+    sel_w, sel_info = train_selector(args.train_n, args.workload, args.thr_ppm, args.thr_weno,
+                                     iters=args.train_iters)
     sel_info["refresh_every"] = SEL_EVERY
     print(f"[bench] selector: {sel_info}", file=sys.stderr, flush=True)
     res = {}
-    for var in ("fp64", "fp32", "fp32_sel"):
+    for var in args.variants.split(","):
         print(f"[bench] {label}: {var} warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
         res[var] = time_variant(cfg, st, var, args.warmup, args.steps, sel_w)
         print(f"[bench] {var}: {1e3 * res[var][0] / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
@@ -284,10 +306,11 @@ def main(argv=None):
     def rate(v):
         return zones * args.steps / res[v][0]
 
-    wall, kt, frac = res["fp32_sel"]
-    value = rate("fp32_sel")
+    main_var = "fp32_sel" if "fp32_sel" in res else list(res)[-1]
+    wall, kt, frac = res[main_var]
+    value = rate(main_var)
     dom = max(KERNELS, key=lambda k: kt[k])
-    achieved = kernel_bytes(cfg, dom, 4) / (kt[dom] * 1e-3) / 1e9
+    achieved = kernel_bytes(cfg, dom, 8 if main_var == "fp64" else 4) / (kt[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
                 "kernel_ms": kt[dom],
@@ -316,7 +339,7 @@ def main(argv=None):
         "config": {"workload": label + ", FP32 split + learned selector (FP64 accumulators)", "zones": zones,
                    "riemann": cfg.riemann, "recon": "selector: TVD-MC / PPM / WENO-Z per cell"},
         "variants": {v: {"value": rate(v), "ms_per_step": 1e3 * res[v][0] / args.steps} for v in res},
-        "speedup_vs_fp64": value / rate("fp64"),
+        "speedup_vs_fp64": value / rate("fp64") if "fp64" in res else None,
         "selector": dict(sel_info, class_fractions=frac),
         "l1_vs_fp64": err,
         "roofline": roofline,

@@ -359,7 +359,7 @@ int cumc3d_initial_update(cumc3d_ctx *x) {
         f32::initial_update(x);
     } else {
         hipLaunchKernelGGL(k_init_interior<double>, dim3(gi), dim3(256), 0, x->stream, make_args<double>(x));
-        hipLaunchKernelGGL(k_ghost<double>, dim3(gg), dim3(256), 0, x->stream, make_args<double>(x));
+        hipLaunchKernelGGL(k_ghost<double>, dim3(nblk(ghost_count(*x))), dim3(256), 0, x->stream, make_args<double>(x));
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(x->stream));

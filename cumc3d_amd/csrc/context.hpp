@@ -120,6 +120,7 @@ template <class W> __global__ void k_init_interior(ArgsT<W> a) {
     double b[3] = {bavg<0>(a, j, a.Bin[0][c], a.Bin[0][c - 1]), bavg<1>(a, j, a.Bin[1][c], a.Bin[1][c - a.NX]),
                    bavg<2>(a, j, a.Bin[2][c], a.Bin[2][c - a.NX * a.NY])};
     for (int d = 0; d < 3; ++d) a.Bc[d][c] = (W)b[d];
+    cell_emf(a, c, (W)v[0], (W)v[1], (W)v[2], (W)b[0], (W)b[1], (W)b[2]);
     double p, cs;
     pressure(a, c, rho, eps, p, cs);
     a.P[4][c] = (W)p;
@@ -305,7 +306,7 @@ template <class W> static void run_stage_t(cumc3d_ctx *x, int stage) {
     tmark(x, 2);
     if (x->d.n_dim > 2) launch_sweep(x, a, 2, st);
     tmark(x, 3);
-    hipLaunchKernelGGL(k_emf_prep<W>, dim3(nblk((long)(x->nx + 2) * (x->ny + 2) * (x->nz + 2))), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_emf_prep<W>, dim3(nblk(emf_prep_count(a))), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_edge<W>, dim3(nblk((long)(x->nx + 1) * (x->ny + 1) * (x->nz + 1))), dim3(256), 0, st, a);
     tmark(x, 4);
     if (stage == 1) hipLaunchKernelGGL((k_update<1, W>), dim3(nblk(nn)), dim3(256), 0, st, a);
@@ -314,7 +315,7 @@ template <class W> static void run_stage_t(cumc3d_ctx *x, int stage) {
     tmark(x, 5);
     x->cur = 1 - x->cur;
     ArgsT<W> g = make_args<W>(x);
-    hipLaunchKernelGGL(k_ghost<W>, dim3(nblk(x->G)), dim3(256), 0, st, g);
+    hipLaunchKernelGGL(k_ghost<W>, dim3(nblk(ghost_count(*x))), dim3(256), 0, st, g);
     tmark(x, 6);
     if (x->timing && x->tstage < TSTAGES) x->tstage += 1;
 }

@@ -29,7 +29,7 @@ void split_prim(cumc3d_ctx *x, const double *aos, const double *eps64) {
 void initial_update(cumc3d_ctx *x) {
     const unsigned gi = nblk((long)x->nx * x->ny * x->nz), gg = nblk(x->G);
     hipLaunchKernelGGL(k_init_interior<float>, dim3(gi), dim3(256), 0, x->stream, make_args<float>(x));
-    hipLaunchKernelGGL(k_ghost<float>, dim3(gg), dim3(256), 0, x->stream, make_args<float>(x));
+    hipLaunchKernelGGL(k_ghost<float>, dim3(nblk(ghost_count(*x))), dim3(256), 0, x->stream, make_args<float>(x));
 }
 
 void join_prim(cumc3d_ctx *x, double *tmp) {

@@ -154,18 +154,42 @@ template <class T> __device__ __forceinline__ void load_w(const double *w, T out
 
 template <int R> constexpr int nweights() { return R <= 2 ? 2 : (R == 3 ? 14 : 10); }
 
+// Stencil width a reconstruction loads: 3 cells for TVD, 5 otherwise.
+template <int R> constexpr int stencil_r() { return R == RSEL ? 3 : R; }
+
+// Slide the stencil of the previous cell along the sweep to cell c: the
+// values move down one place and only the cell entering the stencil is loaded.
 template <class T, int R, int D>
-__device__ __forceinline__ void recon_at(const ArgsT<T> &a, int j, int k, int l, long st, Face<T> &fm, Face<T> &fp) {
-    long c = cidx(a, j, k, l);
-    int i = (D == 0) ? j : (D == 1 ? k : l);
+__device__ __forceinline__ void slide_stencil(const ArgsT<T> &a, long c, long st, T s[7][5]) {
+    constexpr int t1 = (D == 0) ? 1 : 0;
+    constexpr int t2 = (D == 2) ? 1 : 2;
+    const T *src[7] = {a.P[0], a.P[1], a.P[2], a.P[3], a.P[4], a.Bc[t1], a.Bc[t2]};
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+        if constexpr (stencil_r<R>() >= 3) {
+            s[v][0] = s[v][1];
+            s[v][1] = s[v][2];
+            s[v][2] = s[v][3];
+            s[v][3] = s[v][4];
+            s[v][4] = (T)ldf(src[v], c + 2 * st);
+        } else {
+            s[v][1] = s[v][2];
+            s[v][2] = s[v][3];
+            s[v][3] = (T)ldf(src[v], c + st);
+        }
+    }
+}
+
+// Both faces of cell c (index i along D) from its loaded stencil.
+template <class T, int R, int D>
+__device__ __forceinline__ void recon_stencil(const ArgsT<T> &a, long c, int i, const T s[7][5], Face<T> &fm,
+                                              Face<T> &fp) {
     if constexpr (R == RSEL) {
         // the highest class among the wave's active lanes, so the wave runs
         // one reconstruction (ballots see active lanes only)
         const int own = a.mask[D][c];
         const unsigned long long b2 = __ballot(own == 2), b1 = __ballot(own >= 1);
         const int cls = b2 ? 2 : (b1 ? 1 : 0);
-        T s[7][5];
-        load_stencil<T, 3, D>(a, c, st, s);
         T w[NW];
         if (cls == 0) {
             load_w<T>(a.ax[D].wsel[0] + (long)(i + 3) * NW, w, 2);
@@ -178,12 +202,18 @@ __device__ __forceinline__ void recon_at(const ArgsT<T> &a, int j, int k, int l,
             recon_cell<T, 4>(w, s, (T)a.g1, fm, fp);
         }
     } else {
-        T s[7][5];
-        load_stencil<T, R, D>(a, c, st, s);
         T w[NW];
         load_w<T>(a.ax[D].w + (long)(i + 3) * NW, w, nweights<R>());
         recon_cell<T, R>(w, s, (T)a.g1, fm, fp);
     }
+}
+
+template <class T, int R, int D>
+__device__ __forceinline__ void recon_at(const ArgsT<T> &a, int j, int k, int l, long st, Face<T> &fm, Face<T> &fp) {
+    const long c = cidx(a, j, k, l);
+    T s[7][5];
+    load_stencil<T, stencil_r<R>(), D>(a, c, st, s);
+    recon_stencil<T, R, D>(a, c, (D == 0) ? j : (D == 1 ? k : l), s, fm, fp);
 }
 
 template <int D, class A> __device__ __forceinline__ long stride(const A &a) {
@@ -289,14 +319,33 @@ __global__ __launch_bounds__(256) void k_flux_yz(ArgsT<T> a, int nseg) {
         int i1 = (int)(((long)(s + 1) * (nd + 1)) / nseg);
         const long st = stride<D>(a);
         Face<T> prev, fm;
-        recon_at<T, R, D>(a, (D == 1) ? j : j, (D == 1) ? i0 : o2, (D == 1) ? o2 : i0, st, fm, prev);
+        // with the selector (single precision) the stencil stays in registers
+        // and slides one cell per step: one new load per field instead of
+        // five.  The fixed-scheme kernels reload it from cache; the window's
+        // registers cost them more occupancy than the loads they save
+        // (measured on MI355X at 256^3)
+        constexpr bool slide = sizeof(T) == 4 && R == RSEL;
+        T win[slide ? 7 : 1][5];
+        long cc = cidx(a, j, (D == 1) ? i0 : o2, (D == 1) ? o2 : i0);
+        if constexpr (slide) {
+            load_stencil<T, stencil_r<R>(), D>(a, cc, st, win);
+            recon_stencil<T, R, D>(a, cc, i0, win, fm, prev);
+        } else {
+            recon_at<T, R, D>(a, j, (D == 1) ? i0 : o2, (D == 1) ? o2 : i0, st, fm, prev);
+        }
         const T carry = (T)a.S->carry[D];
 #pragma unroll 1
         for (int i = i0 + 1; i <= i1; ++i) {
             int k = (D == 1) ? i : o2;
             int l = (D == 1) ? o2 : i;
             Face<T> fp;
-            recon_at<T, R, D>(a, j, k, l, st, fm, fp);
+            cc += st;
+            if constexpr (slide) {
+                slide_stencil<T, R, D>(a, cc, st, win);
+                recon_stencil<T, R, D>(a, cc, i, win, fm, fp);
+            } else {
+                recon_at<T, R, D>(a, j, k, l, st, fm, fp);
+            }
             int kk = (D == 1) ? i - 1 : k;
             int ll = (D == 1) ? l : i - 1;
             long c = cidx(a, j, kk, ll);
@@ -371,6 +420,16 @@ template <int E, class W> __device__ __forceinline__ W ecell_raw(const ArgsT<W> 
     else return vy * bx - vx * by;
 }
 
+// Store the cell EMFs of cell c from its velocity and cell-centred field
+// (times a parity factor per component for mapped ghost cells).
+template <class W>
+__device__ __forceinline__ void cell_emf(const ArgsT<W> &a, long c, W vx, W vy, W vz, W bx, W by, W bz, W s0 = W(1),
+                                         W s1 = W(1), W s2 = W(1)) {
+    a.Ec[0][c] = s0 * (vz * by - vy * bz);
+    a.Ec[1][c] = s1 * (vx * bz - vz * bx);
+    a.Ec[2][c] = s2 * (vy * bx - vx * by);
+}
+
 // ecell after BOUNDARY_EFIELD (mhd_module.f90:478-840): ghost cells take the
 // mapped interior value times the parity factors of component E.
 template <int E, class W> __device__ __forceinline__ W ecell_at(const ArgsT<W> &a, int j, int k, int l) {
@@ -405,20 +464,51 @@ template <int E, int D, class W> __device__ __forceinline__ W eface_at(const Arg
     return (W)(gsign(a, 0, sx, 9 + E) * gsign(a, 1, sy, 9 + E) * gsign(a, 2, sz, 9 + E)) * v;
 }
 
-// Cell and face EMFs on the cells 0..n+1 the edge stencil reads (flux_ct,
-// mhd_module.f90:310-360): ecell of every cell (ghosts mapped as in
-// BOUNDARY_EFIELD) and, for face EMFs, the ghost rows along their two
-// cell-centred axes plus the faces of inactive directions (1D/2D fill).
+// Face EMFs on the cells 0..n+1 the edge stencil reads (flux_ct,
+// mhd_module.f90:310-360, ghosts mapped as in BOUNDARY_EFIELD): the ghost rows
+// along their two cell-centred axes plus the faces of inactive directions
+// (1D/2D fill).  The cell EMFs (ecell) are stored by the update for interior
+// cells and by the ghost fill (mapped) for ghost cells.
+// Cells of the (n+2)^3 box 0..n+1 the face-EMF fill visits: in 3D only its
+// outer shell (a ghost index on some axis), in 1D/2D every cell (the
+// inactive directions' faces are filled everywhere).
+template <class A> __host__ __device__ __forceinline__ long emf_prep_count(const A &a) {
+    const long m = (long)(a.nx + 2) * (a.ny + 2) * (a.nz + 2);
+    return a.ndim == 3 ? m - (long)a.nx * a.ny * a.nz : m;
+}
+
 template <class W> __global__ __launch_bounds__(256) void k_emf_prep(ArgsT<W> a) {
     if (!a.S->active) return;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long n0 = a.nx + 2, n1 = a.ny + 2;
-    if (t >= n0 * n1 * (a.nz + 2)) return;
-    const int j = (int)(t % n0), k = (int)((t / n0) % n1), l = (int)(t / (n0 * n1));
+    if (t >= emf_prep_count(a)) return;
+    int j, k, l;
+    if (a.ndim == 3) {
+        // shell of the box: the two z planes, the two y rows of each inner
+        // plane, the two x cells of each inner row
+        const long n0 = a.nx + 2, n1 = a.ny + 2, pz = 2 * n0 * n1, py = 2 * n0 * (long)a.nz;
+        long q = t;
+        if (q < pz) {
+            l = (q / (n0 * n1)) == 0 ? 0 : a.nz + 1;
+            k = (int)((q % (n0 * n1)) / n0);
+            j = (int)(q % n0);
+        } else if (q < pz + py) {
+            q -= pz;
+            l = 1 + (int)(q / (2 * n0));
+            k = ((q % (2 * n0)) / n0) == 0 ? 0 : a.ny + 1;
+            j = (int)(q % n0);
+        } else {
+            q -= pz + py;
+            l = 1 + (int)(q / (2L * a.ny));
+            k = 1 + (int)((q % (2L * a.ny)) / 2);
+            j = (q % 2) == 0 ? 0 : a.nx + 1;
+        }
+    } else {
+        const long n0 = a.nx + 2, n1 = a.ny + 2;
+        j = (int)(t % n0);
+        k = (int)((t / n0) % n1);
+        l = (int)(t / (n0 * n1));
+    }
     const long c = cidx(a, j, k, l);
-    a.Ec[0][c] = ecell_at<0>(a, j, k, l);
-    a.Ec[1][c] = ecell_at<1>(a, j, k, l);
-    a.Ec[2][c] = ecell_at<2>(a, j, k, l);
     const bool gx = j < 1 || j > a.nx, gy = k < 1 || k > a.ny, gz = l < 1 || l > a.nz;
     // x faces (face index j <= nx), cell axes y, z
     if (j <= a.nx && (a.ndim < 1 || gy || gz)) {
@@ -466,26 +556,26 @@ template <class W> __global__ __launch_bounds__(256) void k_edge(ArgsT<W> a) {
 // CUSTOM_CHECKRHO + FROMRVETOU + FINDPRESSURE (+ FINDDT) for one interior cell
 // ---------------------------------------------------------------------------
 
-// l_rk of the face fields (l_operator, spatial.f90:639-680); the edge EMFs
-// are read in the working type and differenced in double
-template <class A> __device__ __forceinline__ double lB_x(const A &a, int jf, int k, int l) {
-    long c = cidx(a, jf, k, l), sy = a.NX, sz = a.NX * a.NY;
-    const double ez0 = a.E[2][c], ez1 = a.E[2][c - sy], ey0 = a.E[1][c], ey1 = a.E[1][c - sz];
+// l_rk of the face fields (l_operator, spatial.f90:639-680) from the four
+// edge EMFs around the face (read in the working type, differenced in double):
+// x face jf: ez at (jf,k,l),(jf,k-1,l), ey at (jf,k,l),(jf,k,l-1);
+// y face kf: ex at (j,kf,l),(j,kf,l-1), ez at (j,kf,l),(j-1,kf,l);
+// z face lf: ey at (j,k,lf),(j-1,k,lf), ex at (j,k,lf),(j,k-1,lf).
+template <class A>
+__device__ __forceinline__ double lB_x(const A &a, int jf, int k, int l, double ez0, double ez1, double ey0, double ey1) {
     double dy = a.ax[1].d[k + 3], dz = a.ax[2].d[l + 3];
     if (a.coord == 0) return -((ez0 - ez1) / (dy) - (ey0 - ey1) / (dz));
     double xF = a.ax[0].xF[jf + 3];
     return -(ez0 - ez1) / (xF * dy) + (ey0 - ey1) / (dz);
 }
-template <class A> __device__ __forceinline__ double lB_y(const A &a, int j, int kf, int l) {
-    long c = cidx(a, j, kf, l), sz = a.NX * a.NY;
-    const double ex0 = a.E[0][c], ex1 = a.E[0][c - sz], ez0 = a.E[2][c], ez1 = a.E[2][c - 1];
+template <class A>
+__device__ __forceinline__ double lB_y(const A &a, int j, int l, double ex0, double ex1, double ez0, double ez1) {
     double dx = a.ax[0].d[j + 3], dz = a.ax[2].d[l + 3];
     if (a.coord == 0) return -((ex0 - ex1) / (dz) - (ez0 - ez1) / (dx));
     return -(ex0 - ex1) / (dz) + (ez0 - ez1) / (dx);
 }
-template <class A> __device__ __forceinline__ double lB_z(const A &a, int j, int k, int lf) {
-    long c = cidx(a, j, k, lf), sy = a.NX;
-    const double ey0 = a.E[1][c], ey1 = a.E[1][c - 1], ex0 = a.E[0][c], ex1 = a.E[0][c - sy];
+template <class A>
+__device__ __forceinline__ double lB_z(const A &a, int j, int k, double ey0, double ey1, double ex0, double ex1) {
     double dx = a.ax[0].d[j + 3], dy = a.ax[1].d[k + 3];
     if (a.coord == 0) return -((ey0 - ey1) / (dx) - (ex0 - ex1) / (dy));
     double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2], x = a.ax[0].c[j + 3];
@@ -584,167 +674,189 @@ __device__ __forceinline__ double bavg(const A &a, int j, double up, double lo) 
     return 0.5 * (up + lo);
 }
 
+// One interior cell: FLUX_DIFF + sources + l_operator + RK combine +
+// FROMUTORVE + CUSTOM_CHECKRHO + FROMRVETOU + FINDPRESSURE; returns the
+// cell's CFL time step in stage 3.
+template <int STAGE, class W>
+__device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, int l, double dt) {
+    constexpr bool split = sizeof(W) == 4;   // FP32 split: U holds the conserved state
+    const long c = cidx(a, j, k, l), sy = a.NX, sz = a.NX * a.NY;
+    // stage-start primitives: FROMRVETOU input (fromrvetou.f90:428-438) in the
+    // FP64 mode, source terms only in the FP32 split
+    double rho = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
+    if (!split || a.coord == 1 || a.model == 1 || a.rt_gravity) {
+        rho = a.P[0][c]; vx = a.P[1][c]; vy = a.P[2][c]; vz = a.P[3][c];
+        bx = a.Bc[0][c]; by = a.Bc[1][c]; bz = a.Bc[2][c];
+    }
+    double u[5], uo[5];
+    if constexpr (split) {
+        for (int i = 0; i < 5; ++i) u[i] = a.U[i][c];
+    } else {
+        double eps = a.eps[c];
+        double vsq = vx * vx + vy * vy + vz * vz, bsq = bx * bx + by * by + bz * bz;
+        u[0] = rho;
+        u[1] = vx * rho;
+        u[2] = vy * rho;
+        u[3] = vz * rho;
+        u[4] = rho * (eps + 0.5 * vsq) + 0.5 * bsq;
+    }
+    if (STAGE == 1) {
+        for (int i = 0; i < 5; ++i) { uo[i] = u[i]; a.Uo[i][c] = u[i]; }
+    } else {
+        for (int i = 0; i < 5; ++i) uo[i] = a.Uo[i][c];
+    }
+    // sources: GET_SOURCE (spatial.f90:195-226) + CUSTOM_SOURCE
+    double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (a.coord == 1) {
+        double bsq = bx * bx + by * by + bz * bz, xj = a.ax[0].c[j + 3];
+        sc[1] = sc[1] + (a.P[4][c] + rho * vy * vy + 0.5 * bsq - by * by) / (xj);
+        sc[2] = sc[2] - (rho * vx * vy - bx * by) / (xj);
+    }
+    if (a.model == 1) {
+        double diff = rho - a.rfl[c];
+        double f = fmax2(fsign1(diff), 0.0);
+        double xj = a.ax[0].c[j + 3], zl = a.ax[2].c[l + 3];
+        double radius = sqrtT(xj * xj + zl * zl);
+        double dphidr = xj / radius / ((radius - a.r_sh) * (radius - a.r_sh));
+        double dphidz = zl / radius / ((radius - a.r_sh) * (radius - a.r_sh));
+        sc[1] = sc[1] + (-f * rho * dphidr);
+        sc[3] = sc[3] + (-f * rho * dphidz);
+        sc[4] = sc[4] + (-f * rho * (vx * dphidr + vz * dphidz));
+    } else if (a.rt_gravity) {
+        sc[2] = sc[2] - 0.1 * rho;
+    }
+    // FLUX_DIFF (spatial.f90:436-583) + l_operator hydro part; fluxes are read
+    // in the working type and differenced in double
+    double L[5];
+    {
+        double dx = a.ax[0].d[j + 3];
+        for (int i = 0; i < 5; ++i) {
+            const double f1 = a.F[0][i][c], f0 = a.F[0][i][c - 1];
+            double df;
+            if (a.coord == 1)
+                df = (a.ax[0].xF[j + 3] * f1 - a.ax[0].xF[j + 2] * f0) / (a.ax[0].c[j + 3] * dx);
+            else df = (f1 - f0) / (dx);
+            L[i] = -df;
+        }
+        if (a.ndim > 1) {
+            double dy = a.ax[1].d[k + 3];
+            for (int i = 0; i < 5; ++i) {
+                const double f1 = a.F[1][i][c], f0 = a.F[1][i][c - sy];
+                double df = (a.coord == 1) ? (f1 - f0) / (a.ax[0].c[j + 3] * dy) : (f1 - f0) / (dy);
+                L[i] = L[i] - df;
+            }
+        }
+        if (a.ndim > 2) {
+            double dz = a.ax[2].d[l + 3];
+            for (int i = 0; i < 5; ++i) {
+                const double f1 = a.F[2][i][c], f0 = a.F[2][i][c - sz];
+                L[i] = L[i] - (f1 - f0) / dz;
+            }
+        }
+        for (int i = 0; i < 5; ++i) L[i] = L[i] + sc[i];
+    }
+    double un[5];
+    for (int i = 0; i < 5; ++i) un[i] = rk<STAGE>(uo[i], u[i], dt, L[i]);
+    // the edge EMFs around this cell's six faces, each loaded once
+    const double ex_c = a.E[0][c], ex_my = a.E[0][c - sy], ex_mz = a.E[0][c - sz], ex_mymz = a.E[0][c - sy - sz];
+    const double ey_c = a.E[1][c], ey_mx = a.E[1][c - 1], ey_mz = a.E[1][c - sz], ey_mxmz = a.E[1][c - 1 - sz];
+    const double ez_c = a.E[2][c], ez_my = a.E[2][c - sy], ez_mx = a.E[2][c - 1], ez_mxmy = a.E[2][c - 1 - sy];
+    // CT update of the six faces of this cell (faces j-1/j, k-1/k, l-1/l)
+    double bn_up[3], bn_lo[3], bprim_lo[3];
+    {
+        const long cl[3] = {c - 1, c - sy, c - sz};
+        const int fidx[3] = {j, k, l};
+        const double Lup[3] = {lB_x(a, j, k, l, ez_c, ez_my, ey_c, ey_mz), lB_y(a, j, l, ex_c, ex_mz, ez_c, ez_mx),
+                               lB_z(a, j, k, ey_c, ey_mx, ex_c, ex_my)};
+        const double Llo[2] = {lB_x(a, j - 1, k, l, ez_mx, ez_mxmy, ey_mx, ey_mxmz),
+                               lB_y(a, j, l, ex_my, ex_mymz, ez_my, ez_mxmy)};
+        for (int d = 0; d < 3; ++d) {
+            double bup = a.Bin[d][c];
+            double uoup;
+            if (STAGE == 1) {
+                uoup = bup;
+                a.Uo[5 + d][c] = bup;
+            } else {
+                uoup = a.Uo[5 + d][c];
+            }
+            bn_up[d] = rk<STAGE>(uoup, bup, dt, Lup[d]);
+            double blo = a.Bin[d][cl[d]], uolo;
+            if (STAGE == 1) {
+                uolo = blo;
+                if (fidx[d] == 1) a.Uo[5 + d][cl[d]] = blo;
+            } else {
+                uolo = a.Uo[5 + d][cl[d]];
+            }
+            const double llo = d < 2 ? Llo[d] : lB_z(a, j, k, ey_mz, ey_mxmz, ex_mz, ex_mymz);
+            bn_lo[d] = rk<STAGE>(uolo, blo, dt, llo);
+            // the lowest face (index 0) keeps its stored value in prim
+            // (FROMUTORVE copies faces 1..n only, fromrvetou.f90:201-207)
+            bprim_lo[d] = (fidx[d] == 1) ? blo : bn_lo[d];
+        }
+    }
+    // FROMUTORVE (fromrvetou.f90:153-211)
+    double bce[3] = {bavg<0>(a, j, bn_up[0], bn_lo[0]), bavg<1>(a, j, bn_up[1], bn_lo[1]),
+                     bavg<2>(a, j, bn_up[2], bn_lo[2])};
+    double rn = un[0];
+    double v[3] = {un[1] / rn, un[2] / rn, un[3] / rn};
+    double bsq = bce[0] * bce[0] + bce[1] * bce[1] + bce[2] * bce[2];
+    double vsq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    double en = (un[4] - 0.5 * bsq) / rn - 0.5 * vsq;
+    checkrho(a, c, rn, v, en);
+    if (STAGE == 3) checkrho(a, c, rn, v, en);
+    // FROMRVETOU bcell (fromrvetou.f90:380-422)
+    double bcf[3] = {bavg<0>(a, j, bn_up[0], bprim_lo[0]), bavg<1>(a, j, bn_up[1], bprim_lo[1]),
+                     bavg<2>(a, j, bn_up[2], bprim_lo[2])};
+    double p, cs;
+    pressure(a, c, rn, en, p, cs);
+    a.P[0][c] = (W)rn;
+    a.P[1][c] = (W)v[0];
+    a.P[2][c] = (W)v[1];
+    a.P[3][c] = (W)v[2];
+    a.P[4][c] = (W)p;
+    a.eps[c] = (W)en;
+    a.cs[c] = (W)cs;
+    for (int d = 0; d < 3; ++d) {
+        a.Bc[d][c] = (W)bcf[d];
+        a.Bout[d][c] = bn_up[d];
+    }
+    // cell EMFs -v x B of the new state (flux_ct, mhd_module.f90:316-318),
+    // from the values just stored, for the next stage's edge EMFs
+    cell_emf(a, c, (W)v[0], (W)v[1], (W)v[2], (W)bcf[0], (W)bcf[1], (W)bcf[2]);
+    if constexpr (split) {
+        // FROMRVETOU of the double-precision primitives: the next stage's U
+        double vsq2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        double bsq2 = bcf[0] * bcf[0] + bcf[1] * bcf[1] + bcf[2] * bcf[2];
+        a.U[0][c] = rn;
+        a.U[1][c] = v[0] * rn;
+        a.U[2][c] = v[1] * rn;
+        a.U[3][c] = v[2] * rn;
+        a.U[4][c] = rn * (en + 0.5 * vsq2) + 0.5 * bsq2;
+    }
+    return STAGE == 3 ? cfl_dt(a, j, k, l, cs, rn, v, bcf) : 1.0e5;
+}
+
+// One thread per interior cell, x fastest.  (Measured on MI355X at 256^3:
+// this beats a z march carrying the plane below in registers, whose extra
+// live values cut occupancy, 64 x 4 cell tiles, and an XCD-grouped block
+// order; rows in plain round-robin block order stream best.)
 template <int STAGE, class W>
 __global__ __launch_bounds__(256) void k_update(ArgsT<W> a) {
-    constexpr bool split = sizeof(W) == 4;   // FP32 split: U holds the conserved state
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nn = (long)a.nx * a.ny * a.nz;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < nn;
+    const int j = 1 + (int)(t % a.nx), k = 1 + (int)((t / a.nx) % a.ny), l = 1 + (int)(t / ((long)a.nx * a.ny));
     if (!a.S->active) {
         // a step past total_time: keep the state, but the host swaps the face
         // buffers after every stage, so carry the interior faces across
-        if (t < nn) {
-            const int j = 1 + (int)(t % a.nx), k = 1 + (int)((t / a.nx) % a.ny), l = 1 + (int)(t / ((long)a.nx * a.ny));
+        if (live) {
             const long c = cidx(a, j, k, l);
             for (int d = 0; d < 3; ++d) a.Bout[d][c] = a.Bin[d][c];
         }
         return;
     }
     double dtloc = 1.0e5;
-    if (t < nn) {
-        const int j = 1 + (int)(t % a.nx), k = 1 + (int)((t / a.nx) % a.ny), l = 1 + (int)(t / ((long)a.nx * a.ny));
-        const long c = cidx(a, j, k, l), sy = a.NX, sz = a.NX * a.NY;
-        const double dt = a.S->dt;
-        // stage-start primitives and conserved variables (FROMRVETOU, fromrvetou.f90:428-438)
-        // stage-start primitives: FROMRVETOU input in the FP64 mode, source
-        // terms only in the FP32 split
-        double rho = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
-        if (!split || a.coord == 1 || a.model == 1 || a.rt_gravity) {
-            rho = a.P[0][c]; vx = a.P[1][c]; vy = a.P[2][c]; vz = a.P[3][c];
-            bx = a.Bc[0][c]; by = a.Bc[1][c]; bz = a.Bc[2][c];
-        }
-        double u[5], uo[5];
-        if constexpr (split) {
-            for (int i = 0; i < 5; ++i) u[i] = a.U[i][c];
-        } else {
-            double eps = a.eps[c];
-            double vsq = vx * vx + vy * vy + vz * vz, bsq = bx * bx + by * by + bz * bz;
-            u[0] = rho;
-            u[1] = vx * rho;
-            u[2] = vy * rho;
-            u[3] = vz * rho;
-            u[4] = rho * (eps + 0.5 * vsq) + 0.5 * bsq;
-        }
-        if (STAGE == 1) {
-            for (int i = 0; i < 5; ++i) { uo[i] = u[i]; a.Uo[i][c] = u[i]; }
-        } else {
-            for (int i = 0; i < 5; ++i) uo[i] = a.Uo[i][c];
-        }
-        // sources: GET_SOURCE (spatial.f90:195-226) + CUSTOM_SOURCE
-        double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (a.coord == 1) {
-            double bsq = bx * bx + by * by + bz * bz, xj = a.ax[0].c[j + 3];
-            sc[1] = sc[1] + (a.P[4][c] + rho * vy * vy + 0.5 * bsq - by * by) / (xj);
-            sc[2] = sc[2] - (rho * vx * vy - bx * by) / (xj);
-        }
-        if (a.model == 1) {
-            double diff = rho - a.rfl[c];
-            double f = fmax2(fsign1(diff), 0.0);
-            double xj = a.ax[0].c[j + 3], zl = a.ax[2].c[l + 3];
-            double radius = sqrtT(xj * xj + zl * zl);
-            double dphidr = xj / radius / ((radius - a.r_sh) * (radius - a.r_sh));
-            double dphidz = zl / radius / ((radius - a.r_sh) * (radius - a.r_sh));
-            sc[1] = sc[1] + (-f * rho * dphidr);
-            sc[3] = sc[3] + (-f * rho * dphidz);
-            sc[4] = sc[4] + (-f * rho * (vx * dphidr + vz * dphidz));
-        } else if (a.rt_gravity) {
-            sc[2] = sc[2] - 0.1 * rho;
-        }
-        // FLUX_DIFF (spatial.f90:436-583) + l_operator hydro part
-        double L[5];
-        {
-            double dx = a.ax[0].d[j + 3];
-            // fluxes are read in the working type and differenced in double
-            for (int i = 0; i < 5; ++i) {
-                const double f1 = a.F[0][i][c], f0 = a.F[0][i][c - 1];
-                double df;
-                if (a.coord == 1)
-                    df = (a.ax[0].xF[j + 3] * f1 - a.ax[0].xF[j + 2] * f0) / (a.ax[0].c[j + 3] * dx);
-                else df = (f1 - f0) / (dx);
-                L[i] = -df;
-            }
-            if (a.ndim > 1) {
-                double dy = a.ax[1].d[k + 3];
-                for (int i = 0; i < 5; ++i) {
-                    const double f1 = a.F[1][i][c], f0 = a.F[1][i][c - sy];
-                    double df = (a.coord == 1) ? (f1 - f0) / (a.ax[0].c[j + 3] * dy) : (f1 - f0) / (dy);
-                    L[i] = L[i] - df;
-                }
-            }
-            if (a.ndim > 2) {
-                double dz = a.ax[2].d[l + 3];
-                for (int i = 0; i < 5; ++i) {
-                    const double f1 = a.F[2][i][c], f0 = a.F[2][i][c - sz];
-                    L[i] = L[i] - (f1 - f0) / dz;
-                }
-            }
-            for (int i = 0; i < 5; ++i) L[i] = L[i] + sc[i];
-        }
-        double un[5];
-        for (int i = 0; i < 5; ++i) un[i] = rk<STAGE>(uo[i], u[i], dt, L[i]);
-        // CT update of the six faces of this cell (faces j-1/j, k-1/k, l-1/l)
-        double bn_up[3], bn_lo[3], bprim_lo[3];
-        {
-            long cl[3] = {c - 1, c - sy, c - sz};
-            int fidx[3] = {j, k, l};
-            double Lup[3] = {lB_x(a, j, k, l), lB_y(a, j, k, l), lB_z(a, j, k, l)};
-            double Llo[3] = {lB_x(a, j - 1, k, l), lB_y(a, j, k - 1, l), lB_z(a, j, k, l - 1)};
-            for (int d = 0; d < 3; ++d) {
-                double bup = a.Bin[d][c], blo = a.Bin[d][cl[d]];
-                double uoup, uolo;
-                if (STAGE == 1) {
-                    uoup = bup;
-                    uolo = blo;
-                    a.Uo[5 + d][c] = bup;
-                    if (fidx[d] == 1) a.Uo[5 + d][cl[d]] = blo;
-                } else {
-                    uoup = a.Uo[5 + d][c];
-                    uolo = a.Uo[5 + d][cl[d]];
-                }
-                bn_up[d] = rk<STAGE>(uoup, bup, dt, Lup[d]);
-                bn_lo[d] = rk<STAGE>(uolo, blo, dt, Llo[d]);
-                // the lowest face (index 0) keeps its stored value in prim
-                // (FROMUTORVE copies faces 1..n only, fromrvetou.f90:201-207)
-                bprim_lo[d] = (fidx[d] == 1) ? blo : bn_lo[d];
-            }
-        }
-        // FROMUTORVE (fromrvetou.f90:153-211)
-        double bce[3] = {bavg<0>(a, j, bn_up[0], bn_lo[0]), bavg<1>(a, j, bn_up[1], bn_lo[1]),
-                         bavg<2>(a, j, bn_up[2], bn_lo[2])};
-        double rn = un[0];
-        double v[3] = {un[1] / rn, un[2] / rn, un[3] / rn};
-        double bsq = bce[0] * bce[0] + bce[1] * bce[1] + bce[2] * bce[2];
-        double vsq = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-        double en = (un[4] - 0.5 * bsq) / rn - 0.5 * vsq;
-        checkrho(a, c, rn, v, en);
-        if (STAGE == 3) checkrho(a, c, rn, v, en);
-        // FROMRVETOU bcell (fromrvetou.f90:380-422)
-        double bcf[3] = {bavg<0>(a, j, bn_up[0], bprim_lo[0]), bavg<1>(a, j, bn_up[1], bprim_lo[1]),
-                         bavg<2>(a, j, bn_up[2], bprim_lo[2])};
-        double p, cs;
-        pressure(a, c, rn, en, p, cs);
-        a.P[0][c] = (W)rn;
-        a.P[1][c] = (W)v[0];
-        a.P[2][c] = (W)v[1];
-        a.P[3][c] = (W)v[2];
-        a.P[4][c] = (W)p;
-        a.eps[c] = (W)en;
-        a.cs[c] = (W)cs;
-        for (int d = 0; d < 3; ++d) {
-            a.Bc[d][c] = (W)bcf[d];
-            a.Bout[d][c] = bn_up[d];
-        }
-        if constexpr (split) {
-            // FROMRVETOU of the double-precision primitives: the next stage's U
-            double vsq2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-            double bsq2 = bcf[0] * bcf[0] + bcf[1] * bcf[1] + bcf[2] * bcf[2];
-            a.U[0][c] = rn;
-            a.U[1][c] = v[0] * rn;
-            a.U[2][c] = v[1] * rn;
-            a.U[3][c] = v[2] * rn;
-            a.U[4][c] = rn * (en + 0.5 * vsq2) + 0.5 * bsq2;
-        }
-        if (STAGE == 3) dtloc = cfl_dt(a, j, k, l, cs, rn, v, bcf);
-    }
+    if (live) dtloc = update_cell<STAGE, W>(a, j, k, l, a.S->dt);
     if constexpr (STAGE == 3) {
         dtloc = block_min_f64(dtloc, a.S);
         if (threadIdx.x == 0) atomicMin(&a.S->dtmin_bits, (unsigned long long)__double_as_longlong(dtloc));
@@ -755,10 +867,44 @@ __global__ __launch_bounds__(256) void k_update(ArgsT<W> a) {
 // ghost fill: BOUNDARYP_NM + BOUNDARY1D_NM(epsilon) + custom BCs, then
 // FINDPRESSURE on the ghost cells
 // ---------------------------------------------------------------------------
+// Number of ghost cells: the box minus the interior.
+template <class A> __host__ __device__ __forceinline__ long ghost_count(const A &a) {
+    return a.G - (long)a.nx * a.ny * a.nz;
+}
+
+// q-th ghost cell: the six z planes first, then the six y rows of each
+// interior plane, then the six x cells of each interior row.
+template <class A> __device__ __forceinline__ void ghost_cell(const A &a, long q, int &j, int &k, int &l) {
+    const long NX = a.NX, NY = a.NY, pz = 6 * NX * NY, py = 6 * NX * (long)a.nz;
+    if (q < pz) {
+        const int p = (int)(q / (NX * NY));
+        const long r = q % (NX * NY);
+        l = p < 3 ? p - 2 : a.nz + 1 + (p - 3);
+        k = (int)(r / NX) - 2;
+        j = (int)(r % NX) - 2;
+    } else if (q < pz + py) {
+        q -= pz;
+        l = 1 + (int)(q / (6 * NX));
+        const long r = q % (6 * NX);
+        const int p = (int)(r / NX);
+        k = p < 3 ? p - 2 : a.ny + 1 + (p - 3);
+        j = (int)(r % NX) - 2;
+    } else {
+        q -= pz + py;
+        l = 1 + (int)(q / (6L * a.ny));
+        const long r = q % (6L * a.ny);
+        k = 1 + (int)(r / 6);
+        const int p = (int)(r % 6);
+        j = p < 3 ? p - 2 : a.nx + 1 + (p - 3);
+    }
+}
+
 template <class W> __global__ __launch_bounds__(256) void k_ghost(ArgsT<W> a) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.G) return;
-    const int j = (int)(t % a.NX) - 2, k = (int)((t / a.NX) % a.NY) - 2, l = (int)(t / (a.NX * a.NY)) - 2;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ghost_count(a)) return;
+    int j, k, l;
+    ghost_cell(a, q, j, k, l);
+    const long t = cidx(a, j, k, l);
     int sx, sy, sz;
     const int jj = gmap(j, a.nx, a.bc[0], a.bc[1], sx);
     const int kk = gmap(k, a.ny, a.bc[2], a.bc[3], sy);
@@ -778,6 +924,13 @@ template <class W> __global__ __launch_bounds__(256) void k_ghost(ArgsT<W> a) {
     double v[3];
     for (int d = 0; d < 3; ++d)
         v[d] = a.P[1 + d][cs_] * (gsign(a, 0, sx, 2 + d) * gsign(a, 1, sy, 2 + d) * gsign(a, 2, sz, 2 + d));
+    // cell EMFs of the ghost: the mapped interior value times the parity of
+    // each component (BOUNDARY_EFIELD, mhd_module.f90:478-840), computed from
+    // the source cell before the no-inflow clamps below touch v
+    cell_emf(a, c, a.P[1][cs_], a.P[2][cs_], a.P[3][cs_], a.Bc[0][cs_], a.Bc[1][cs_], a.Bc[2][cs_],
+             (W)(gsign(a, 0, sx, 9) * gsign(a, 1, sy, 9) * gsign(a, 2, sz, 9)),
+             (W)(gsign(a, 0, sx, 10) * gsign(a, 1, sy, 10) * gsign(a, 2, sz, 10)),
+             (W)(gsign(a, 0, sx, 11) * gsign(a, 1, sy, 11) * gsign(a, 2, sz, 11)));
     for (int d = 0; d < 3; ++d)
         a.Bc[d][c] = (W)(a.Bc[d][cs_] * (gsign(a, 0, sx, 6 + d) * gsign(a, 1, sy, 6 + d) * gsign(a, 2, sz, 6 + d)));
     // epsilon: BOUNDARY1D_NM takes the y-outer branch from boundary_flag(3) (boundary.f90:243)

@@ -213,8 +213,11 @@ __device__ __forceinline__ int sel_argmax(sel_f4 d) {
 // Class mask of every cell the sweep along D reconstructs.  Each wave takes
 // 64 consecutive cells (one per lane, coalesced stencil loads), stages their
 // BF16 features in its own LDS tile, and runs four 16-cell MFMA groups from
-// it.  Waves step through the cells grid-stride and never wait on each other,
-// so only wavefront-scope ordering is needed around the tile.
+// it.  Waves never wait on each other, so only wavefront-scope ordering is
+// needed around the tile.  Blocks are dealt to the 8 XCDs round-robin; with a
+// block count that is a multiple of 8, the waves of one XCD walk one
+// contiguous eighth of the cells, so the rows above and below (which the y
+// and z stencils read, and neighbouring chunks share) stay in that XCD's L2.
 template <class W, int D>
 __global__ __launch_bounds__(256) void k_sel_infer(ArgsT<W> a, const unsigned short *frag, const float *w,
                                                    unsigned char *mask) {
@@ -224,10 +227,20 @@ __global__ __launch_bounds__(256) void k_sel_infer(ArgsT<W> a, const unsigned sh
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     const long nq = sel_count<D>(a);
     const long nchunk = (nq + 63) / 64;
-    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
+    const int wpb = blockDim.x >> 6;
+    long first = 0, last = nchunk, wave, nwaves;
+    if (gridDim.x % 8 == 0) {
+        const int xcd = blockIdx.x % 8;
+        first = nchunk * xcd / 8;
+        last = nchunk * (xcd + 1) / 8;
+        wave = (long)(blockIdx.x / 8) * wpb + wv;
+        nwaves = (long)(gridDim.x / 8) * wpb;
+    } else {
+        wave = (long)blockIdx.x * wpb + wv;
+        nwaves = (long)gridDim.x * wpb;
+    }
     const long st = D == 0 ? 1L : (D == 1 ? a.NX : a.NX * a.NY);
-    for (long ch = wave; ch < nchunk; ch += nwaves) {
+    for (long ch = first + wave; ch < last; ch += nwaves) {
         const long q = ch * 64 + lane;
         const bool ok = q < nq;
         int i = 0;
