@@ -61,9 +61,11 @@ def kernel_bytes(cfg, name: str, w: int) -> float:
         faces = {"flux_x": (nx + 1) * ny * nz, "flux_y": nx * (ny + 1) * nz, "flux_z": nx * ny * (nz + 1)}[name]
         return 7 * w * cells + (8 + 7 * w) * faces
     if name == "edge":
-        # EMF prep: v, bcell -> 3 cell EMFs on (n+2)^3; edge: 6 face-EMF
-        # arrays + 3 cell-EMF arrays -> 3 edge EMFs on (n+1)^3
-        return 9 * w * (nx + 2) * (ny + 2) * (nz + 2) + 12 * w * (nx + 1) * (ny + 1) * (nz + 1)
+        # edge: 6 face-EMF arrays + 3 cell-EMF arrays -> 3 edge EMFs on
+        # (n+1)^3, plus the face-EMF fill of the (n+2)^3 box's shell (the
+        # mapped interior face EMF read, the ghost one written: 4 per cell)
+        shell = (nx + 2) * (ny + 2) * (nz + 2) - n
+        return 12 * w * (nx + 1) * (ny + 1) * (nz + 1) + 4 * w * shell
     if name == "update":
         # stage 2/3 of the fused update
         if split:
@@ -71,12 +73,38 @@ def kernel_bytes(cfg, name: str, w: int) -> float:
             # and the primitives written back in float (the stage-start
             # primitives are read only for the cylindrical/torus/RT sources)
             prim_in = 8 if (cfg.coordinate_flag == 1 or cfg.model == "torus" or cfg.rt_gravity) else 0
-            return n * (8 * (5 + 8 + 3 + 3 + 5) + w * (prim_in + 15 + 3 + 5 + 1 + 1 + 3))
-        return n * 8 * (5 + 3 + 1 + 8 + 3 + 15 + 3 + 5 + 1 + 1 + 3 + 3)
+            return n * (8 * (5 + 8 + 3 + 3 + 5) + w * (prim_in + 15 + 3 + 5 + 1 + 1 + 3 + 3))
+        # reads: prim 5, bcell 3, eps, u_old 8, face B 3, fluxes 15, edge EMFs 3;
+        # writes: prim 5, eps, cs, bcell 3, face B 3, cell EMFs 3
+        return n * 8 * (5 + 3 + 1 + 8 + 3 + 15 + 3 + 5 + 1 + 1 + 3 + 3 + 3)
     if name == "ghost":
         g = (nx + 6) * (ny + 6) * (nz + 6) - n
         return g * (2 * w * 10 + 2 * 8 * 2)
     raise KeyError(name)
+
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_ot256_hlld.md")
+
+
+def measured_traffic(kernel: str, w: int, n: int):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated against a copy of known size),
+    averaged over the class's launches (the three RK stages of the update);
+    None when the summary does not cover this workload."""
+    if n != 256 or not os.path.isfile(TRAFFIC_SUMMARY):
+        return None
+    prefix = {"flux_x": "k_flux_x<", "flux_y": "k_flux_yz<", "flux_z": "k_flux_yz<", "edge": "k_edge<",
+              "update": "k_update<", "ghost": "k_ghost<"}[kernel]
+    typ = "float" if w == 4 else "double"
+    vals = []
+    for line in open(TRAFFIC_SUMMARY):
+        cols = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cols) < 6 or not cols[0].startswith(prefix) or typ not in cols[0]:
+            continue
+        if kernel in ("flux_y", "flux_z") and not cols[0].rstrip(">").endswith("1" if kernel == "flux_y" else "2"):
+            continue
+        vals.append(float(cols[5]) * 1e6)
+    return sum(vals) / len(vals) if vals else None
 
 
 def make_workload(name: str, n: int):
@@ -311,8 +339,12 @@ def main(argv=None):
     value = rate(main_var)
     dom = max(KERNELS, key=lambda k: kt[k])
     achieved = kernel_bytes(cfg, dom, 8 if main_var == "fp64" else 4) / (kt[dom] * 1e-3) / 1e9
+    traffic = measured_traffic(dom, 8 if main_var == "fp64" else 4, args.n) if args.workload == "ot" else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                "algorithmic_bytes": kernel_bytes(cfg, dom, 8 if main_var == "fp64" else 4),
+                "traffic_source": "profiles/traffic_ot256_hlld.md (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
+                if traffic else None,
                 "kernel_ms": kt[dom],
                 "kernel_ms_all": {v: {k: res[v][1][k] for k in KERNELS + ["selector"]} for v in res},
                 "step_bytes_frac": 512.0 * value / (HBM_PEAK_GBS * 1e9)}

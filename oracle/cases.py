@@ -63,6 +63,16 @@ CASES = {c.name: c for c in [
                                   riemann="hlld", recon="ppm", model="torus",
                                   boundary_flag=(1, 1, 0, 0, 1, 1)),
          "torus", 10),
+    # 100 steps (the length the FP64 parity criterion is stated for): the
+    # 128^3 Orszag-Tang configuration's physics (HLLD + PPM) at 24^3, and the
+    # cylindrical torus
+    Case("ot24_ppm_100", _ot(24, "ppm", "hlld"), "orszag_tang", 100),
+    Case("torus_ppm_100", Config(nx=24, ny=8, nz=24, n_dim=3, coordinate_flag=1,
+                                 x_start=2.0, x_end=26.0, y_start=0.0, y_end=2.0 * math.pi,
+                                 z_start=-12.0, z_end=12.0, cfl=0.4, ggas=5.0 / 3.0,
+                                 riemann="hlld", recon="ppm", model="torus",
+                                 boundary_flag=(1, 1, 0, 0, 1, 1)),
+         "torus", 100),
     # cylindrical axis (3) and equatorial (4) symmetry flags
     Case("torus_sym_tvdmc", Config(nx=16, ny=8, nz=12, n_dim=3, coordinate_flag=1,
                                    x_start=2.0, x_end=26.0, y_start=0.0, y_end=2.0 * math.pi,
