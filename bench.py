@@ -208,16 +208,32 @@ def l1_errors(cfg, st, steps: int, sel_w=None):
             b.finddt()
             b.rungekutta(dt)
     w = (slice(3, -3),) * 3
-    x = a.state()["rho"][w]
-    out = {}
+    sa = a.state()
+    x = sa["rho"][w]
+    out = {"fp64": {"divb_norm": divb_norm(cfg, sa)}}
     for k, b in others.items():
-        y = b.state()["rho"][w]
+        sb = b.state()
+        y = sb["rho"][w]
         d = np.abs(x - y)
         out[k] = {"rho_l1": float(d.mean()), "rho_l1_rel": float(d.mean() / np.abs(x).mean()),
-                  "rho_linf_rel": float(d.max() / np.abs(x).max())}
+                  "rho_linf_rel": float(d.max() / np.abs(x).max()), "divb_norm": divb_norm(cfg, sb)}
     out["steps"] = steps
     out["t"] = a.t
     return out
+
+
+def divb_norm(cfg, st):
+    """max |div B| * h / <|B|> over cells 2..n of a uniform Cartesian grid (the
+    lowest face of each axis stays frozen in the reference, so cells 1 are left
+    out)."""
+    import numpy as np
+    bx, by, bz = st["bx"], st["by"], st["bz"]
+    h = (cfg.x_end - cfg.x_start) / cfg.nx
+    i = (slice(4, -3),) * 3
+    d = ((bx[i] - bx[4:-3, 4:-3, 3:-4]) + (by[i] - by[4:-3, 3:-4, 4:-3]) + (bz[i] - bz[3:-4, 4:-3, 4:-3])) / h
+    bm = np.mean(np.sqrt(st["bcx"][3:-3, 3:-3, 3:-3] ** 2 + st["bcy"][3:-3, 3:-3, 3:-3] ** 2
+                         + st["bcz"][3:-3, 3:-3, 3:-3] ** 2))
+    return float(np.max(np.abs(d)) * h / bm)
 
 
 def cpu_baseline(max_threads: int = 16):
