@@ -18,8 +18,8 @@ from typing import Tuple
 RIEMANN = {"lf": 0, "hll": 1, "hllc": 2, "hlld": 3}
 # Reconstructions (parameter.h:103-115 tvdmm/tvdmc/tvdvl/ppmc/weno flags)
 RECON = {"tvdmm": 0, "tvdvl": 1, "tvdmc": 2, "ppm": 3, "weno": 4}
-# In-step model hooks (model/riemann and model/torus-cyn)
-MODELS = {"riemann": 0, "torus": 1}
+# In-step model hooks (model/riemann, model/torus-cyn, model/torus-sph)
+MODELS = {"riemann": 0, "torus": 1, "torus_sph": 2}
 # Arithmetic of the spatial operator
 PRECISION = {"fp64": 0, "fp32": 1}
 
@@ -30,7 +30,7 @@ class Config:
     ny: int = 1
     nz: int = 1
     n_dim: int = 1
-    coordinate_flag: int = 0              # 0 Cartesian, 1 cylindrical (r, phi, z)
+    coordinate_flag: int = 0              # 0 Cartesian, 1 cylindrical (r, phi, z), 2 spherical (r, theta, phi)
     boundary_flag: Tuple[int, ...] = (1, 1, 1, 1, 1, 1)
     x_start: float = 0.0
     x_end: float = 1.0
@@ -50,6 +50,10 @@ class Config:
     # model/torus-cyn/src/param.h:19 Paczynski-Wiita radius (0 = Newtonian, GM=1)
     r_sh: float = 0.0
     precision: str = "fp64"
+    # -DFIXPOLE (make.inc:8-21): on spherical grids the polar faces sit at
+    # theta = 0 and pi with sin = 0 (getgrid.f90:91-98), and the radial edge
+    # EMFs on them are averaged over phi (mhd_module.f90:437-461)
+    fixpole: bool = True
 
     def __post_init__(self):
         self.boundary_flag = tuple(int(b) for b in self.boundary_flag)
@@ -63,8 +67,12 @@ class Config:
             raise ValueError(f"unknown model {self.model!r}")
         if self.precision not in PRECISION:
             raise ValueError(f"unknown precision {self.precision!r}")
-        if self.coordinate_flag not in (0, 1):
-            raise ValueError("only Cartesian (0) and cylindrical (1) geometry are built")
+        if self.coordinate_flag not in (0, 1, 2):
+            raise ValueError("coordinate_flag must be 0 (Cartesian), 1 (cylindrical) or 2 (spherical)")
+        if self.model == "torus" and self.coordinate_flag != 1:
+            raise ValueError("the torus-cyn model needs cylindrical coordinates")
+        if self.model == "torus_sph" and self.coordinate_flag != 2:
+            raise ValueError("the torus-sph model needs spherical coordinates")
         if self.n_dim == 1 and (self.ny != 1 or self.nz != 1):
             raise ValueError("n_dim=1 needs ny=nz=1")
         if self.n_dim == 2 and self.nz != 1:
@@ -121,3 +129,16 @@ def torus_cyl(nr: int = 192, nphi: int = 128, nz: int = 192, recon: str = "ppm",
                   z_start=-12.0, z_end=12.0, cfl=0.4, ggas=5.0 / 3.0,
                   riemann=riemann, recon=recon, model="torus", r_sh=0.0,
                   boundary_flag=(1, 1, 0, 0, 1, 1))
+
+
+def torus_sph(nr: int = 128, nth: int = 96, nphi: int = 64, recon: str = "ppm",
+              riemann: str = "hlld", n_dim: int = 3) -> Config:
+    """Spherical (r, theta, phi) accretion torus in the Paczynski-Wiita
+    potential with r_sh = 1 (model/torus-sph/src/param.h:19): the full polar
+    range with axis boundaries at both poles, periodic in phi.  With n_dim=2
+    the run is axisymmetric (one phi cell)."""
+    return Config(nx=nr, ny=nth, nz=nphi if n_dim == 3 else 1, n_dim=n_dim, coordinate_flag=2,
+                  x_start=2.0, x_end=14.0, y_start=0.0, y_end=math.pi,
+                  z_start=0.0, z_end=2.0 * math.pi, cfl=0.4, ggas=5.0 / 3.0,
+                  riemann=riemann, recon=recon, model="torus_sph", r_sh=1.0,
+                  boundary_flag=(1, 1, 3, 3, 0, 0))

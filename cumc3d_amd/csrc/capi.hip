@@ -208,11 +208,22 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
     if (d.n_dim < 1 || d.n_dim > 3) return fail(CUMC3D_EINVAL, "n_dim must be 1, 2 or 3");
     if (d.n_dim == 1 && (d.ny != 1 || d.nz != 1)) return fail(CUMC3D_EINVAL, "n_dim=1 needs ny=nz=1");
     if (d.n_dim == 2 && d.nz != 1) return fail(CUMC3D_EINVAL, "n_dim=2 needs nz=1");
-    if (d.coordinate_flag != 0 && d.coordinate_flag != 1)
-        return fail(CUMC3D_EINVAL, "coordinate_flag must be 0 (Cartesian) or 1 (cylindrical)");
+    if (d.coordinate_flag < 0 || d.coordinate_flag > 2)
+        return fail(CUMC3D_EINVAL, "coordinate_flag must be 0 (Cartesian), 1 (cylindrical) or 2 (spherical)");
     if (d.riemann < 0 || d.riemann > 3) return fail(CUMC3D_EINVAL, "unknown Riemann solver");
     if (d.recon < 0 || d.recon > 4) return fail(CUMC3D_EINVAL, "unknown reconstruction");
-    if (d.model < 0 || d.model > 1) return fail(CUMC3D_EINVAL, "unknown model");
+    if (d.model < 0 || d.model > 2) return fail(CUMC3D_EINVAL, "unknown model");
+    if (d.model == CUMC3D_MODEL_TORUS && d.coordinate_flag != 1)
+        return fail(CUMC3D_EINVAL, "the torus-cyn model needs cylindrical coordinates");
+    if (d.model == CUMC3D_MODEL_TORUS_SPH && d.coordinate_flag != 2)
+        return fail(CUMC3D_EINVAL, "the torus-sph model needs spherical coordinates");
+    if (d.metrics && (!d.metrics->dx || !d.metrics->dy || !d.metrics->dz || !d.metrics->x || !d.metrics->y ||
+                      !d.metrics->z))
+        return fail(CUMC3D_EINVAL, "grid metrics need dx, dy, dz, x, y and z");
+    if (d.metrics && d.coordinate_flag == 2 &&
+        (!d.metrics->dx_cb || !d.metrics->xbar || !d.metrics->dcose || !d.metrics->dsine || !d.metrics->sine ||
+         !d.metrics->sinf))
+        return fail(CUMC3D_EINVAL, "spherical grid metrics need dx_cb, xbar, dcose, dsine, sine and sinf");
     if (d.hlld_bn_mode < 0 || d.hlld_bn_mode > 2) return fail(CUMC3D_EINVAL, "unknown hlld_bn_mode");
     if (d.precision != CUMC3D_FP64 && d.precision != CUMC3D_FP32) return fail(CUMC3D_EINVAL, "unknown precision");
     if (!d.xF || !d.yF || !d.zF) return fail(CUMC3D_EINVAL, "face coordinate arrays are required");
@@ -230,6 +241,7 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         return fail(CUMC3D_EINVAL, "grid too large: each field must stay below 4 GiB (about 800^3 cells)");
     cumc3d_ctx *x = new cumc3d_ctx();
     x->d = d;
+    x->d.metrics = nullptr;   // caller-owned: read during init only
     x->nx = d.nx; x->ny = d.ny; x->nz = d.nz;
     x->NX = d.nx + 6; x->NY = d.ny + 6; x->NZ = d.nz + 6;
     x->G = x->NX * x->NY * x->NZ;
@@ -237,17 +249,24 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         delete x;
         return fail(CUMC3D_EINVAL, "boundary flag combination rejected by SETUP_EQNS (setup.f90:120-322)");
     }
-    // grid metrics: GetGrid (getgrid.f90:43-63) from the face coordinates
+    // grid metrics: GetGrid (getgrid.f90:43-98) from the face coordinates,
+    // or the caller's GETGRID arrays
     const double *Fc[3] = {d.xF, d.yF, d.zF};
+    const cumc3d_grid_metrics *gm = d.metrics;
+    const double *Dm[3] = {gm ? gm->dx : nullptr, gm ? gm->dy : nullptr, gm ? gm->dz : nullptr};
+    const double *Cm[3] = {gm ? gm->x : nullptr, gm ? gm->y : nullptr, gm ? gm->z : nullptr};
     std::vector<double> hm;
     long off[3][7];
+    std::vector<double> widths[3], centres[3];
     for (int a = 0; a < 3; ++a) {
         int n = nd3[a];
         std::vector<double> f(Fc[a], Fc[a] + n + 7), dd(n + 7, 0.0), cc(n + 7, 0.0), w;
         for (int j = -2; j <= n + 3; ++j) {
-            dd[j + 3] = f[j + 3] - f[j + 2];
-            cc[j + 3] = (f[j + 3] + f[j + 2]) / 2;
+            dd[j + 3] = Dm[a] ? Dm[a][j + 2] : f[j + 3] - f[j + 2];
+            cc[j + 3] = Cm[a] ? Cm[a][j + 2] : (f[j + 3] + f[j + 2]) / 2;
         }
+        widths[a] = dd;
+        centres[a] = cc;
         weights(d.recon, dd, n, w);
         off[a][0] = (long)hm.size(); hm.insert(hm.end(), f.begin(), f.end());
         off[a][1] = (long)hm.size(); hm.insert(hm.end(), dd.begin(), dd.end());
@@ -258,6 +277,43 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
             weights(sch == 0 ? 2 : (sch == 1 ? 3 : 4), dd, n, w);
             off[a][4 + sch] = (long)hm.size(); hm.insert(hm.end(), w.begin(), w.end());
         }
+    }
+    // spherical metrics (getgrid.f90:68-98) and the per-index factors the
+    // kernels multiply by, each formed exactly as the reference's expression
+    // forms it (spatial.f90:227-256,467-581; fromrvetou.f90:181-188)
+    long soff = -1;
+    if (d.coordinate_flag == 2) {
+        const int nx = d.nx, ny = d.ny;
+        const std::vector<double> xF(d.xF, d.xF + nx + 7), yF(d.yF, d.yF + ny + 7);
+        const std::vector<double> &dx = widths[0], &xc = centres[0], &dy = widths[1], &yc = centres[1];
+        std::vector<double> t[10];
+        for (int i = 0; i < 5; ++i) t[i].assign(nx + 7, 0.0);
+        for (int i = 5; i < 10; ++i) t[i].assign(ny + 7, 0.0);
+        for (int j = -1; j <= nx + 3; ++j) {
+            double a = xF[j + 3], b = xF[j + 2];
+            double dcb = gm ? gm->dx_cb[j + 2] : a * a * a - b * b * b;
+            double xb = gm ? gm->xbar[j + 2] : (dcb / 3.0) / (xc[j + 3] * dx[j + 3]);
+            t[0][j + 3] = dcb;
+            t[1][j + 3] = dcb / 3.0;
+            t[2][j + 3] = xb;
+            t[3][j + 3] = 3.0 * xc[j + 3] * dx[j + 3] / dcb;
+            t[4][j + 3] = 1.5 * dx[j + 3] / dcb;
+        }
+        for (int k = -3; k <= ny + 3; ++k) t[5][k + 3] = gm ? gm->sinf[k + 3] : std::sin(yF[k + 3]);
+        if (!gm && d.fixpole) {
+            t[5][3] = 0.0;
+            t[5][ny + 3] = 0.0;
+        }
+        for (int k = -1; k <= ny + 3; ++k) {
+            double dc = gm ? gm->dcose[k + 2] : -(std::cos(yF[k + 3]) - std::cos(yF[k + 2]));
+            double ds = gm ? gm->dsine[k + 2] : std::sin(yF[k + 3]) - std::sin(yF[k + 2]);
+            t[6][k + 3] = dc;
+            t[7][k + 3] = ds / dc;
+            t[8][k + 3] = 0.5 * dy[k + 3] / dc;
+        }
+        for (int k = -2; k <= ny + 3; ++k) t[9][k + 3] = gm ? gm->sine[k + 2] : std::sin(yc[k + 3]);
+        soff = (long)hm.size();
+        for (int i = 0; i < 10; ++i) hm.insert(hm.end(), t[i].begin(), t[i].end());
     }
     int rc = 0;
     auto cleanup = [&]() { cumc3d_finalize(x); };
@@ -276,6 +332,12 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         x->ax[a].c = x->met + off[a][2];
         x->ax[a].w = x->met + off[a][3];
         for (int sch = 0; sch < 3; ++sch) x->ax[a].wsel[sch] = x->met + off[a][4 + sch];
+    }
+    if (soff >= 0) {
+        const double *b = x->met + soff;
+        const long nx7 = d.nx + 7, ny7 = d.ny + 7;
+        for (int i = 0; i < 5; ++i) x->sph[i] = b + i * nx7;
+        for (int i = 5; i < 10; ++i) x->sph[i] = b + 5 * nx7 + (i - 5) * ny7;
     }
     // per-cell fields: the working-type block, then the double block, each
     // field G entries, fields 256-byte aligned
@@ -322,13 +384,17 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         return fail(CUMC3D_ENOMEM, "blockmax");
     }
     hipMemset(x->blockmax, 0, x->nblockmax * sizeof(unsigned long long));
+    if (d.riemann == CUMC3D_HLL) {
+        if (hipMalloc(&x->stale, (size_t)3 * x->G) != hipSuccess) { cleanup(); return fail(CUMC3D_ENOMEM, "stale flags"); }
+        if (hipMemset(x->stale, 0, (size_t)3 * x->G) != hipSuccess) { cleanup(); return fail(CUMC3D_EHIP, "memset"); }
+    }
     *out = x;
     return rc;
 }
 
 int cumc3d_upload(cumc3d_ctx *x, const double *prim, const double *eps, const double *floors3) {
     if (!x || !prim || !eps) return fail(CUMC3D_EINVAL, "null argument");
-    if (x->d.model == CUMC3D_MODEL_TORUS && !floors3) return fail(CUMC3D_EINVAL, "torus model needs floor arrays");
+    if (x->d.model != CUMC3D_MODEL_RIEMANN && !floors3) return fail(CUMC3D_EINVAL, "torus models need floor arrays");
     double *tmp = nullptr;
     HIPCHK(hipMalloc(&tmp, (size_t)9 * x->G * sizeof(double)));
     HIPCHK(hipMemcpyAsync(tmp, prim, (size_t)8 * x->G * sizeof(double), hipMemcpyHostToDevice, x->stream));
@@ -532,6 +598,7 @@ int cumc3d_finalize(cumc3d_ctx *x) {
     if (x->met) hipFree(x->met);
     if (x->S) hipFree(x->S);
     if (x->blockmax) hipFree(x->blockmax);
+    if (x->stale) hipFree(x->stale);
     void *sel[] = {x->sel_w, x->sel_m, x->sel_v, x->sel_frag, x->mask, x->samp_feat, x->samp_label,
                    x->samp_counts, x->grad_part, x->loss_part, x->loss_dev};
     for (void *q : sel)

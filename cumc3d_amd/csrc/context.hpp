@@ -36,6 +36,8 @@ struct cumc3d_ctx {
     float *sel_w = nullptr, *sel_m = nullptr, *sel_v = nullptr;   // FP32 parameters + Adam moments
     unsigned short *sel_frag = nullptr;                            // BF16 MFMA fragments
     unsigned char *mask = nullptr;                                 // 3 x G class mask
+    unsigned char *stale = nullptr;                                // HLL: 3 x G unassigned-face flags
+    const double *sph[10] = {};   // spherical tables in met (see ArgsT), null on other geometries
     float *samp_feat = nullptr;
     unsigned char *samp_label = nullptr;
     int samp_cap = 0, *samp_counts = nullptr, adam_step = 0;
@@ -117,8 +119,8 @@ template <class W> __global__ void k_init_interior(ArgsT<W> a) {
     checkrho(a, c, rho, v, eps);
     a.P[0][c] = (W)rho; a.P[1][c] = (W)v[0]; a.P[2][c] = (W)v[1]; a.P[3][c] = (W)v[2];
     a.eps[c] = (W)eps;
-    double b[3] = {bavg<0>(a, j, a.Bin[0][c], a.Bin[0][c - 1]), bavg<1>(a, j, a.Bin[1][c], a.Bin[1][c - a.NX]),
-                   bavg<2>(a, j, a.Bin[2][c], a.Bin[2][c - a.NX * a.NY])};
+    double b[3] = {bavg<0>(a, j, k, a.Bin[0][c], a.Bin[0][c - 1]), bavg<1>(a, j, k, a.Bin[1][c], a.Bin[1][c - a.NX]),
+                   bavg<2>(a, j, k, a.Bin[2][c], a.Bin[2][c - a.NX * a.NY])};
     for (int d = 0; d < 3; ++d) a.Bc[d][c] = (W)b[d];
     cell_emf(a, c, (W)v[0], (W)v[1], (W)v[2], (W)b[0], (W)b[1], (W)b[2]);
     double p, cs;
@@ -161,9 +163,16 @@ template <class W> __global__ __launch_bounds__(256) void k_divb(ArgsT<W> a, uns
         double dx = a.ax[0].d[j + 3], dy = a.ax[1].d[k + 3], dz = a.ax[2].d[l + 3];
         if (a.coord == 0)
             d = (bx[c] - bx[c - 1]) / (dx) + (by[c] - by[c - sy]) / (dy) + (bz[c] - bz[c - sz]) / (dz);
-        else
+        else if (a.coord == 1)
             d = (a.ax[0].xF[j + 3] * bx[c] - a.ax[0].xF[j + 2] * bx[c - 1]) / (a.ax[0].c[j + 3] * dx) +
                 (by[c] - by[c - sy]) / (a.ax[0].c[j + 3] * dy) + (bz[c] - bz[c - sz]) / (dz);
+        else {   // mhd_module.f90:171-186
+            double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2], dcb = a.dcb[j + 3], dc = a.dcose[k + 3];
+            double xdx = a.ax[0].c[j + 3] * dx;
+            d = (xF * xF * bx[c] - xFm * xFm * bx[c - 1]) / (dcb / 3.0) +
+                (a.sinf[k + 3] * by[c] - a.sinf[k + 2] * by[c - sy]) * xdx / (dcb * dc / 3.0) +
+                (bz[c] - bz[c - sz]) * (xdx * dy) / (dcb * dc * dz / 3.0);
+        }
         d = fabsT(d);
     }
     // max of non-negative doubles == max of their bit patterns
@@ -210,6 +219,10 @@ template <class W> static ArgsT<W> make_args(cumc3d_ctx *x) {
     a.S = x->S;
     a.blockmax = x->blockmax;
     for (int d = 0; d < 3; ++d) a.mask[d] = x->mask ? x->mask + (size_t)d * x->G : nullptr;
+    for (int d = 0; d < 3; ++d) a.stale[d] = x->stale ? x->stale + (size_t)d * x->G : nullptr;
+    a.dcb = x->sph[0]; a.dcb3 = x->sph[1]; a.xbar = x->sph[2]; a.gsrc = x->sph[3]; a.hbx = x->sph[4];
+    a.sinf = x->sph[5]; a.dcose = x->sph[6]; a.cot = x->sph[7]; a.hby = x->sph[8]; a.sine = x->sph[9];
+    a.fixpole = x->d.coordinate_flag == 2 && x->d.fixpole;
     return a;
 }
 
@@ -308,6 +321,7 @@ template <class W> static void run_stage_t(cumc3d_ctx *x, int stage) {
     tmark(x, 3);
     hipLaunchKernelGGL(k_emf_prep<W>, dim3(nblk(emf_prep_count(a))), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_edge<W>, dim3(nblk((long)(x->nx + 1) * (x->ny + 1) * (x->nz + 1))), dim3(256), 0, st, a);
+    if (a.fixpole) hipLaunchKernelGGL(k_fixpole<W>, dim3(nblk(2L * x->nx)), dim3(256), 0, st, a);
     tmark(x, 4);
     if (stage == 1) hipLaunchKernelGGL((k_update<1, W>), dim3(nblk(nn)), dim3(256), 0, st, a);
     else if (stage == 2) hipLaunchKernelGGL((k_update<2, W>), dim3(nblk(nn)), dim3(256), 0, st, a);

@@ -47,7 +47,7 @@ constexpr int RSEL = 5;
 // primitives, interface fluxes and EMFs are single precision and the face
 // field, u_old and the conserved state U (the RK accumulators) stay double.
 template <class W> struct ArgsT {
-    int nx, ny, nz, ndim, coord, model, rt_gravity, bn_mode;
+    int nx, ny, nz, ndim, coord, model, rt_gravity, bn_mode, fixpole;
     int bc[6];
     int bfac[6][12];        // [2*axis+side][var 1..11], SETUP_EQNS parity factors
     long NX, NY, G;
@@ -67,6 +67,12 @@ template <class W> struct ArgsT {
     DevScalars *S;
     unsigned long long *blockmax;    // per-block max of double-star interface keys
     const unsigned char *mask[3];    // selector class per cell and sweep direction
+    unsigned char *stale[3];         // HLL only: 1 where a sweep assigned no flux (see store_flux_hll)
+    // spherical grids (getgrid.f90:68-98), index j+3 / k+3: dx_cb, dx_cb/3,
+    // xbar, 3 x dx/dx_cb, 1.5 dx/dx_cb along r; sinf, dcose, dsine/dcose,
+    // 0.5 dy/dcose, sine along theta.  Null on other geometries.
+    const double *dcb, *dcb3, *xbar, *gsrc, *hbx;
+    const double *sinf, *dcose, *cot, *hby, *sine;
 };
 
 // Load field[idx] as scalar base + 32-bit byte offset (the global_load saddr
@@ -232,6 +238,65 @@ __device__ __forceinline__ void store_flux(const ArgsT<T> &a, long c, const Riem
     else { a.F[2][5][c] = o.f[6]; a.F[2][6][c] = -o.f[5]; }
 }
 
+// HLLNM's IF/ELSEIF chain (rsolver_module.f90:303-311) has no ELSE: with a
+// NaN signal speed (a face state with negative pressure) no branch is taken
+// and the reference's single flux array, shared by the three sweeps, keeps
+// the last value any sweep wrote at that index.  The sweeps here have arrays
+// of their own, so such a face takes what the shared array would hold: the
+// flux of the previous sweep whose loop range contains the index -- x from z
+// of the previous stage (from y in 2D), y from x, z from y; the lowest face of
+// the sweep's own axis is in no other range and keeps its own value --
+// converted between the component layouts of store_flux.  A sweep stores
+// only the two transverse field fluxes; its normal-field flux is +0 where it
+// assigned the face (fluxL(ibn) = 0, spatial.f90:381-382) and inherited from
+// its own predecessor otherwise, which the stale flags record.  The oracle's
+// single flux array does the same by construction.
+template <class T, int D>
+__device__ __forceinline__ void store_flux_hll(const ArgsT<T> &a, long c, const RiemannOut<T> &o, bool lowest) {
+    a.stale[D][c] = o.none ? 1 : 0;
+    if (!o.none) {
+        store_flux<T, D>(a, c, o);
+        return;
+    }
+    if (lowest) return;
+    if constexpr (D == 0) {
+        if (a.ndim == 1) return;
+        T fbz, fby;
+        const int p = a.ndim == 3 ? 2 : 1;
+        if (a.ndim == 3) {
+            fbz = a.stale[2][c] ? -a.F[1][5][c] : T(0);
+            fby = a.F[2][5][c];
+        } else {
+            fbz = -a.F[1][5][c];
+            fby = a.stale[1][c] ? -a.F[0][6][c] : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < 5; ++i) a.F[0][i][c] = a.F[p][i][c];
+        a.F[0][5][c] = fbz;
+        a.F[0][6][c] = -fby;
+    } else if constexpr (D == 1) {
+        T fbz = a.F[0][5][c];
+        T fbx = a.stale[0][c] ? (a.ndim == 3 ? -a.F[2][6][c] : a.F[1][6][c]) : T(0);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) a.F[1][i][c] = a.F[0][i][c];
+        a.F[1][5][c] = -fbz;
+        a.F[1][6][c] = fbx;
+    } else {
+        T fbx = a.F[1][6][c];
+        T fby = a.stale[1][c] ? -a.F[0][6][c] : T(0);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) a.F[2][i][c] = a.F[1][i][c];
+        a.F[2][5][c] = fby;
+        a.F[2][6][c] = -fbx;
+    }
+}
+
+template <class T, int D, int S>
+__device__ __forceinline__ void store_face(const ArgsT<T> &a, long c, const RiemannOut<T> &o, bool lowest) {
+    if constexpr (S == 1) store_flux_hll<T, D>(a, c, o, lowest);
+    else store_flux<T, D>(a, c, o);
+}
+
 __device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v) {
     __shared__ unsigned long long red[16];
 #pragma unroll
@@ -288,7 +353,7 @@ __global__ __launch_bounds__(256) void k_flux_x(ArgsT<T> a) {
         Q<T> qL = to_q<T, 0>(fp, bn), qR = to_q<T, 0>(nm, bn);
         RiemannOut<T> o;
         riemann<T, 0, S>(qL, qR, fp.eps, nm.eps, fp.cs, nm.cs, a.bn_mode, (T)a.S->carry[0], o);
-        store_flux<T, 0>(a, c, o);
+        store_face<T, 0, S>(a, c, o, j == 0);
         if (o.ds) key = serial_key<0>(a, j, k, l);
     }
     if constexpr (S == 3) {
@@ -353,7 +418,7 @@ __global__ __launch_bounds__(256) void k_flux_yz(ArgsT<T> a, int nseg) {
             Q<T> qL = to_q<T, D>(prev, bn), qR = to_q<T, D>(fm, bn);
             RiemannOut<T> o;
             riemann<T, D, S>(qL, qR, prev.eps, fm.eps, prev.cs, fm.cs, a.bn_mode, carry, o);
-            store_flux<T, D>(a, c, o);
+            store_face<T, D, S>(a, c, o, i == 1);
             if (o.ds) {
                 unsigned long long kk2 = serial_key<D>(a, j, kk, ll);
                 key = kk2 > key ? kk2 : key;
@@ -551,27 +616,53 @@ template <class W> __global__ __launch_bounds__(256) void k_edge(ArgsT<W> a) {
     }
 }
 
+// FIXPOLE (mhd_module.f90:437-461): the radial edge EMF on each polar face
+// (theta faces 0 and ny) becomes its average over phi, written to every phi
+// edge l=-2..nz+3.  The reference's accumulators sum1/sum2 are REAL (:293):
+// each addition is done in double and rounded to single, in the order of the
+// l loop of a one-thread run.  One thread per (radius j, pole); nz loads each.
+template <class W> __global__ __launch_bounds__(256) void k_fixpole(ArgsT<W> a) {
+    if (!a.S->active) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * a.nx) return;
+    const int j = 1 + (t >> 1), k = (t & 1) ? a.ny : 0;
+    W *ex = a.E[0];
+    float sum = 0.0f;
+    for (int l = 1; l <= a.nz; ++l) sum = (float)((double)sum + (double)ex[cidx(a, j, k, l)]);
+    const W avg = (W)((double)sum / (double)a.nz);
+    for (int l = -2; l <= a.nz + 3; ++l) ex[cidx(a, j, k, l)] = avg;
+}
+
 // ---------------------------------------------------------------------------
 // update: FLUX_DIFF + sources + l_operator + RK combine + FROMUTORVE +
 // CUSTOM_CHECKRHO + FROMRVETOU + FINDPRESSURE (+ FINDDT) for one interior cell
 // ---------------------------------------------------------------------------
 
-// l_rk of the face fields (l_operator, spatial.f90:639-680) from the four
+// l_rk of the face fields (l_operator, spatial.f90:639-702) from the four
 // edge EMFs around the face (read in the working type, differenced in double):
 // x face jf: ez at (jf,k,l),(jf,k-1,l), ey at (jf,k,l),(jf,k,l-1);
 // y face kf: ex at (j,kf,l),(j,kf,l-1), ez at (j,kf,l),(j-1,kf,l);
 // z face lf: ey at (j,k,lf),(j-1,k,lf), ex at (j,k,lf),(j,k-1,lf).
+// parameter.h:24 small_num = TINY(1.0D0), the smallest normal double
+constexpr double SMALL_NUM = 2.2250738585072014e-308;
+
 template <class A>
 __device__ __forceinline__ double lB_x(const A &a, int jf, int k, int l, double ez0, double ez1, double ey0, double ey1) {
     double dy = a.ax[1].d[k + 3], dz = a.ax[2].d[l + 3];
     if (a.coord == 0) return -((ez0 - ez1) / (dy) - (ey0 - ey1) / (dz));
     double xF = a.ax[0].xF[jf + 3];
+    if (a.coord == 2)   // spatial.f90:689
+        return (-(a.sinf[k + 3] * ez0 - a.sinf[k + 2] * ez1) + (ey0 - ey1) * (dy / dz)) / (xF * a.dcose[k + 3] + SMALL_NUM);
     return -(ez0 - ez1) / (xF * dy) + (ey0 - ey1) / (dz);
 }
 template <class A>
-__device__ __forceinline__ double lB_y(const A &a, int j, int l, double ex0, double ex1, double ez0, double ez1) {
+__device__ __forceinline__ double lB_y(const A &a, int j, int kf, int l, double ex0, double ex1, double ez0, double ez1) {
     double dx = a.ax[0].d[j + 3], dz = a.ax[2].d[l + 3];
     if (a.coord == 0) return -((ex0 - ex1) / (dz) - (ez0 - ez1) / (dx));
+    if (a.coord == 2) {   // spatial.f90:692
+        double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2], x = a.ax[0].c[j + 3];
+        return (-(ex0 - ex1) / (a.xbar[j + 3] * a.sinf[kf + 3] * dz + SMALL_NUM) + (xF * ez0 - xFm * ez1) / (x * dx));
+    }
     return -(ex0 - ex1) / (dz) + (ez0 - ez1) / (dx);
 }
 template <class A>
@@ -579,6 +670,8 @@ __device__ __forceinline__ double lB_z(const A &a, int j, int k, double ey0, dou
     double dx = a.ax[0].d[j + 3], dy = a.ax[1].d[k + 3];
     if (a.coord == 0) return -((ey0 - ey1) / (dx) - (ex0 - ex1) / (dy));
     double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2], x = a.ax[0].c[j + 3];
+    if (a.coord == 2)   // spatial.f90:695
+        return (-(xF * ey0 - xFm * ey1) / (x * dx) + (ex0 - ex1) / (a.xbar[j + 3] * dy));
     return -(xF * ey0 - xFm * ey1) / (x * dx) + (ex0 - ex1) / (x * dy);
 }
 
@@ -634,9 +727,12 @@ template <class A> __device__ __forceinline__ double cfl_dt(const A &a, int j, i
     if (a.coord == 0) {
         if (a.ndim > 1) h = fmin2(h, a.ax[1].d[k + 3]);
         if (a.ndim > 2) h = fmin2(h, a.ax[2].d[l + 3]);
-    } else {
+    } else if (a.coord == 1) {
         if (a.ndim > 1 && a.ny > 1) h = fmin2(h, a.ax[0].c[j + 3] * a.ax[1].d[k + 3]);
         if (a.ndim > 2) h = fmin2(h, a.ax[2].d[l + 3]);
+    } else {
+        if (a.ndim > 1) h = fmin2(h, a.ax[0].c[j + 3] * a.ax[1].d[k + 3]);
+        if (a.ndim > 2) h = fmin2(h, a.ax[0].c[j + 3] * a.sine[k + 3] * a.ax[2].d[l + 3]);
     }
     return h * a.cfl / lam;
 }
@@ -666,11 +762,16 @@ __device__ __forceinline__ double block_min_f64(double v, DevScalars *S) {
 
 // geometry-weighted cell average of face field component D (fromrvetou.f90:153-195)
 template <int D, class A>
-__device__ __forceinline__ double bavg(const A &a, int j, double up, double lo) {
+__device__ __forceinline__ double bavg(const A &a, int j, int k, double up, double lo) {
     if (D == 0 && a.coord == 1) {
         double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2];
         return (xF * up + xFm * lo) / (xF + xFm);
     }
+    if (D == 0 && a.coord == 2) {
+        double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2];
+        return a.hbx[j + 3] * (xF * xF * up + xFm * xFm * lo);
+    }
+    if (D == 1 && a.coord == 2) return a.hby[k + 3] * (a.sinf[k + 3] * up + a.sinf[k + 2] * lo);
     return 0.5 * (up + lo);
 }
 
@@ -684,7 +785,7 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
     // stage-start primitives: FROMRVETOU input (fromrvetou.f90:428-438) in the
     // FP64 mode, source terms only in the FP32 split
     double rho = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
-    if (!split || a.coord == 1 || a.model == 1 || a.rt_gravity) {
+    if (!split || a.coord != 0 || a.model != 0 || a.rt_gravity) {
         rho = a.P[0][c]; vx = a.P[1][c]; vy = a.P[2][c]; vz = a.P[3][c];
         bx = a.Bc[0][c]; by = a.Bc[1][c]; bz = a.Bc[2][c];
     }
@@ -711,8 +812,21 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
         double bsq = bx * bx + by * by + bz * bz, xj = a.ax[0].c[j + 3];
         sc[1] = sc[1] + (a.P[4][c] + rho * vy * vy + 0.5 * bsq - by * by) / (xj);
         sc[2] = sc[2] - (rho * vx * vy - bx * by) / (xj);
+    } else if (a.coord == 2) {   // spatial.f90:227-256
+        double bsq = bx * bx + by * by + bz * bz, p = a.P[4][c];
+        double g = a.gsrc[j + 3], cot = a.cot[k + 3];
+        sc[1] = sc[1] + (2.0 * p + bx * bx + rho * (vy * vy + vz * vz)) * g;
+        sc[2] = sc[2] + (p + rho * vz * vz + 0.5 * bsq - bz * bz) * g * cot - (rho * vx * vy - bx * by) * g;
+        sc[3] = sc[3] - (rho * vx * vz - bx * bz) * g - (rho * vy * vz - by * bz) * g * cot;
     }
-    if (a.model == 1) {
+    if (a.model == 2) {   // model/torus-sph/src/custom.f90:283-296
+        double diff = rho - a.rfl[c];
+        double f = fmax2(fsign1(diff), 0.0);
+        double xj = a.ax[0].c[j + 3];
+        double dphidr = 1.0 / ((xj - a.r_sh) * (xj - a.r_sh));
+        sc[1] = sc[1] + (-f * rho * dphidr);
+        sc[4] = sc[4] + (-f * rho * vx * dphidr);
+    } else if (a.model == 1) {
         double diff = rho - a.rfl[c];
         double f = fmax2(fsign1(diff), 0.0);
         double xj = a.ax[0].c[j + 3], zl = a.ax[2].c[l + 3];
@@ -735,14 +849,19 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
             double df;
             if (a.coord == 1)
                 df = (a.ax[0].xF[j + 3] * f1 - a.ax[0].xF[j + 2] * f0) / (a.ax[0].c[j + 3] * dx);
-            else df = (f1 - f0) / (dx);
+            else if (a.coord == 2) {
+                double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2];
+                df = (xF * xF * f1 - xFm * xFm * f0) / a.dcb3[j + 3];
+            } else df = (f1 - f0) / (dx);
             L[i] = -df;
         }
         if (a.ndim > 1) {
             double dy = a.ax[1].d[k + 3];
             for (int i = 0; i < 5; ++i) {
                 const double f1 = a.F[1][i][c], f0 = a.F[1][i][c - sy];
-                double df = (a.coord == 1) ? (f1 - f0) / (a.ax[0].c[j + 3] * dy) : (f1 - f0) / (dy);
+                double df = (a.coord == 1) ? (f1 - f0) / (a.ax[0].c[j + 3] * dy)
+                            : (a.coord == 2) ? (a.sinf[k + 3] * f1 - a.sinf[k + 2] * f0) / (a.xbar[j + 3] * a.dcose[k + 3])
+                                             : (f1 - f0) / (dy);
                 L[i] = L[i] - df;
             }
         }
@@ -750,7 +869,9 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
             double dz = a.ax[2].d[l + 3];
             for (int i = 0; i < 5; ++i) {
                 const double f1 = a.F[2][i][c], f0 = a.F[2][i][c - sz];
-                L[i] = L[i] - (f1 - f0) / dz;
+                if (a.coord == 2)
+                    L[i] = L[i] - (f1 - f0) * a.ax[1].d[k + 3] / (a.xbar[j + 3] * dz * a.dcose[k + 3]);
+                else L[i] = L[i] - (f1 - f0) / dz;
             }
         }
         for (int i = 0; i < 5; ++i) L[i] = L[i] + sc[i];
@@ -766,10 +887,10 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
     {
         const long cl[3] = {c - 1, c - sy, c - sz};
         const int fidx[3] = {j, k, l};
-        const double Lup[3] = {lB_x(a, j, k, l, ez_c, ez_my, ey_c, ey_mz), lB_y(a, j, l, ex_c, ex_mz, ez_c, ez_mx),
+        const double Lup[3] = {lB_x(a, j, k, l, ez_c, ez_my, ey_c, ey_mz), lB_y(a, j, k, l, ex_c, ex_mz, ez_c, ez_mx),
                                lB_z(a, j, k, ey_c, ey_mx, ex_c, ex_my)};
         const double Llo[2] = {lB_x(a, j - 1, k, l, ez_mx, ez_mxmy, ey_mx, ey_mxmz),
-                               lB_y(a, j, l, ex_my, ex_mymz, ez_my, ez_mxmy)};
+                               lB_y(a, j, k - 1, l, ex_my, ex_mymz, ez_my, ez_mxmy)};
         for (int d = 0; d < 3; ++d) {
             double bup = a.Bin[d][c];
             double uoup;
@@ -795,8 +916,8 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
         }
     }
     // FROMUTORVE (fromrvetou.f90:153-211)
-    double bce[3] = {bavg<0>(a, j, bn_up[0], bn_lo[0]), bavg<1>(a, j, bn_up[1], bn_lo[1]),
-                     bavg<2>(a, j, bn_up[2], bn_lo[2])};
+    double bce[3] = {bavg<0>(a, j, k, bn_up[0], bn_lo[0]), bavg<1>(a, j, k, bn_up[1], bn_lo[1]),
+                     bavg<2>(a, j, k, bn_up[2], bn_lo[2])};
     double rn = un[0];
     double v[3] = {un[1] / rn, un[2] / rn, un[3] / rn};
     double bsq = bce[0] * bce[0] + bce[1] * bce[1] + bce[2] * bce[2];
@@ -805,8 +926,8 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
     checkrho(a, c, rn, v, en);
     if (STAGE == 3) checkrho(a, c, rn, v, en);
     // FROMRVETOU bcell (fromrvetou.f90:380-422)
-    double bcf[3] = {bavg<0>(a, j, bn_up[0], bprim_lo[0]), bavg<1>(a, j, bn_up[1], bprim_lo[1]),
-                     bavg<2>(a, j, bn_up[2], bprim_lo[2])};
+    double bcf[3] = {bavg<0>(a, j, k, bn_up[0], bprim_lo[0]), bavg<1>(a, j, k, bn_up[1], bprim_lo[1]),
+                     bavg<2>(a, j, k, bn_up[2], bprim_lo[2])};
     double p, cs;
     pressure(a, c, rn, en, p, cs);
     a.P[0][c] = (W)rn;
@@ -953,10 +1074,14 @@ template <class W> __global__ __launch_bounds__(256) void k_ghost(ArgsT<W> a) {
     if (gy || gz) a.Bin[0][c] = a.Bin[0][cidx(a, j, kk, ll)] * (gsign(a, 1, sy, 6) * gsign(a, 2, sz, 6));
     if (gx || gz) a.Bin[1][c] = a.Bin[1][cidx(a, jj, k, ll)] * (gsign(a, 0, sx, 7) * gsign(a, 2, sz, 7));
     if (gx || gy) a.Bin[2][c] = a.Bin[2][cidx(a, jj, kk, l)] * (gsign(a, 0, sx, 8) * gsign(a, 1, sy, 8));
-    // torus no-inflow clamps (model/torus-cyn/src/custom.f90:156-161, 194-199)
-    if (a.model == 1) {
+    // torus no-inflow clamps: radial in both torus models
+    // (model/torus-cyn/src/custom.f90:156-161, model/torus-sph/src/custom.f90:160-167),
+    // vertical in the cylindrical one (model/torus-cyn/src/custom.f90:194-199)
+    if (a.model != 0) {
         if (sx == 0) v[0] = fmin2(v[0], 0.0);
         if (sx == 1) v[0] = fmax2(v[0], 0.0);
+    }
+    if (a.model == 1) {
         if (sz == 0) v[2] = fmin2(v[2], 0.0);
         if (sz == 1) v[2] = fmax2(v[2], 0.0);
     }

@@ -274,9 +274,11 @@ __device__ __forceinline__ T sstar_hllc(T pl, T pr, T rl, T rr, T vl, T vr, T sl
 // Result of one interface.  ds: the HLLD double-star branch was taken;
 // uss_t1/uss_t2: the transverse fields of that double-star state, needed to
 // reproduce the reference's carried usstar(ibn) (see riemann()).
+// none: HLL took none of its branches (a NaN signal speed) and assigned no flux.
 template <class T> struct RiemannOut {
     T f[8];
     bool ds;
+    bool none;
     T uss_t1, uss_t2;
 };
 
@@ -289,6 +291,7 @@ __device__ __forceinline__ void riemann(const Q<T> &qL, const Q<T> &qR, T epsL, 
     build_state<T, D>(qL, epsL, uL, fL);
     build_state<T, D>(qR, epsR, uR, fR);
     out.ds = false;
+    out.none = false;
     out.uss_t1 = T(0);
     out.uss_t2 = T(0);
     T cfsL = signalspeed(csL, qL.b[n], qL.b[t1], qL.b[t2], qL.rho);
@@ -302,10 +305,11 @@ __device__ __forceinline__ void riemann(const Q<T> &qL, const Q<T> &qR, T epsL, 
     T cbar = roe_avg(cfsL, cfsR, qL.rho, qR.rho);
     T sL = fmin2(qL.v[n] - cfsL, ubar - cbar);
     T sR = fmax2(qR.v[n] + cfsR, ubar + cbar);
-    if constexpr (S == 1) {   // HLLNM, rsolver_module.f90:303-311
+    if constexpr (S == 1) {   // HLLNM, rsolver_module.f90:303-311 (no ELSE branch)
         if (sL >= T(0)) { for (int i = 0; i < 8; ++i) out.f[i] = fL[i]; }
         else if (sL <= T(0) && sR >= T(0)) { for (int i = 0; i < 8; ++i) out.f[i] = hll_flux(fL[i], fR[i], uL[i], uR[i], sL, sR); }
-        else { for (int i = 0; i < 8; ++i) out.f[i] = fR[i]; }
+        else if (sR <= T(0)) { for (int i = 0; i < 8; ++i) out.f[i] = fR[i]; }
+        else out.none = true;
         return;
     }
     if (sL >= T(0)) { for (int i = 0; i < 8; ++i) out.f[i] = fL[i]; return; }

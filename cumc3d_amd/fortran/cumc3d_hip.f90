@@ -11,9 +11,17 @@ INTEGER(C_INT), PARAMETER :: CUMC3D_OK = 0, CUMC3D_EINVAL = 1, CUMC3D_EHIP = 2, 
 INTEGER(C_INT), PARAMETER :: CUMC3D_LF = 0, CUMC3D_HLL = 1, CUMC3D_HLLC = 2, CUMC3D_HLLD = 3
 INTEGER(C_INT), PARAMETER :: CUMC3D_TVDMM = 0, CUMC3D_TVDVL = 1, CUMC3D_TVDMC = 2, &
                              CUMC3D_PPM = 3, CUMC3D_WENO = 4
-INTEGER(C_INT), PARAMETER :: CUMC3D_MODEL_RIEMANN = 0, CUMC3D_MODEL_TORUS = 1
+INTEGER(C_INT), PARAMETER :: CUMC3D_MODEL_RIEMANN = 0, CUMC3D_MODEL_TORUS = 1, CUMC3D_MODEL_TORUS_SPH = 2
 INTEGER(C_INT), PARAMETER :: CUMC3D_FP64 = 0, CUMC3D_FP32 = 1
 INTEGER(C_INT), PARAMETER :: CUMC3D_BN_ZERO = 0, CUMC3D_BN_FIELD = 1, CUMC3D_BN_CARRY = 2
+
+TYPE, BIND(C) :: cumc3d_grid_metrics
+  TYPE(C_PTR) :: dx, dy, dz
+  TYPE(C_PTR) :: x, y, z
+  TYPE(C_PTR) :: dx_cb, xbar
+  TYPE(C_PTR) :: dcose, dsine, sine
+  TYPE(C_PTR) :: sinf
+END TYPE
 
 TYPE, BIND(C) :: cumc3d_desc
   INTEGER(C_INT) :: nx, ny, nz
@@ -23,6 +31,8 @@ TYPE, BIND(C) :: cumc3d_desc
   INTEGER(C_INT) :: riemann, recon, model, rt_gravity, hlld_bn_mode, precision
   REAL(C_DOUBLE) :: cfl, ggas, r_sh
   TYPE(C_PTR) :: xF, yF, zF
+  INTEGER(C_INT) :: fixpole
+  TYPE(C_PTR) :: metrics
 END TYPE
 
 INTERFACE

@@ -11,7 +11,13 @@
 ! releases the device state (CLEAR_DEVICE, openacc.f90:30-47).
 !
 ! Build: compile with -cpp next to the solver's modules (and the model's
-! param.h), -DCUMC3D_TORUS for model/torus-cyn; link libcumc3d_hip.so.
+! param.h), with the solver's own -DFIXPOLE, -DCUMC3D_TORUS for
+! model/torus-cyn, -DCUMC3D_TORUS -DCUMC3D_TORUS_SPH for model/torus-sph;
+! link libcumc3d_hip.so.
+!
+! The grid goes over as the module's GETGRID arrays (face coordinates plus
+! dx, x, dx_cb, dcose, sinf, ...), so CUSTOM_GRID and FIXPOLE (which moves
+! yF after dy and the sines are taken, getgrid.f90:91-98) carry over exactly.
 MODULE CUMC3D_HIP_STATE
 USE ISO_C_BINDING
 USE CUMC3D_HIP
@@ -19,6 +25,7 @@ IMPLICIT NONE
 TYPE(C_PTR), SAVE :: hip_ctx = C_NULL_PTR
 LOGICAL, SAVE :: hip_attached = .false.
 REAL(C_DOUBLE), ALLOCATABLE, TARGET, SAVE :: hip_grid(:)
+TYPE(cumc3d_grid_metrics), TARGET, SAVE :: hip_metrics
 
 CONTAINS
 
@@ -33,7 +40,7 @@ SUBROUTINE CUMC3D_HIP_ATTACH
   INCLUDE "param.h"
   TYPE(cumc3d_desc) :: d
   REAL(C_DOUBLE), ALLOCATABLE :: floors(:,:,:,:)
-  INTEGER :: o1, o2
+  INTEGER :: o(16)
   IF (hip_attached) RETURN
   d%nx = nx
   d%ny = ny
@@ -54,7 +61,11 @@ SUBROUTINE CUMC3D_HIP_ATTACH
   d%precision = CUMC3D_FP64
   d%cfl = cfl
   d%ggas = ggas
-#ifdef CUMC3D_TORUS
+#if defined(CUMC3D_TORUS_SPH)
+  d%model = CUMC3D_MODEL_TORUS_SPH
+  d%rt_gravity = 0
+  d%r_sh = r_sh
+#elif defined(CUMC3D_TORUS)
   d%model = CUMC3D_MODEL_TORUS
   d%rt_gravity = 0
   d%r_sh = r_sh
@@ -64,16 +75,62 @@ SUBROUTINE CUMC3D_HIP_ATTACH
   d%rt_gravity = MERGE(1, 0, test_model == 6 .OR. test_model == 18)
   d%r_sh = 0.0D0
 #endif
-  ! face coordinates xF(-3:n+3) etc. (setup.f90:363-365), n+7 values each
-  o1 = nx + 7
-  o2 = o1 + ny + 7
-  ALLOCATE(hip_grid(o2 + nz + 7))
-  hip_grid(1:o1) = xF(-3:nx+3)
-  hip_grid(o1+1:o2) = yF(-3:ny+3)
-  hip_grid(o2+1:) = zF(-3:nz+3)
-  d%xF = C_LOC(hip_grid(1))
-  d%yF = C_LOC(hip_grid(o1+1))
-  d%zF = C_LOC(hip_grid(o2+1))
+#ifdef FIXPOLE
+  d%fixpole = 1
+#else
+  d%fixpole = 0
+#endif
+  ! GETGRID arrays (setup.f90:360-374) copied into one target buffer: faces
+  ! xF(-3:n+3) (n+7 values), widths and centres (-2:n+3), the spherical
+  ! dx_cb, xbar (-2:nx+3), dcose, dsine, sine (-2:ny+3) and sinf(-3:ny+3)
+  o(1) = 1
+  o(2) = o(1) + nx + 7
+  o(3) = o(2) + ny + 7
+  o(4) = o(3) + nz + 7
+  o(5) = o(4) + nx + 6
+  o(6) = o(5) + ny + 6
+  o(7) = o(6) + nz + 6
+  o(8) = o(7) + nx + 6
+  o(9) = o(8) + ny + 6
+  o(10) = o(9) + nz + 6
+  o(11) = o(10) + nx + 6
+  o(12) = o(11) + nx + 6
+  o(13) = o(12) + ny + 6
+  o(14) = o(13) + ny + 6
+  o(15) = o(14) + ny + 6
+  o(16) = o(15) + ny + 7
+  ALLOCATE(hip_grid(o(16) - 1))
+  hip_grid(o(1):o(2)-1) = xF(-3:nx+3)
+  hip_grid(o(2):o(3)-1) = yF(-3:ny+3)
+  hip_grid(o(3):o(4)-1) = zF(-3:nz+3)
+  hip_grid(o(4):o(5)-1) = dx(-2:nx+3)
+  hip_grid(o(5):o(6)-1) = dy(-2:ny+3)
+  hip_grid(o(6):o(7)-1) = dz(-2:nz+3)
+  hip_grid(o(7):o(8)-1) = x(-2:nx+3)
+  hip_grid(o(8):o(9)-1) = y(-2:ny+3)
+  hip_grid(o(9):o(10)-1) = z(-2:nz+3)
+  hip_grid(o(10):o(11)-1) = dx_cb(-2:nx+3)
+  hip_grid(o(11):o(12)-1) = xbar(-2:nx+3)
+  hip_grid(o(12):o(13)-1) = dcose(-2:ny+3)
+  hip_grid(o(13):o(14)-1) = dsine(-2:ny+3)
+  hip_grid(o(14):o(15)-1) = sine(-2:ny+3)
+  hip_grid(o(15):o(16)-1) = sinf(-3:ny+3)
+  d%xF = C_LOC(hip_grid(o(1)))
+  d%yF = C_LOC(hip_grid(o(2)))
+  d%zF = C_LOC(hip_grid(o(3)))
+  hip_metrics%dx = C_LOC(hip_grid(o(4)))
+  hip_metrics%dy = C_LOC(hip_grid(o(5)))
+  hip_metrics%dz = C_LOC(hip_grid(o(6)))
+  hip_metrics%x = C_LOC(hip_grid(o(7)))
+  hip_metrics%y = C_LOC(hip_grid(o(8)))
+  hip_metrics%z = C_LOC(hip_grid(o(9)))
+  hip_metrics%dx_cb = C_LOC(hip_grid(o(10)))
+  hip_metrics%xbar = C_LOC(hip_grid(o(11)))
+  hip_metrics%dcose = C_LOC(hip_grid(o(12)))
+  hip_metrics%dsine = C_LOC(hip_grid(o(13)))
+  hip_metrics%sine = C_LOC(hip_grid(o(14)))
+  hip_metrics%sinf = C_LOC(hip_grid(o(15)))
+  d%metrics = C_LOC(hip_metrics)
   CALL cumc3d_check(cumc3d_init(d, hip_ctx), 'cumc3d_init')
 #ifdef CUMC3D_TORUS
   ALLOCATE(floors(-2:nx+3,-2:ny+3,-2:nz+3,3))

@@ -7,7 +7,7 @@ ghosted box (-2..n+3 per axis, solver/mhd/setup.f90:348-351):
     bx, by, bz              face-centred field: bx[l,k,j] is Bx on face j+1/2
                             (prim(ibx..ibz), solver/mhd/fromrvetou.f90:54-56)
     eps                     specific internal energy (epsilon)
-    rho_floor, p_floor, eps_floor   (torus model only, model/torus-cyn/src/def.f90)
+    rho_floor, p_floor, eps_floor   (torus models only, model/torus-cyn/src/def.f90)
 
 The generators build the five measurement workloads; the Brio-Wu tube follows
 model/riemann/src/riemann1d.f90:229-250.  Random perturbations use a counter
@@ -317,10 +317,135 @@ def torus(cfg: Config, seed: int = 20261202, r_in: float = 6.0, r_max: float = 1
     return st
 
 
+def _sph_face_field(cfg: Config, g: Grid, m: dict, ac: np.ndarray):
+    """Face Bx (radial) and By (polar) on faces j,k = 0..n from a corner
+    vector potential A_phi(xF(j), yF(k)) given as ac[k+1, j+1] for j,k = -1..n,
+    by the spherical curl of model/torus-sph/src/model.f90:146-152:
+    Bx = (sinf(k) A(j,k) - sinf(k-1) A(j,k-1)) / (xF(j) dcose(k) + tiny),
+    By = -(xF(j) A(j,k) - xF(j-1) A(j-1,k)) / (x(j) dx(j))."""
+    nx, ny = cfg.nx, cfg.ny
+    tiny = np.finfo(np.float64).tiny                        # parameter.h:24 small_num
+    xF = g.x.faces[2:nx + 4]                                # xF(-1..nx)
+    xc = g.x.c[2:nx + 3][None, :]                           # x(0..nx)
+    dx = g.x.d[2:nx + 3][None, :]
+    sinf = m["sinf"][2:ny + 4][:, None]                     # sinf(-1..ny)
+    dcose = m["dcose"][3:ny + 4][:, None]                   # dcose(0..ny)
+    bx = (sinf[1:] * ac[1:, 1:] - sinf[:-1] * ac[:-1, 1:]) / (xF[None, 1:] * dcose + tiny)
+    by = -(xF[None, 1:] * ac[1:, 1:] - xF[None, :-1] * ac[1:, :-1]) / (xc * dx)
+    return bx, by                                           # [k, j] for k,j = 0..n
+
+
+def torus_sph(cfg: Config, seed: int = 20261202, p_beta: float = 350.0, q_grad: float = 2.0,
+              s_in: float = 3.0, s_max: float = 4.7, rho_cut: float = 0.5, rho_fac: float = 1.0e-4,
+              rho_max: float = 1.0, noise: float = 1.0e-4) -> dict:
+    """Spherical (r, theta, phi) torus of model/torus-sph/src/model.f90:57-301:
+    a power-law angular-momentum torus (q = 2) in the Paczynski-Wiita
+    potential -1/(r - r_sh), polytropic with rho_max = 1 at s_max, inner edge
+    s_in, atmosphere rho_fac*rho_max; poloidal loops from
+    A_phi = max(rho - rho_cut, 0)/rho_max at cell corners, normalised to
+    minimum plasma beta p_beta; density perturbed by 1e-4 relative noise inside
+    the torus.  Parameters default to model/torus-sph/src/param.h."""
+    assert cfg.coordinate_flag == 2
+    g = Grid(cfg)
+    m = g.spherical()
+    gam, rsh = cfg.ggas, cfg.r_sh
+    nx, ny = cfg.nx, cfg.ny
+    psi = lambda r: -1.0 / (r - rsh)
+    e = 2.0 - 2.0 * q_grad
+    lkep2 = s_max ** (2.0 * q_grad - 1.0) / (s_max - rsh) ** 2
+    c_const = psi(s_in) - lkep2 * s_in ** e / e
+    h_max = -psi(s_max) + lkep2 * s_max ** e / e + c_const
+    k_poly = h_max * (gam - 1.0) / gam / rho_max ** (gam - 1.0)
+    if k_poly < 0.0:
+        raise ValueError("the polytropic constant is negative")
+    R = g.x.c[3:nx + 3][None, :]                            # x(1..nx)
+    s_eq = R * m["sine"][4:ny + 4][:, None]                 # x(j) sine(k), k = 1..ny
+    h = np.maximum(-psi(R) + lkep2 * s_eq ** e / e + c_const, 0.0)
+    rho_loc = (h * (gam - 1.0) / gam / k_poly) ** (1.0 / (gam - 1.0))
+    rho_atm = rho_fac * rho_max
+    inside = (rho_loc > rho_atm) & (s_eq >= s_in)
+    rho = np.where(inside, rho_loc, rho_atm)
+    vphi = np.where(inside, math.sqrt(lkep2) * s_eq ** (-q_grad) * s_eq, 0.0)
+    A = np.zeros((ny + 2, nx + 2))                          # A_phi(j,k), j,k = 0..n+1 at [k, j]
+    A[1:ny + 1, 1:nx + 1] = np.where(rho >= rho_cut, (rho - rho_cut) / rho_max, 0.0)
+
+    def faces(scale):
+        Av = A * scale
+        ac = np.zeros((ny + 2, nx + 2))                     # A_corner(j,k), j,k = -1..n at [k+1, j+1]
+        ac[2:, 2:] = 0.25 * (Av[1:ny + 1, 1:nx + 1] + Av[1:ny + 1, 2:] + Av[2:, 1:nx + 1] + Av[2:, 2:])
+        return _sph_face_field(cfg, g, m, ac)
+
+    bx, by = faces(1.0)
+    # minimum-beta normalisation over cells with field (model.f90:160-180)
+    xF = g.x.faces
+    dcb, dxx = m["dx_cb"][4:nx + 4][None, :], g.x.d[3:nx + 3][None, :]
+    bcx = 1.5 * dxx / dcb * (xF[4:nx + 4] ** 2 * bx[1:, 1:] + xF[3:nx + 3] ** 2 * bx[1:, :-1])
+    dyy, dce = g.y.d[3:ny + 3][:, None], m["dcose"][4:ny + 4][:, None]
+    sf = m["sinf"]
+    bcy = 0.5 * dyy / dce * (sf[4:ny + 4][:, None] * by[1:, 1:] + sf[3:ny + 3][:, None] * by[:-1, 1:])
+    b2 = bcx ** 2 + bcy ** 2
+    p = k_poly * rho ** gam
+    if np.any(b2 > 0):
+        beta_min = float(np.min(2.0 * p[b2 > 0] / b2[b2 > 0]))
+        bx, by = faces(math.sqrt(beta_min / p_beta))
+    st = empty_state(cfg)
+    I = (slice(3, cfg.nz + 3), slice(3, ny + 3), slice(3, nx + 3))
+    F = (slice(2, cfg.nz + 3), slice(2, ny + 3), slice(2, nx + 3))   # faces 0..n
+    st["rho"][I] = rho
+    st["vz"][I] = vphi
+    st["p"][I] = p
+    st["eps"][I] = k_poly * rho ** (gam - 1.0) / (gam - 1.0)
+    st["bx"][F] = bx
+    st["by"][F] = by
+    fl = np.zeros(cfg.shape)
+    fl[I] = rho_atm
+    st["rho_floor"] = fl
+    st["p_floor"] = k_poly * fl ** gam
+    st["eps_floor"] = np.where(fl > 0, k_poly * fl ** (gam - 1.0) / (gam - 1.0), 0.0)
+    u = interior_uniform(cfg, seed)
+    r = st["rho"]
+    st["rho"] = np.where(r > fl, r + r * (u - 0.5) / 0.5 * noise, r)
+    return st
+
+
+def sph_blast(cfg: Config, seed: int = 20261202, centre=(1.0, 0.35, 1.0), radius: float = 0.3,
+              p_in: float = 10.0, p_out: float = 0.1, b0: float = 1.0) -> dict:
+    """Magnetised blast on a spherical grid near the polar axis: uniform
+    density 1, an over-pressured ball (centre given as r, theta, phi) in a
+    uniform field b0 along the polar axis, set from the corner vector
+    potential A_phi = b0 r sin(theta)/2 by the same spherical curl as the
+    torus, so the face field is divergence-free to round-off."""
+    assert cfg.coordinate_flag == 2
+    g = Grid(cfg)
+    m = g.spherical()
+    nx, ny = cfg.nx, cfg.ny
+    X, Y, Z = g.centres()                                    # r, theta, phi of the cell centres
+    r0, t0, f0 = centre
+    cx = np.array([r0 * math.sin(t0) * math.cos(f0), r0 * math.sin(t0) * math.sin(f0), r0 * math.cos(t0)])
+    px = X * np.sin(Y) * np.cos(Z) - cx[0]
+    py = X * np.sin(Y) * np.sin(Z) - cx[1]
+    pz = X * np.cos(Y) - cx[2]
+    inside = np.sqrt(px * px + py * py + pz * pz) < radius
+    st = empty_state(cfg)
+    st["rho"][:] = 1.0
+    st["p"][:] = np.where(inside, p_in, p_out)
+    st["eps"][:] = st["p"] / st["rho"] / (cfg.ggas - 1.0)
+    xF = g.x.faces[2:nx + 4]                                 # xF(-1..nx)
+    sinf = m["sinf"][2:ny + 4]                               # sinf(-1..ny)
+    ac = 0.5 * b0 * xF[None, :] * sinf[:, None]
+    bx, by = _sph_face_field(cfg, g, m, ac)
+    F = (slice(2, cfg.nz + 3), slice(2, ny + 3), slice(2, nx + 3))
+    st["bx"][F] = bx
+    st["by"][F] = by
+    return st
+
+
 GENERATORS = {
     "briowu": brio_wu,
     "orszag_tang": orszag_tang,
     "blast": blast,
     "turbulence": turbulence,
     "torus": torus,
+    "torus_sph": torus_sph,
+    "sph_blast": sph_blast,
 }

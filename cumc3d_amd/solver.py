@@ -44,7 +44,7 @@ class Desc(C.Structure):
                 ("riemann", C.c_int), ("recon", C.c_int), ("model", C.c_int),
                 ("rt_gravity", C.c_int), ("hlld_bn_mode", C.c_int), ("precision", C.c_int),
                 ("cfl", C.c_double), ("ggas", C.c_double), ("r_sh", C.c_double),
-                ("xF", PD), ("yF", PD), ("zF", PD)]
+                ("xF", PD), ("yF", PD), ("zF", PD), ("fixpole", C.c_int), ("metrics", C.c_void_p)]
 
 
 class CumcError(RuntimeError):
@@ -125,6 +125,8 @@ class Solver:
         d.precision = PRECISION[cfg.precision]
         d.cfl, d.ggas, d.r_sh = cfg.cfl, cfg.ggas, cfg.r_sh
         d.xF, d.yF, d.zF = (_p(f) for f in self._faces)
+        d.fixpole = int(cfg.fixpole)
+        d.metrics = None
         h = C.c_void_p()
         _check(L.cumc3d_init(C.byref(d), C.byref(h)))
         self._h = h

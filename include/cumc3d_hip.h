@@ -37,8 +37,8 @@ enum {
 enum { CUMC3D_LF = 0, CUMC3D_HLL = 1, CUMC3D_HLLC = 2, CUMC3D_HLLD = 3 };
 /* Reconstructions: parameter.h:103-115 (tvdmm, tvdvl, tvdmc, ppmc, weno flags) */
 enum { CUMC3D_TVDMM = 0, CUMC3D_TVDVL = 1, CUMC3D_TVDMC = 2, CUMC3D_PPM = 3, CUMC3D_WENO = 4 };
-/* In-step model hooks: model/riemann, model/torus-cyn */
-enum { CUMC3D_MODEL_RIEMANN = 0, CUMC3D_MODEL_TORUS = 1 };
+/* In-step model hooks: model/riemann, model/torus-cyn, model/torus-sph */
+enum { CUMC3D_MODEL_RIEMANN = 0, CUMC3D_MODEL_TORUS = 1, CUMC3D_MODEL_TORUS_SPH = 2 };
 /* Arithmetic of the spatial operator */
 enum { CUMC3D_FP64 = 0, CUMC3D_FP32 = 1 };
 /* Value of HLLD's unassigned usstar(ibn) (rsolver_module.f90:704-741):
@@ -46,10 +46,27 @@ enum { CUMC3D_FP64 = 0, CUMC3D_FP32 = 1 };
  * normal field (Miyoshi & Kusano 2005), CUMC3D_BN_ZERO uses 0. */
 enum { CUMC3D_BN_ZERO = 0, CUMC3D_BN_FIELD = 1, CUMC3D_BN_CARRY = 2 };
 
+/* GETGRID outputs the step reads (getgrid.f90:43-98, after CUSTOM_GRID and
+ * the FIXPOLE override), in the reference's index ranges.  Optional: with
+ * cumc3d_desc.metrics == NULL the library derives them from the face
+ * coordinates by the same formulas, taking the faces as given before the
+ * FIXPOLE override and applying it itself.  A caller holding the reference's
+ * module arrays passes them here instead: GETGRID takes dy, y and the sines
+ * before FIXPOLE moves yF(0) and yF(ny) to 0 and pi, so the overridden faces
+ * alone do not reproduce them.  The spherical arrays may be NULL on other
+ * geometries. */
+typedef struct cumc3d_grid_metrics {
+    const double *dx, *dy, *dz;          /* dx(-2:nx+3), dy(-2:ny+3), dz(-2:nz+3) */
+    const double *x, *y, *z;             /* cell centres, same ranges */
+    const double *dx_cb, *xbar;          /* dx_cb(-2:nx+3), xbar(-2:nx+3); entry -2 unused */
+    const double *dcose, *dsine, *sine;  /* (-2:ny+3); entry -2 of dcose/dsine unused */
+    const double *sinf;                  /* sinf(-3:ny+3) */
+} cumc3d_grid_metrics;
+
 typedef struct cumc3d_desc {
     int nx, ny, nz;             /* parameter.h:70-72 */
     int n_dim;                  /* parameter.h:37 */
-    int coordinate_flag;        /* parameter.h:34: 0 Cartesian, 1 cylindrical (r,phi,z) */
+    int coordinate_flag;        /* parameter.h:34: 0 Cartesian, 1 cylindrical (r,phi,z), 2 spherical (r,theta,phi) */
     int boundary_flag[6];       /* parameter.h:52: x-in,x-out,y-in,y-out,z-in,z-out; 0..4 */
     int riemann;                /* CUMC3D_LF .. CUMC3D_HLLD */
     int recon;                  /* CUMC3D_TVDMM .. CUMC3D_WENO */
@@ -59,8 +76,12 @@ typedef struct cumc3d_desc {
     int precision;              /* CUMC3D_FP64 or CUMC3D_FP32 */
     double cfl;                 /* parameter.h:81 */
     double ggas;                /* adiabatic index (DEFINITION ggas) */
-    double r_sh;                /* model/torus-cyn/src/param.h:19 */
+    double r_sh;                /* model/torus-cyn/src/param.h:19, model/torus-sph/src/param.h:19 */
     const double *xF, *yF, *zF; /* face coordinates xF(-3:nx+3) etc. (n+7 values; GETGRID output) */
+    int fixpole;                /* -DFIXPOLE (make.inc:8-21), spherical only: polar faces at theta
+                                   0 and pi with sinf = 0 (getgrid.f90:91-98) and the radial edge
+                                   EMFs on them averaged over phi (mhd_module.f90:437-461) */
+    const cumc3d_grid_metrics *metrics;   /* optional, see cumc3d_grid_metrics */
 } cumc3d_desc;
 
 typedef struct cumc3d_ctx cumc3d_ctx;

@@ -44,7 +44,8 @@ OUT = os.path.join(HERE, "_ref")
 REPO = os.path.dirname(HERE)
 HIPLIB_DIR = os.path.join(REPO, "cumc3d_amd")
 DROPIN_SRC = [os.path.join(REPO, "cumc3d_amd", "fortran", f) for f in ("cumc3d_hip.f90", "rungekutta_hip.f90")]
-DROPIN_CASES = ("c1_briowu", "ot_hlld_weno", "ot_bc_mixed", "torus_hlld_ppm", "ot2d_hlld_ppm")
+DROPIN_CASES = ("c1_briowu", "ot_hlld_weno", "ot_bc_mixed", "torus_hlld_ppm", "ot2d_hlld_ppm",
+                "sph_torus_ppm", "sph_blast_weno")
 FFLAGS = ["-cpp", "-O2", "-ffp-contract=off", "-fopenmp", "-DDIVB", "-DFIXPOLE"]
 
 # solver/mhd/source.list minus main.f90, dataio.f90 (HDF5) and openacc.f90
@@ -58,7 +59,10 @@ MODEL_SRC = {
                                       "soundspeed.f90"]),
     "torus": ("model/torus-cyn/src", ["def.f90", "custom.f90", "findpressure.f90",
                                       "getepsilon.f90", "soundspeed.f90"]),
+    "torus_sph": ("model/torus-sph/src", ["def.f90", "custom.f90", "findpressure.f90",
+                                          "getepsilon.f90", "soundspeed.f90"]),
 }
+TORUS_MODELS = ("torus", "torus_sph")
 
 
 def fort_real(v: float) -> str:
@@ -96,8 +100,10 @@ def parameter_h(cfg) -> str:
 
 
 def torus_param_h(cfg) -> str:
-    # Names are those of model/torus-cyn/src/param.h:9-61; custom.f90 uses
-    # r_sh (CUSTOM_SOURCE) and refine_grid (CUSTOM_GRID).
+    # Names are those of model/torus-cyn/src/param.h:9-61 (model/torus-sph's
+    # param.h declares the same names); custom.f90 uses r_sh (CUSTOM_SOURCE)
+    # and refine_grid (CUSTOM_GRID); the rest only feed GET_MODEL, which the
+    # file-fed initial model replaces.
     return "\n".join([
         "! torus-cyn model parameters for the oracle build",
         "REAL*8, PARAMETER :: p_beta = 100.0D0",
@@ -134,7 +140,7 @@ def build_case(name: str) -> str:
     with open(os.path.join(work, "parameter.h"), "w") as f:
         f.write(parameter_h(cfg))
     mdir, msrc = MODEL_SRC[cfg.model]
-    if cfg.model == "torus":
+    if cfg.model in TORUS_MODELS:
         with open(os.path.join(work, "param.h"), "w") as f:
             f.write(torus_param_h(cfg))
     else:
@@ -144,7 +150,7 @@ def build_case(name: str) -> str:
             f.write("INTEGER, PARAMETER :: test_model = 0\n")
     srcs = [(os.path.join(REF, "solver/mhd", s), s) for s in SOLVER]
     srcs += [(os.path.join(REF, mdir, s), "model_" + s) for s in msrc]
-    harness = ["harness_io.f90", "ic_torus.f90" if cfg.model == "torus" else "ic_plain.f90",
+    harness = ["harness_io.f90", "ic_torus.f90" if cfg.model in TORUS_MODELS else "ic_plain.f90",
                "driver.f90"]
     objs = []
     for path, link in srcs:
@@ -171,7 +177,7 @@ def build_case(name: str) -> str:
 
 def build_dropin(name, cfg, work, objs, libdir):
     """Link the reference program with the Fortran drop-in instead of rungekutta.f90."""
-    extra = ["-DCUMC3D_TORUS"] if cfg.model == "torus" else []
+    extra = {"torus": ["-DCUMC3D_TORUS"], "torus_sph": ["-DCUMC3D_TORUS", "-DCUMC3D_TORUS_SPH"]}.get(cfg.model, [])
     dobjs = [o for o in objs if o not in ("rungekutta.o", "driver.o")]
     for src in DROPIN_SRC:
         obj = "dropin_" + os.path.basename(src).replace(".f90", ".o")

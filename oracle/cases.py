@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from cumc3d_amd.config import Config, briowu_1d  # noqa: E402
+from cumc3d_amd.config import Config, briowu_1d, torus_sph  # noqa: E402
 
 
 @dataclasses.dataclass
@@ -80,6 +80,25 @@ CASES = {c.name: c for c in [
                                    riemann="hllc", recon="tvdmc", model="torus",
                                    boundary_flag=(3, 1, 0, 0, 4, 1)),
          "torus", 8),
+    # spherical (r, theta, phi) with FIXPOLE: the torus-sph hooks (gravity,
+    # floors, no-inflow clamp) on the full polar range with axis boundaries,
+    # in 3D and axisymmetric 2D (one phi cell: the polar EMF average then
+    # rounds it to single precision, mhd_module.f90:293,443-449).  WENO-Z
+    # with HLLC or HLLD turns the polar cells to NaN within two steps in the
+    # reference itself on these grids, so WENO-Z is paired with HLL here.
+    Case("sph_torus_ppm", torus_sph(20, 16, 8, recon="ppm"), "torus_sph", 10),
+    Case("sph2d_torus_tvdmc", torus_sph(24, 24, recon="tvdmc", n_dim=2), "torus_sph", 20),
+    # magnetised blast next to the polar axis (plain hooks), field threading the pole
+    Case("sph_blast_weno", Config(nx=16, ny=16, nz=12, n_dim=3, coordinate_flag=2,
+                                  x_start=0.4, x_end=2.0, y_start=0.0, y_end=math.pi,
+                                  z_start=0.0, z_end=2.0 * math.pi, cfl=0.4, ggas=5.0 / 3.0,
+                                  riemann="hll", recon="weno", boundary_flag=(1, 1, 3, 3, 0, 0)),
+         "sph_blast", 10),
+    Case("sph_blast_100", Config(nx=16, ny=16, nz=12, n_dim=3, coordinate_flag=2,
+                                 x_start=0.4, x_end=2.0, y_start=0.0, y_end=math.pi,
+                                 z_start=0.0, z_end=2.0 * math.pi, cfl=0.4, ggas=5.0 / 3.0,
+                                 riemann="hllc", recon="ppm", boundary_flag=(1, 1, 3, 3, 0, 0)),
+         "sph_blast", 100),
 ]}
 
 

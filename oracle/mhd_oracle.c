@@ -12,9 +12,13 @@
  */
 #include "mhd_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* parameter.h:24 small_num = TINY(1.0D0) */
+#define SMALL_NUM DBL_MIN
 
 enum { IRHO = 1, IVX, IVY, IVZ, ITAU, IBX, IBY, IBZ, IEX, IEY, IEZ };  /* setup.f90:198-255 */
 enum { IYX = 1, IZX, IXY, IZY, IXZ, IYZ };                           /* mhd_module.f90:19-24 */
@@ -133,6 +137,13 @@ oracle *oracle_create(const oracle_desc *d)
     }
     for (int k = -2; k <= o->ny + 3; ++k) A1(o->sine, k) = sin(A1(o->y, k));
     for (int k = -3; k <= o->ny + 3; ++k) A1(o->sinf, k) = sin(A1(o->yF, k));
+    /* FIXPOLE, getgrid.f90:91-98: after dy, y and the sines are taken */
+    if (d->coordinate_flag == 2 && d->fixpole) {
+        A1(o->yF, 0) = 0.0;
+        A1(o->yF, o->ny) = 4.0 * atan(1.0);
+        A1(o->sinf, 0) = 0.0;
+        A1(o->sinf, o->ny) = 0.0;
+    }
 
     /* reconstruction weights */
     o->wx = (double *)calloc((o->nx + 7) * 14, sizeof(double));
@@ -696,6 +707,12 @@ static void riemann(oracle *o, int dir)
                     qL[i] = o->primL[i][c]; qR[i] = o->primR[i][c];
                     uL[i] = o->uL[i][c]; uR[i] = o->uR[i][c];
                     fL[i] = o->fluxL[i][c]; fR[i] = o->fluxR[i][c];
+                    /* HLLNM's IF/ELSEIF chain (rsolver_module.f90:303-311) has no
+                     * ELSE: with a NaN signal speed no branch is taken and
+                     * flux(:,j,k,l) keeps what the shared array held, i.e. the
+                     * last sweep (of any direction, stage or step) that wrote
+                     * this index */
+                    fl[i] = o->flux[i][c];
                 }
                 riemann_core(o->d.riemann, &p, o->d.hlld_bn_mode, o->uss_keep, qL, qR, o->csL[c], o->csR[c],
                              uL, uR, fL, fR, fl);
@@ -707,7 +724,7 @@ static void riemann(oracle *o, int dir)
 /* sources, divergence, EMF                                                 */
 /* ------------------------------------------------------------------------ */
 
-/* GET_SOURCE, spatial.f90:195-256 (Cartesian and cylindrical) */
+/* GET_SOURCE, spatial.f90:195-256 */
 static void get_source(oracle *o)
 {
     for (int i = 1; i <= 8; ++i)
@@ -725,11 +742,29 @@ static void get_source(oracle *o)
                     o->sc[IVX][c] = o->sc[IVX][c] + (o->prim[ITAU][c] + rho * vy * vy + 0.5 * bsq - by * by) / (A1(o->x, j));
                     o->sc[IVY][c] = o->sc[IVY][c] - (rho * vx * vy - bx * by) / (A1(o->x, j));
                 }
+    } else if (o->d.coordinate_flag == 2) {
+        for (int l = 1; l <= o->nz; ++l)
+            for (int k = 1; k <= o->ny; ++k)
+                for (int j = 1; j <= o->nx; ++j) {
+                    long c = CI(o, j, k, l);
+                    double bx = o->bcell[IBX][c], by = o->bcell[IBY][c], bz = o->bcell[IBZ][c];
+                    double bsq = bx * bx + by * by + bz * bz;
+                    double rho = o->prim[IRHO][c], vx = o->prim[IVX][c], vy = o->prim[IVY][c], vz = o->prim[IVZ][c];
+                    double p = o->prim[ITAU][c];
+                    double g = 3.0 * A1(o->x, j) * A1(o->dx, j) / A1(o->dx_cb, j);
+                    double cot = A1(o->dsine, k) / A1(o->dcose, k);
+                    o->sc[IVX][c] = o->sc[IVX][c] + (2.0 * p + bx * bx + rho * (vy * vy + vz * vz)) * g;
+                    o->sc[IVY][c] = o->sc[IVY][c] + (p + rho * vz * vz + 0.5 * bsq - bz * bz) * g * cot
+                                  - (rho * vx * vy - bx * by) * g;
+                    o->sc[IVZ][c] = o->sc[IVZ][c] - (rho * vx * vz - bx * bz) * g
+                                  - (rho * vy * vz - by * bz) * g * cot;
+                }
     }
 }
 
-/* CUSTOM_SOURCE: model/riemann/src/custom.f90:143-155 (RT gravity) and
- * model/torus-cyn/src/custom.f90:297-313 (point-mass gravity with floor mask) */
+/* CUSTOM_SOURCE: model/riemann/src/custom.f90:143-155 (RT gravity),
+ * model/torus-cyn/src/custom.f90:297-313 and model/torus-sph/src/custom.f90:283-296
+ * (Paczynski-Wiita gravity with floor mask) */
 static void custom_source(oracle *o)
 {
     if (o->d.model == 0) {
@@ -743,6 +778,21 @@ static void custom_source(oracle *o)
         return;
     }
     double rsh = o->d.r_sh;
+    if (o->d.model == 2) {
+        for (int l = 1; l <= o->nz; ++l)
+            for (int k = 1; k <= o->ny; ++k)
+                for (int j = 1; j <= o->nx; ++j) {
+                    long c = CI(o, j, k, l);
+                    double diff = o->prim[IRHO][c] - o->rho_floor[c];
+                    double factor = fmax2(fsign(1.0, diff), 0.0);
+                    double xj = A1(o->x, j);
+                    double dphidr = 1.0 / ((xj - rsh) * (xj - rsh));
+                    double rho = o->prim[IRHO][c];
+                    o->sc[IVX][c] = o->sc[IVX][c] + (-factor * rho * dphidr);
+                    o->sc[ITAU][c] = o->sc[ITAU][c] + (-factor * rho * o->prim[IVX][c] * dphidr);
+                }
+        return;
+    }
     for (int l = 1; l <= o->nz; ++l)
         for (int k = 1; k <= o->ny; ++k)
             for (int j = 1; j <= o->nx; ++j) {
@@ -760,10 +810,10 @@ static void custom_source(oracle *o)
             }
 }
 
-/* FLUX_DIFF, spatial.f90:436-583 (Cartesian and cylindrical) */
+/* FLUX_DIFF, spatial.f90:436-583 */
 static void flux_diff(oracle *o, int dir)
 {
-    int cyl = o->d.coordinate_flag == 1;
+    int cyl = o->d.coordinate_flag == 1, sph = o->d.coordinate_flag == 2;
     long sy = o->NX, sz = (long)o->NX * o->NY;
     for (int l = 1; l <= o->nz; ++l)
         for (int k = 1; k <= o->ny; ++k)
@@ -774,14 +824,21 @@ static void flux_diff(oracle *o, int dir)
                     double dflux;
                     if (dir == XDIR) {
                         if (cyl) dflux = (A1(o->xF, j) * f[c] - A1(o->xF, j - 1) * f[c - 1]) / (A1(o->x, j) * A1(o->dx, j));
-                        else dflux = (f[c] - f[c - 1]) / (A1(o->dx, j));
+                        else if (sph) {
+                            double a = A1(o->xF, j), b = A1(o->xF, j - 1);
+                            dflux = (a * a * f[c] - b * b * f[c - 1]) / (A1(o->dx_cb, j) / 3.0);
+                        } else dflux = (f[c] - f[c - 1]) / (A1(o->dx, j));
                         o->l_rk[i][c] = -dflux;
                     } else if (dir == YDIR) {
                         if (cyl) dflux = (f[c] - f[c - sy]) / (A1(o->x, j) * A1(o->dy, k));
+                        else if (sph) dflux = (A1(o->sinf, k) * f[c] - A1(o->sinf, k - 1) * f[c - sy]) /
+                                              (A1(o->xbar, j) * A1(o->dcose, k));
                         else dflux = (f[c] - f[c - sy]) / (A1(o->dy, k));
                         o->l_rk[i][c] = o->l_rk[i][c] - dflux;
                     } else {
-                        dflux = (f[c] - f[c - sz]) / A1(o->dz, l);
+                        if (sph) dflux = (f[c] - f[c - sz]) * A1(o->dy, k) /
+                                         (A1(o->xbar, j) * A1(o->dz, l) * A1(o->dcose, k));
+                        else dflux = (f[c] - f[c - sz]) / A1(o->dz, l);
                         o->l_rk[i][c] = o->l_rk[i][c] - dflux;
                     }
                 }
@@ -879,7 +936,7 @@ static void boundary_efield(oracle *o)
     }
 }
 
-/* flux_ct, mhd_module.f90:285-473 (Cartesian and cylindrical; no FIXPOLE) */
+/* flux_ct, mhd_module.f90:285-473 */
 static void flux_ct(oracle *o)
 {
     double **E = o->ecell, **F = o->eface;
@@ -928,9 +985,30 @@ static void flux_ct(oracle *o)
                 o->efz[c] = 0.50 * (F[IZX][c] + F[IZX][c + sy] + F[IZY][c] + F[IZY][c + 1])
                           - 0.25 * (E[IEZ][c] + E[IEZ][c + sy] + E[IEZ][c + 1] + E[IEZ][c + 1 + sy]);
             }
+    /* FIXPOLE, mhd_module.f90:437-461: the radial EMF on the polar faces is
+     * replaced by its phi average.  The accumulators sum1/sum2 are declared
+     * REAL (:293), so each addition is done in double and rounded to single;
+     * the order is the serial l loop of a one-thread run. */
+    if (o->d.coordinate_flag == 2 && o->d.fixpole) {
+        for (int j = 1; j <= o->nx; ++j) {
+            float sum1 = 0.0f, sum2 = 0.0f;
+            double dsum1 = 0.0, dsum2 = 0.0;   /* fixpole == 2: a planted mistake for tests */
+            for (int l = 1; l <= o->nz; ++l) {
+                sum1 = (float)((double)sum1 + o->efx[CI(o, j, 0, l)]);
+                sum2 = (float)((double)sum2 + o->efx[CI(o, j, o->ny, l)]);
+                dsum1 += o->efx[CI(o, j, 0, l)];
+                dsum2 += o->efx[CI(o, j, o->ny, l)];
+            }
+            double a1 = o->d.fixpole == 2 ? dsum1 : (double)sum1, a2 = o->d.fixpole == 2 ? dsum2 : (double)sum2;
+            for (int l = -2; l <= o->nz + 3; ++l) {
+                o->efx[CI(o, j, 0, l)] = a1 / (double)o->nz;
+                o->efx[CI(o, j, o->ny, l)] = a2 / (double)o->nz;
+            }
+        }
+    }
 }
 
-/* l_operator, spatial.f90:620-680 */
+/* l_operator, spatial.f90:620-702 */
 static void l_operator(oracle *o)
 {
     for (int l = 1; l <= o->nz; ++l)
@@ -951,6 +1029,14 @@ static void l_operator(oracle *o)
                     o->l_rk[IBX][c] = -((ez[c] - ez[c - sy]) / (dyk) - (ey[c] - ey[c - sz]) / (dzl));
                     o->l_rk[IBY][c] = -((ex[c] - ex[c - sz]) / (dzl) - (ez[c] - ez[c - 1]) / (dxj));
                     o->l_rk[IBZ][c] = -((ey[c] - ey[c - 1]) / (dxj) - (ex[c] - ex[c - sy]) / (dyk));
+                } else if (o->d.coordinate_flag == 2) {
+                    double xFj = A1(o->xF, j), xFm = A1(o->xF, j - 1), xj = A1(o->x, j), xb = A1(o->xbar, j);
+                    double sk = A1(o->sinf, k), skm = A1(o->sinf, k - 1), dc = A1(o->dcose, k);
+                    o->l_rk[IBX][c] = (-(sk * ez[c] - skm * ez[c - sy]) + (ey[c] - ey[c - sz]) * (dyk / dzl)) /
+                                      (xFj * dc + SMALL_NUM);
+                    o->l_rk[IBY][c] = (-(ex[c] - ex[c - sz]) / (xb * sk * dzl + SMALL_NUM) +
+                                       (xFj * ez[c] - xFm * ez[c - 1]) / (xj * dxj));
+                    o->l_rk[IBZ][c] = (-(xFj * ey[c] - xFm * ey[c - 1]) / (xj * dxj) + (ex[c] - ex[c - sy]) / (xb * dyk));
                 } else {
                     double xFj = A1(o->xF, j), xFm = A1(o->xF, j - 1), xj = A1(o->x, j);
                     o->l_rk[IBX][c] = -(ez[c] - ez[c - sy]) / (xFj * dyk) + (ey[c] - ey[c - sz]) / (dzl);
@@ -987,6 +1073,15 @@ static void bcell_from(oracle *o, double **B)
         for (int k = 1; k <= o->ny; ++k)
             for (int j = 1; j <= o->nx; ++j) {
                 long c = CI(o, j, k, l);
+                if (o->d.coordinate_flag == 2) {
+                    /* fromrvetou.f90:76-88 and :181-193 */
+                    double a = A1(o->xF, j), b = A1(o->xF, j - 1);
+                    o->bcell[IBX][c] = 1.5 * A1(o->dx, j) / A1(o->dx_cb, j) * (a * a * B[IBX][c] + b * b * B[IBX][c - 1]);
+                    o->bcell[IBY][c] = 0.5 * A1(o->dy, k) / A1(o->dcose, k) *
+                                       (A1(o->sinf, k) * B[IBY][c] + A1(o->sinf, k - 1) * B[IBY][c - sy]);
+                    o->bcell[IBZ][c] = 0.5 * (B[IBZ][c] + B[IBZ][c - sz]);
+                    continue;
+                }
                 if (o->d.coordinate_flag == 0)
                     o->bcell[IBX][c] = 0.5 * (B[IBX][c] + B[IBX][c - 1]);
                 else
@@ -1035,7 +1130,8 @@ static void fromutorve(oracle *o)
             }
 }
 
-/* CUSTOM_CHECKRHO: model/riemann/src/custom.f90:115-121, model/torus-cyn/src/custom.f90:254-271 */
+/* CUSTOM_CHECKRHO: model/riemann/src/custom.f90:115-121, model/torus-cyn/src/custom.f90:254-271
+ * (model/torus-sph/src/custom.f90:240-259 is the same) */
 static void checkrho(oracle *o)
 {
     for (int l = 1; l <= o->nz; ++l)
@@ -1109,7 +1205,8 @@ static void boundaryp(oracle *o)
                     o->prim[IBY][cd] = (flag >= 2) ? b[IBY] * o->prim[IBY][cs] : o->prim[IBY][cs];
                 }
     }
-    if (o->d.model == 1) {   /* CUSTOM_BOUNDARY_X / _Z, model/torus-cyn/src/custom.f90:146-201 */
+    if (o->d.model >= 1) {   /* CUSTOM_BOUNDARY_X, model/torus-cyn/src/custom.f90:146-163,
+                                model/torus-sph/src/custom.f90:150-174 */
         for (int l = -2; l <= nz + 3; ++l)
             for (int k = -2; k <= ny + 3; ++k)
                 for (int m = 1; m <= 3; ++m) {
@@ -1117,6 +1214,8 @@ static void boundaryp(oracle *o)
                     o->prim[IVX][ci] = fmin2(o->prim[IVX][ci], 0.0);
                     o->prim[IVX][co] = fmax2(o->prim[IVX][co], 0.0);
                 }
+    }
+    if (o->d.model == 1) {   /* CUSTOM_BOUNDARY_Z, model/torus-cyn/src/custom.f90:186-201 */
         for (int m = 1; m <= 3; ++m)
             for (int k = -2; k <= ny + 3; ++k)
                 for (int j = -2; j <= nx + 3; ++j) {
@@ -1222,9 +1321,12 @@ double oracle_finddt(oracle *o)
                 if (o->d.coordinate_flag == 0) {
                     if (o->d.n_dim > 1) h = fmin2(h, A1(o->dy, k));
                     if (o->d.n_dim > 2) h = fmin2(h, A1(o->dz, l));
-                } else {
+                } else if (o->d.coordinate_flag == 1) {
                     if (o->d.n_dim > 1 && o->ny > 1) h = fmin2(h, A1(o->x, j) * A1(o->dy, k));
                     if (o->d.n_dim > 2) h = fmin2(h, A1(o->dz, l));
+                } else {
+                    if (o->d.n_dim > 1) h = fmin2(h, A1(o->x, j) * A1(o->dy, k));
+                    if (o->d.n_dim > 2) h = fmin2(h, A1(o->x, j) * A1(o->sine, k) * A1(o->dz, l));
                 }
                 h = h * o->d.cfl / lambda;
                 dt_out = fmin2(dt_out, h);
@@ -1281,7 +1383,7 @@ int oracle_run(oracle *o, int max_steps, double total_time, double *t, double *d
     return n;
 }
 
-/* FIND_DIVB, mhd_module.f90:133-189 (Cartesian and cylindrical) */
+/* FIND_DIVB, mhd_module.f90:133-189 */
 double oracle_find_divb(oracle *o)
 {
     double m = 0.0;
@@ -1295,13 +1397,35 @@ double oracle_find_divb(oracle *o)
                 if (o->d.coordinate_flag == 0)
                     d = (bx[c] - bx[c - 1]) / (A1(o->dx, j)) + (by[c] - by[c - sy]) / (A1(o->dy, k)) +
                         (bz[c] - bz[c - sz]) / (A1(o->dz, l));
-                else
+                else if (o->d.coordinate_flag == 1)
                     d = (A1(o->xF, j) * bx[c] - A1(o->xF, j - 1) * bx[c - 1]) / (A1(o->x, j) * A1(o->dx, j)) +
                         (by[c] - by[c - sy]) / (A1(o->x, j) * A1(o->dy, k)) + (bz[c] - bz[c - sz]) / (A1(o->dz, l));
+                else {
+                    double a = A1(o->xF, j), b = A1(o->xF, j - 1), dcb = A1(o->dx_cb, j), dc = A1(o->dcose, k);
+                    double xdx = A1(o->x, j) * A1(o->dx, j);
+                    d = (a * a * bx[c] - b * b * bx[c - 1]) / (dcb / 3.0) +
+                        (A1(o->sinf, k) * by[c] - A1(o->sinf, k - 1) * by[c - sy]) * xdx / (dcb * dc / 3.0) +
+                        (bz[c] - bz[c - sz]) * (xdx * A1(o->dy, k)) / (dcb * dc * A1(o->dz, l) / 3.0);
+                }
                 m = fmax2(m, fabs(d));
             }
     o->maxdivb = m;
     return m;
+}
+
+int oracle_grid(const oracle *o, const char *name, double *out)
+{
+    const char *names[12] = {"dx", "dy", "dz", "x", "y", "z", "dx_cb", "xbar", "dcose", "dsine", "sine", "sinf"};
+    const double *arr[12] = {o->dx, o->dy, o->dz, o->x, o->y, o->z, o->dx_cb, o->xbar, o->dcose, o->dsine,
+                             o->sine, o->sinf};
+    const int len[12] = {o->nx + 7, o->ny + 7, o->nz + 7, o->nx + 7, o->ny + 7, o->nz + 7, o->nx + 7, o->nx + 7,
+                         o->ny + 7, o->ny + 7, o->ny + 7, o->ny + 7};
+    for (int i = 0; i < 12; ++i)
+        if (!strcmp(name, names[i])) {
+            memcpy(out, arr[i], (size_t)len[i] * sizeof(double));
+            return len[i];
+        }
+    return -1;
 }
 
 int oracle_get(oracle *o, const char *name, double *out)

@@ -24,12 +24,14 @@ typedef struct oracle_desc {
     int boundary_flag[6];
     int riemann;   /* 0 LF, 1 HLL, 2 HLLC, 3 HLLD            (parameter.h:91-101) */
     int recon;     /* 0 minmod, 1 van Leer, 2 MC, 3 PPM, 4 WENO (parameter.h:103-115) */
-    int model;     /* 0 riemann hooks, 1 torus-cyn hooks */
+    int model;     /* 0 riemann hooks, 1 torus-cyn hooks, 2 torus-sph hooks */
     int rt_gravity;
     int hlld_bn_mode; /* value of the unassigned usstar(ibn) in HLLDNM: 0 -> 0.0, 1 -> Bn,
                       2 -> carried from earlier double-star interfaces (serial reference) */
     double cfl, ggas, r_sh;
-    const double *xF, *yF, *zF;  /* face coordinates -3..n+3 (n+7 values) */
+    const double *xF, *yF, *zF;  /* face coordinates -3..n+3 (n+7 values), before the FIXPOLE override */
+    int fixpole;                 /* -DFIXPOLE: polar sinf = 0 and phi-averaged polar EMFs (spherical only);
+                                    2 = the same with a double accumulator, a planted mistake for tests */
 } oracle_desc;
 
 typedef struct oracle oracle;
@@ -45,6 +47,9 @@ void oracle_rungekutta(oracle *o, double dt);  /* rungekutta.f90:12-208 */
 /* main.f90:381-399 loop: returns steps taken; *t is advanced */
 int oracle_run(oracle *o, int max_steps, double total_time, double *t, double *dts);
 double oracle_find_divb(oracle *o);         /* mhd_module.f90:110-201 */
+/* GETGRID metric arrays, index j+3 for j = -3..n+3: "dx","dy","dz","x","y","z",
+ * "dx_cb","xbar","dcose","dsine","sine","sinf"; returns the length or -1. */
+int oracle_grid(const oracle *o, const char *name, double *out);
 /* Copy one named array (SoA, ghosted box) out: "rho","vx","vy","vz","p","bx","by",
  * "bz","eps","cs","bcx","bcy","bcz","u1".."u8","lrk1".."lrk8","flux1".."flux8". */
 int oracle_get(oracle *o, const char *name, double *out);

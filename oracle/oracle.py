@@ -27,7 +27,7 @@ class _Desc(C.Structure):
                 ("riemann", C.c_int), ("recon", C.c_int), ("model", C.c_int),
                 ("rt_gravity", C.c_int), ("hlld_bn_mode", C.c_int),
                 ("cfl", C.c_double), ("ggas", C.c_double), ("r_sh", C.c_double),
-                ("xF", PD), ("yF", PD), ("zF", PD)]
+                ("xF", PD), ("yF", PD), ("zF", PD), ("fixpole", C.c_int)]
 
 
 def build() -> None:
@@ -56,6 +56,7 @@ def lib():
         L.oracle_find_divb.argtypes = [C.c_void_p]
         L.oracle_find_divb.restype = C.c_double
         L.oracle_get.argtypes = [C.c_void_p, C.c_char_p, PD]
+        L.oracle_grid.argtypes = [C.c_void_p, C.c_char_p, PD]
         L.oracle_cells.argtypes = [C.c_void_p]
         L.oracle_cells.restype = C.c_long
         L.oracle_tvd.argtypes = [C.c_int, PD, C.c_double, C.c_double, C.c_double, PD, PD]
@@ -75,7 +76,8 @@ def _p(a: np.ndarray):
 class Oracle:
     """One CPU instance of the reference algorithm on a configuration."""
 
-    def __init__(self, cfg: Config, state: dict, hlld_bn_mode: int = 2, grid: Grid | None = None):
+    def __init__(self, cfg: Config, state: dict, hlld_bn_mode: int = 2, grid: Grid | None = None,
+                 fixpole: int | None = None):
         self.cfg = cfg
         self.grid = grid or Grid(cfg)
         L = lib()
@@ -92,6 +94,7 @@ class Oracle:
         d.hlld_bn_mode = hlld_bn_mode
         d.cfl, d.ggas, d.r_sh = cfg.cfl, cfg.ggas, cfg.r_sh
         d.xF, d.yF, d.zF = (_p(f) for f in self._faces)
+        d.fixpole = int(cfg.fixpole) if fixpole is None else fixpole
         self._h = L.oracle_create(C.byref(d))
         if not self._h:
             raise ValueError("invalid boundary flag combination")
@@ -129,6 +132,14 @@ class Oracle:
 
     def divb(self) -> float:
         return lib().oracle_find_divb(self._h)
+
+    def grid_metric(self, name: str) -> np.ndarray:
+        """GETGRID array `name` over -3..n+3 (index j+3)."""
+        out = np.zeros(max(self.cfg.nx, self.cfg.ny, self.cfg.nz) + 7)
+        n = lib().oracle_grid(self._h, name.encode(), _p(out))
+        if n < 0:
+            raise KeyError(name)
+        return out[:n]
 
     def get(self, name: str) -> np.ndarray:
         out = np.empty(self.cfg.shape)

@@ -7,6 +7,9 @@
 ! cumc3d_amd/fortran/rungekutta_hip.f90 instead of the reference's and
 ! refreshes the host state from the device before FIND_DIVB and the dump.
 !
+! It also writes OUT_FILE.grid, the GETGRID arrays the step reads
+! (getgrid.f90:26-98, after CUSTOM_GRID and the FIXPOLE override).
+!
 ! usage: cumc3d_ref IC_FILE OUT_FILE MAX_STEPS [THREADS]
 PROGRAM REF_HARNESS
 USE DEFINITION
@@ -38,6 +41,14 @@ n_step = 0
 global_time = 0.0D0
 CALL INITIAL_MODEL
 CALL INITIAL_UPDATE
+
+OPEN(NEWUNIT=u, FILE=TRIM(out_path)//'.grid', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE', &
+     ACTION='WRITE')
+WRITE(u) nx, ny, nz
+WRITE(u) xF(-3:nx+3), yF(-3:ny+3), zF(-3:nz+3)
+WRITE(u) dx(-2:nx+3), dy(-2:ny+3), dz(-2:nz+3), x(-2:nx+3), y(-2:ny+3), z(-2:nz+3)
+WRITE(u) dx_cb(-1:nx+3), xbar(-1:nx+3), dcose(-1:ny+3), dsine(-1:ny+3), sine(-2:ny+3), sinf(-3:ny+3)
+CLOSE(u)
 
 ALLOCATE(dts(MAX(max_steps,1)))
 cnt = 0

@@ -1,5 +1,6 @@
-! File-fed initial model for the torus-cyn hooks (test infrastructure only).
-! Replaces model/torus-cyn/src/model.f90:6-315 for the oracle build: prim,
+! File-fed initial model for the torus-cyn and torus-sph hooks (test
+! infrastructure only).  Replaces model/torus-cyn/src/model.f90:6-315 and
+! model/torus-sph/src/model.f90:6-324 for the oracle build: prim,
 ! epsilon, ggas and the floor arrays of module CUSTOM_DEF
 ! (model/torus-cyn/src/def.f90:6-14) are read from a binary file written by
 ! cumc3d_amd.ic.

@@ -44,6 +44,28 @@ def read_dump(path: str) -> dict:
     return out
 
 
+GRID_ARRAYS = (("xF", -3, 0), ("yF", -3, 1), ("zF", -3, 2),
+               ("dx", -2, 0), ("dy", -2, 1), ("dz", -2, 2), ("x", -2, 0), ("y", -2, 1), ("z", -2, 2),
+               ("dx_cb", -1, 0), ("xbar", -1, 0), ("dcose", -1, 1), ("dsine", -1, 1), ("sine", -2, 1),
+               ("sinf", -3, 1))
+
+
+def read_grid(path: str) -> dict:
+    """The GETGRID arrays written by the harness (OUT_FILE.grid), each
+    returned over -3..n+3 at index j+3 with NaN where the reference's range
+    does not reach."""
+    with open(path, "rb") as f:
+        n3 = [int(v) for v in np.fromfile(f, dtype=np.int32, count=3)]
+        out = {}
+        for name, lo, ax in GRID_ARRAYS:
+            n = n3[ax]
+            vals = np.fromfile(f, dtype=np.float64, count=n + 3 - lo + 1)
+            a = np.full(n + 7, np.nan)
+            a[lo + 3:] = vals
+            out[name] = a
+    return out
+
+
 def dropin_binary(case: str) -> str:
     """The reference program stepping through the Fortran drop-in on the GPU."""
     return os.path.join(HERE, "_ref", "dropin", case, "cumc3d_dropin")
@@ -61,4 +83,7 @@ def run_ref(case: str, ic_path: str, steps: int, threads: int = 1, timeout: floa
                            capture_output=True, text=True, timeout=timeout, env=env)
         if r.returncode != 0:
             raise RuntimeError(f"reference oracle failed: {r.stderr[-2000:]}")
-        return read_dump(out)
+        res = read_dump(out)
+        if os.path.exists(out + ".grid"):
+            res["grid"] = read_grid(out + ".grid")
+        return res

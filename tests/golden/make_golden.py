@@ -5,7 +5,8 @@ Runs oracle/_ref/<case>/cumc3d_ref (the reference Fortran compiled from its own
 sources by oracle/build_ref.py) single-threaded with zero-filled allocations
 (oracle/refio.run_ref) on each case of oracle/cases.py and stores, per case,
 the dt sequence and the final state on cells 0..n+1 of every axis:
-rho, vx, vy, vz, p, bx, by, bz (face), eps, bcx, bcy, bcz.
+rho, vx, vy, vz, p, bx, by, bz (face), eps, bcx, bcy, bcz; for spherical
+cases also the GETGRID metric arrays the step reads (grid_*).
 
 usage: python tests/golden/make_golden.py [case ...]
 """
@@ -50,6 +51,10 @@ def main(names):
         out["dts"] = ref["dts"]
         out["t"] = np.array(ref["t"])
         out["maxdivb"] = np.array(ref["maxdivb"])
+        if case.cfg.coordinate_flag == 2:
+            # the reference's GETGRID arrays (after FIXPOLE), -3..n+3 at j+3
+            for k, v in ref["grid"].items():
+                out["grid_" + k] = v
         path = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(path, **out)
         print(f"{name}: {ref['nstep']} steps -> {path} ({os.path.getsize(path) // 1024} KB)")

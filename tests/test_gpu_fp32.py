@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from cumc3d_amd import ic
-from cumc3d_amd.config import blast_3d, orszag_tang_3d, torus_cyl
+from cumc3d_amd.config import blast_3d, orszag_tang_3d, torus_cyl, torus_sph
 from cumc3d_amd.solver import make_solver
 
 pytestmark = pytest.mark.gpu
@@ -126,3 +126,15 @@ def test_fp32_deterministic():
     assert np.array_equal(da, db)
     for f in ("rho", "p", "bx"):
         assert np.array_equal(a.state()[f], b.state()[f])
+
+
+def test_fp32_spherical_torus_runs_close():
+    """Spherical geometry with FIXPOLE in the FP32 split (the polar EMF
+    average accumulates in single precision in both modes)."""
+    cfg = torus_sph(32, 24, 16, recon="ppm")
+    print("case spherical torus 32x24x16 fp32 vs fp64", flush=True)
+    a, b, _ = run_pair(cfg, "torus_sph", 4)
+    l1, linf = rel_errors(a, b, "rho")
+    print(f"  rho: rel L1 {l1:.2e}  rel Linf {linf:.2e}", flush=True)
+    assert abs_l1(a, b, "rho") <= L1_ABS_TOL
+    assert l1 <= L1_TOL

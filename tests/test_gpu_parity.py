@@ -6,7 +6,7 @@ import pytest
 
 from conftest import interior_window, load_golden
 from cumc3d_amd import ic
-from cumc3d_amd.config import Config, orszag_tang_3d, torus_cyl
+from cumc3d_amd.config import Config, orszag_tang_3d, torus_cyl, torus_sph
 from cumc3d_amd.solver import make_solver
 from oracle import oracle as O
 from oracle.cases import CASES
@@ -42,7 +42,8 @@ def test_gpu_matches_reference_fixture(name):
 
 @pytest.mark.parametrize("threads", [1, 37])
 @pytest.mark.parametrize("name", ["ot_hlld_weno", "ot_hlld_tvdmc", "ot_hllc_ppm", "ot_bc_mixed",
-                                  "blast_weno", "ot2d_hlld_ppm", "torus_hlld_ppm", "torus_sym_tvdmc"])
+                                  "blast_weno", "ot2d_hlld_ppm", "torus_hlld_ppm", "torus_sym_tvdmc",
+                                  "sph_torus_ppm", "sph_blast_weno"])
 def test_gpu_long_march_segments(name, threads, monkeypatch):
     """The y/z sweeps with long segments: CUMC3D_MARCH_THREADS=1 gives one
     segment per column (every thread marches the whole column), 37 an uneven
@@ -63,7 +64,9 @@ def test_gpu_long_march_segments(name, threads, monkeypatch):
     (orszag_tang_3d(32, recon="weno"), "orszag_tang", 4),
     (orszag_tang_3d(24, recon="ppm").with_(nx=40, boundary_flag=(1, 2, 0, 0, 2, 1)), "orszag_tang", 4),
     (torus_cyl(32, 16, 32, recon="ppm"), "torus", 4),
-], ids=["ot32_weno", "ot_aniso_ppm", "torus_ppm"])
+    (torus_sph(40, 32, 16, recon="weno", riemann="hll"), "torus_sph", 4),
+    (CASES["sph_blast_weno"].cfg.with_(nx=40, ny=24, nz=20), "sph_blast", 6),
+], ids=["ot32_weno", "ot_aniso_ppm", "torus_ppm", "sph_torus_weno_hll", "sph_blast_hll"])
 def test_gpu_matches_oracle_larger(cfg, ic_name, steps):
     """Bigger and anisotropic grids (several waves per row, uneven x tiling) against the C oracle."""
     print(f"case {cfg.nx}x{cfg.ny}x{cfg.nz} {cfg.recon}/{cfg.riemann}", flush=True)

@@ -49,8 +49,10 @@ def make_desc(**kw):
 
 
 @pytest.mark.parametrize("field,value", [
-    ("nx", 0), ("n_dim", 0), ("n_dim", 4), ("ny", 2), ("coordinate_flag", 2), ("riemann", 4),
-    ("recon", 5), ("model", 2), ("hlld_bn_mode", 3), ("precision", 7),
+    ("nx", 0), ("n_dim", 0), ("n_dim", 4), ("ny", 2), ("coordinate_flag", 3), ("riemann", 4),
+    ("recon", 5), ("model", 3), ("hlld_bn_mode", 3), ("precision", 7),
+    # each torus model's hooks belong to its geometry (model/torus-cyn, model/torus-sph)
+    ("model", 1), ("model", 2),
 ])
 def test_init_rejects_invalid_descriptor(field, value):
     L = solver.load_library()
@@ -87,4 +89,9 @@ def test_config_validation():
     with pytest.raises(ValueError):
         Config(nx=8, riemann="roe")
     with pytest.raises(ValueError):
-        Config(nx=8, coordinate_flag=2)
+        Config(nx=8, coordinate_flag=3)
+    with pytest.raises(ValueError):
+        Config(nx=8, model="torus_sph")
+    with pytest.raises(ValueError):
+        Config(nx=8, coordinate_flag=2, model="torus")
+    Config(nx=8, coordinate_flag=2, model="torus_sph")

@@ -286,3 +286,65 @@ def test_divb_zero_away_from_frozen_faces():
     bmag = np.abs(bx).max()
     assert np.abs(div).max() * h / bmag < 1e-12
     assert o.divb() > 1e-3    # the frozen lower faces break it at cells 1
+
+
+SPHERICAL = [n for n in CASES if CASES[n].cfg.coordinate_flag == 2]
+GRID_NAMES = ("dx", "dy", "dz", "x", "y", "z", "dx_cb", "xbar", "dcose", "dsine", "sine", "sinf")
+
+
+@pytest.mark.parametrize("name", SPHERICAL)
+def test_oracle_spherical_metrics_match_reference(name):
+    """The GETGRID arrays the step reads (getgrid.f90:43-98: the cube
+    differences, libm cos/sin of the faces, xbar, and the FIXPOLE override of
+    sinf at both poles) equal the reference's own, as its harness dumped them."""
+    gold = load_golden(name)
+    o = O.Oracle(CASES[name].cfg, ic.GENERATORS[CASES[name].ic](CASES[name].cfg))
+    for g in GRID_NAMES:
+        ref = gold["grid_" + g]
+        m = np.isfinite(ref)
+        assert np.array_equal(o.grid_metric(g)[m], ref[m]), g
+    ny = CASES[name].cfg.ny
+    assert gold["grid_sinf"][3] == 0.0 and gold["grid_sinf"][ny + 3] == 0.0
+
+
+@pytest.mark.parametrize("fixpole", [0, 2])
+def test_fixpole_average_is_pinned_by_the_fixture(fixpole):
+    """The polar EMF average of FIXPOLE accumulates in single precision
+    (REAL sum1/sum2, mhd_module.f90:293).  Planting either mistake in the
+    oracle -- no average (fixpole 0) or a double accumulator (fixpole 2) --
+    breaks bit parity with the reference on the blast whose field threads the
+    pole, so the fixture pins this detail."""
+    name = "sph_blast_weno"
+    case = CASES[name]
+    gold = load_golden(name)
+    o = O.Oracle(case.cfg, ic.GENERATORS[case.ic](case.cfg), fixpole=fixpole)
+    o.run(case.steps)
+    w = interior_window(case.cfg)
+    assert not all(np.array_equal(o.get(f)[w], gold[f]) for f in ("rho", "bx", "by", "bz"))
+
+
+def test_spherical_divb_zero_away_from_frozen_faces():
+    """Spherical CT (l_operator, spatial.f90:681-702) keeps FIND_DIVB's
+    spherical divergence (mhd_module.f90:171-186) at round-off in every cell
+    whose faces all evolve: away from the frozen lowest r and phi faces; the
+    frozen polar face has zero area (sinf = 0).  Computed here from the face
+    fields with the reference's divergence formula in numpy."""
+    case = CASES["sph_blast_100"]
+    cfg = case.cfg
+    o = O.Oracle(cfg, ic.GENERATORS[case.ic](cfg))
+    o.run(40)
+    bx, by, bz = o.get("bx"), o.get("by"), o.get("bz")
+    G = {g: o.grid_metric(g) for g in GRID_NAMES}
+    xF = o.grid.x.faces
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    J = np.arange(2, nx + 1)[None, None, :]
+    K = np.arange(1, ny + 1)[None, :, None]
+    L = np.arange(2, nz + 1)[:, None, None]
+    c = lambda a: a[L + 2, K + 2, J + 2]
+    cm = lambda a, dj, dk, dl: a[L + 2 - dl, K + 2 - dk, J + 2 - dj]
+    dcb, dc, xdx = G["dx_cb"][J + 3], G["dcose"][K + 3], G["x"][J + 3] * G["dx"][J + 3]
+    div = ((xF[J + 3] ** 2 * c(bx) - xF[J + 2] ** 2 * cm(bx, 1, 0, 0)) / (dcb / 3.0)
+           + (G["sinf"][K + 3] * c(by) - G["sinf"][K + 2] * cm(by, 0, 1, 0)) * xdx / (dcb * dc / 3.0)
+           + (c(bz) - cm(bz, 0, 0, 1)) * (xdx * G["dy"][K + 3]) / (dcb * dc * G["dz"][L + 3] / 3.0))
+    assert np.abs(div).max() < 1e-10 * np.abs(bx).max()
+    assert o.divb() > 1e-3    # the frozen lower faces break it at cells 1
