@@ -93,7 +93,7 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out);
 
 /* POPULATE_DEVICE (openacc.f90:5-24): copy the initial model (GET_MODEL
  * output: prim and epsilon, plus rho/p/eps floor arrays for the torus
- * model, NULL otherwise) to the device. */
+ * models, NULL otherwise) to the device. */
 int cumc3d_upload(cumc3d_ctx *ctx, const double *prim, const double *eps, const double *floors3);
 
 /* INITIAL_UPDATE (initial.f90:86-116): floors, cons, BOUNDARY, UPDATE(0). */
