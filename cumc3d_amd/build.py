@@ -1,4 +1,5 @@
-"""Build libcumc3d_hip.so in-tree with hipcc for gfx950.
+"""Build libcumc3d_hip.so in-tree with hipcc for gfx950, and libcumc3d_io.so
+(the HDF5 profile writer, host C) with gcc against the HDF5 C library.
 
 usage: python -m cumc3d_amd.build
 """
@@ -24,6 +25,10 @@ SOURCES = [(os.path.join(HERE, "csrc", "capi.hip"), []),
 DEPS = [os.path.join(HERE, "csrc", f) for f in ("kernels.hpp", "physics.hpp", "selector.hpp", "context.hpp")] + \
        [os.path.join(ROOT, "include", "cumc3d_hip.h")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+IO_LIB = os.path.join(HERE, "libcumc3d_io.so")
+IO_SRC = os.path.join(HERE, "csrc", "io.c")
+# the HDF5 C library of this image (1.10): headers and libhdf5.so
+HDF5_ROOT = os.environ.get("CUMC3D_HDF5", "/opt/conda")
 ARCH = os.environ.get("CUMC3D_ARCH", "gfx950")
 
 # -ffp-contract=off: no fused multiply-add, so the FP64 mode reproduces the
@@ -39,12 +44,28 @@ def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed: " + " ".join(cmd[-3:]))
+        raise RuntimeError(f"{os.path.basename(cmd[0])} failed: " + " ".join(cmd[-3:]))
+
+
+def build_io(verbose: bool = True) -> str | None:
+    """libcumc3d_io.so; skipped (None) where no HDF5 C library is installed."""
+    inc, lib = os.path.join(HDF5_ROOT, "include"), os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        if verbose:
+            print(f"no HDF5 headers under {inc}: libcumc3d_io.so not built")
+        return None
+    _run(["gcc", "-O2", "-std=c11", "-Wall", "-fPIC", "-shared", f"-I{inc}", IO_SRC, "-o", IO_LIB + ".tmp",
+          f"-L{lib}", "-lhdf5", f"-Wl,-rpath,{lib}", "-lm"])
+    os.replace(IO_LIB + ".tmp", IO_LIB)
+    if verbose:
+        print(f"built {IO_LIB}")
+    return IO_LIB
 
 
 def build(verbose: bool = True) -> str:
     from concurrent.futures import ThreadPoolExecutor
     t0 = time.time()
+    build_io(verbose)
     objs = [os.path.join(HERE, "csrc", os.path.basename(src) + ".o") for src, _ in SOURCES]
     with ThreadPoolExecutor(len(SOURCES)) as ex:
         list(ex.map(_run, [[HIPCC, *FLAGS, *extra, "-c", src, "-o", obj]

@@ -576,6 +576,29 @@ int cumc3d_download(cumc3d_ctx *x, double *prim, double *eps, double *bcell, dou
     return 0;
 }
 
+int cumc3d_profile(cumc3d_ctx *x, double *primitive, double *epsilon, double *bfield) {
+    if (!x) return fail(CUMC3D_EINVAL, "null context");
+    if (!x->uploaded) return fail(CUMC3D_ESTATE, "upload an initial model first");
+    const size_t np = (size_t)(x->nx + 2) * (x->ny + 2) * (x->nz + 2);
+    const size_t nb = (size_t)(x->nx + 3) * (x->ny + 3) * (x->nz + 3);
+    double *tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, (5 * np + np + 3 * nb) * sizeof(double)));
+    double *dp = primitive ? tmp : nullptr, *de = epsilon ? tmp + 5 * np : nullptr,
+           *db = bfield ? tmp + 6 * np : nullptr;
+    if (x->fp32)
+        hipLaunchKernelGGL(k_pack_profile<float>, dim3(nblk((long)nb)), dim3(256), 0, x->stream, make_args<float>(x),
+                           dp, de, db);
+    else
+        hipLaunchKernelGGL(k_pack_profile<double>, dim3(nblk((long)nb)), dim3(256), 0, x->stream,
+                           make_args<double>(x), dp, de, db);
+    if (primitive) HIPCHK(hipMemcpyAsync(primitive, dp, 5 * np * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    if (epsilon) HIPCHK(hipMemcpyAsync(epsilon, de, np * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    if (bfield) HIPCHK(hipMemcpyAsync(bfield, db, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipFree(tmp));
+    return 0;
+}
+
 int cumc3d_divb(cumc3d_ctx *x, double *maxdivb) {
     if (!x || !maxdivb) return fail(CUMC3D_EINVAL, "null argument");
     unsigned long long *d = nullptr, h = 0;

@@ -87,6 +87,26 @@ template <class W> __global__ void k_join_prim(double *aos, ArgsT<W> a) {
     for (int d = 0; d < 3; ++d) q[5 + d] = a.Bin[d][c];
 }
 
+// print_hydroprofile's three arrays (dataio.f90:178-233) in its order, Fortran
+// column-major: primitive(1:5, 0:nx+1, 0:ny+1, 0:nz+1), epsilon(0:nx+1, ...)
+// and bfield(1:3, -1:nx+1, -1:ny+1, -1:nz+1) (the face field).  One thread
+// per cell of the bfield box; any output pointer may be null.
+template <class W> __global__ void k_pack_profile(ArgsT<W> a, double *prim5, double *eps, double *bf3) {
+    const long bx = a.nx + 3, by = a.ny + 3;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= bx * by * (a.nz + 3)) return;
+    const int j = -1 + (int)(t % bx), k = -1 + (int)((t / bx) % by), l = -1 + (int)(t / (bx * by));
+    const long c = cidx(a, j, k, l);
+    if (bf3)
+        for (int d = 0; d < 3; ++d) bf3[3 * t + d] = a.Bin[d][c];
+    if (j >= 0 && k >= 0 && l >= 0) {
+        const long q = j + (long)(a.nx + 2) * (k + (long)(a.ny + 2) * l);
+        if (prim5)
+            for (int v = 0; v < 5; ++v) prim5[5 * q + v] = (double)a.P[v][c];
+        if (eps) eps[q] = (double)a.eps[c];
+    }
+}
+
 template <class W> __global__ void k_join3(double *aos, const W *f0, const W *f1, const W *f2, long G) {
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= G) return;

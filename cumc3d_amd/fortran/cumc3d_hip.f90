@@ -76,6 +76,11 @@ INTERFACE
     TYPE(C_PTR), VALUE :: ctx
     REAL(C_DOUBLE), INTENT(OUT), OPTIONAL :: prim(*), eps(*), bcell(*), cs(*)
   END FUNCTION
+  INTEGER(C_INT) FUNCTION cumc3d_profile(ctx, primitive, epsilon, bfield) BIND(C, NAME='cumc3d_profile')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR), VALUE :: ctx
+    REAL(C_DOUBLE), INTENT(OUT), OPTIONAL :: primitive(*), epsilon(*), bfield(*)
+  END FUNCTION
   INTEGER(C_INT) FUNCTION cumc3d_divb(ctx, maxdivb) BIND(C, NAME='cumc3d_divb')
     IMPORT :: C_INT, C_PTR, C_DOUBLE
     TYPE(C_PTR), VALUE :: ctx

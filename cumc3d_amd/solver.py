@@ -26,7 +26,7 @@ PD = C.POINTER(C.c_double)
 
 # symbols declared in include/cumc3d_hip.h
 EXPORTS = ("cumc3d_init", "cumc3d_upload", "cumc3d_initial_update", "cumc3d_finddt",
-           "cumc3d_rk3_step", "cumc3d_run", "cumc3d_download", "cumc3d_divb",
+           "cumc3d_rk3_step", "cumc3d_run", "cumc3d_download", "cumc3d_profile", "cumc3d_divb",
            "cumc3d_finalize", "cumc3d_error_string", "cumc3d_last_error",
            "cumc3d_set_timing", "cumc3d_get_timing",
            "cumc3d_selector_init", "cumc3d_selector_set_params", "cumc3d_selector_get_params",
@@ -70,6 +70,7 @@ def load_library(path: str = LIB_PATH):
     L.cumc3d_rk3_step.argtypes = [vp, C.c_double]
     L.cumc3d_run.argtypes = [vp, C.c_int, C.c_double, C.c_int, PD, PD, C.POINTER(C.c_int)]
     L.cumc3d_download.argtypes = [vp, PD, PD, PD, PD]
+    L.cumc3d_profile.argtypes = [vp, PD, PD, PD]
     L.cumc3d_divb.argtypes = [vp, PD]
     L.cumc3d_finalize.argtypes = [vp]
     L.cumc3d_error_string.argtypes = [C.c_int]
@@ -203,6 +204,20 @@ class Solver:
         out.update(eps=eps, cs=cs, bcx=np.ascontiguousarray(bcell[..., 0]),
                    bcy=np.ascontiguousarray(bcell[..., 1]), bcz=np.ascontiguousarray(bcell[..., 2]))
         return out
+
+    def profile(self) -> dict:
+        """print_hydroprofile's arrays (dataio.f90:178-233) packed on the
+        device: primitive (nz+2, ny+2, nx+2, 5), epsilon (nz+2, ny+2, nx+2)
+        and bfield (nz+3, ny+3, nx+3, 3) -- C-order views of the reference's
+        Fortran sections prim(1:5, 0:nx+1, ...), epsilon(0:nx+1, ...),
+        prim(ibx:ibz, -1:nx+1, ...)."""
+        cfg = self.cfg
+        nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+        prim = np.empty((nz + 2, ny + 2, nx + 2, 5))
+        eps = np.empty((nz + 2, ny + 2, nx + 2))
+        bf = np.empty((nz + 3, ny + 3, nx + 3, 3))
+        _check(load_library().cumc3d_profile(self._h, _p(prim), _p(eps), _p(bf)))
+        return {"primitive": prim, "epsilon": eps, "bfield": bf}
 
     # -- measurement -------------------------------------------------------
     def set_timing(self, on: bool):

@@ -117,6 +117,17 @@ int cumc3d_run(cumc3d_ctx *ctx, int max_steps, double total_time, int sync_each_
  * Any pointer may be NULL.  prim and bcell use the AoS layout above. */
 int cumc3d_download(cumc3d_ctx *ctx, double *prim, double *eps, double *bcell, double *cs);
 
+/* The three arrays print_hydroprofile writes (dataio.f90:178-233), packed on
+ * the device in its order (Fortran column-major) and copied to the host:
+ *   primitive  prim(1:5, 0:nx+1, 0:ny+1, 0:nz+1)   (rho, vx, vy, vz, p)
+ *   epsilon    epsilon(0:nx+1, 0:ny+1, 0:nz+1)
+ *   bfield     prim(ibx:ibz, -1:nx+1, -1:ny+1, -1:nz+1)   (face field)
+ * i.e. 5(nx+2)(ny+2)(nz+2), (nx+2)(ny+2)(nz+2) and 3(nx+3)(ny+3)(nz+3)
+ * doubles.  Any pointer may be NULL.  Replaces the host update and array
+ * sections of dataio.f90:26,220,196; libcumc3d_io writes the file
+ * (include/cumc3d_io.h). */
+int cumc3d_profile(cumc3d_ctx *ctx, double *primitive, double *epsilon, double *bfield);
+
 /* FIND_DIVB (mhd_module.f90:110-201): max |div B| over the interior. */
 int cumc3d_divb(cumc3d_ctx *ctx, double *maxdivb);
 

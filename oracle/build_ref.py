@@ -19,7 +19,8 @@ Outputs: oracle/_ref/<case>/cumc3d_ref.
 For the cases in DROPIN_CASES it also links oracle/_ref/dropin/<case>/cumc3d_dropin:
 the same reference program with solver/mhd/rungekutta.f90 replaced by the
 Fortran drop-in of this repository (cumc3d_amd/fortran/*.f90) over
-libcumc3d_hip.so, i.e. the reference's own driver stepping on the GPU.
+libcumc3d_hip.so, i.e. the reference's own driver stepping on the GPU, and
+writing its profiles through the drop-in print_hydroprofile (libcumc3d_io.so).
 
 usage: python oracle/build_ref.py [case ...]     (default: every case)
 """
@@ -43,7 +44,8 @@ SCRATCH = os.environ.get("CUMC3D_REF_SCRATCH", "/tmp/cumc3d_ref_build")
 OUT = os.path.join(HERE, "_ref")
 REPO = os.path.dirname(HERE)
 HIPLIB_DIR = os.path.join(REPO, "cumc3d_amd")
-DROPIN_SRC = [os.path.join(REPO, "cumc3d_amd", "fortran", f) for f in ("cumc3d_hip.f90", "rungekutta_hip.f90")]
+DROPIN_SRC = [os.path.join(REPO, "cumc3d_amd", "fortran", f)
+              for f in ("cumc3d_hip.f90", "rungekutta_hip.f90", "cumc3d_io.f90", "dataio_hip.f90")]
 DROPIN_CASES = ("c1_briowu", "ot_hlld_weno", "ot_bc_mixed", "torus_hlld_ppm", "ot2d_hlld_ppm",
                 "sph_torus_ppm", "sph_blast_weno")
 FFLAGS = ["-cpp", "-O2", "-ffp-contract=off", "-fopenmp", "-DDIVB", "-DFIXPOLE"]
@@ -170,7 +172,8 @@ def build_case(name: str) -> str:
     libdir = os.path.join(os.path.dirname(os.path.dirname(FLANG)), "lib")
     run([FLANG, "-fopenmp", "-o", os.path.join(outdir, "cumc3d_ref"), *objs,
          f"-Wl,-rpath,{libdir}"], work)
-    if name in DROPIN_CASES and os.path.exists(os.path.join(HIPLIB_DIR, "libcumc3d_hip.so")):
+    if name in DROPIN_CASES and all(os.path.exists(os.path.join(HIPLIB_DIR, f))
+                                    for f in ("libcumc3d_hip.so", "libcumc3d_io.so")):
         build_dropin(name, cfg, work, objs, libdir)
     return outdir
 
@@ -190,7 +193,7 @@ def build_dropin(name, cfg, work, objs, libdir):
     os.makedirs(outdir, exist_ok=True)
     # the library is found relative to the binary, so the tree can move
     run([FLANG, "-fopenmp", "-o", os.path.join(outdir, "cumc3d_dropin"), *dobjs,
-         f"-L{HIPLIB_DIR}", "-lcumc3d_hip", f"-Wl,-rpath,{libdir}",
+         f"-L{HIPLIB_DIR}", "-lcumc3d_hip", "-lcumc3d_io", f"-Wl,-rpath,{libdir}",
          "-Wl,-rpath,$ORIGIN/../../../../cumc3d_amd"], work)
 
 

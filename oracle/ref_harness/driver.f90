@@ -5,7 +5,10 @@
 !
 ! Built with -DHIPDROP it links the drop-in RUNGEKUTTA/FINDDT of
 ! cumc3d_amd/fortran/rungekutta_hip.f90 instead of the reference's and
-! refreshes the host state from the device before FIND_DIVB and the dump.
+! refreshes the host state from the device before FIND_DIVB and the dump; it
+! also writes ./outfile/rkiter-0-nm.hdf5 after INITIAL_UPDATE (from the host
+! arrays) and ./outfile/rkiter-1-nm.hdf5 after the last step (packed on the
+! device) through the print_hydroprofile of cumc3d_amd/fortran/dataio_hip.f90.
 !
 ! It also writes OUT_FILE.grid, the GETGRID arrays the step reads
 ! (getgrid.f90:26-98, after CUSTOM_GRID and the FIXPOLE override).
@@ -41,6 +44,10 @@ n_step = 0
 global_time = 0.0D0
 CALL INITIAL_MODEL
 CALL INITIAL_UPDATE
+#ifdef HIPDROP
+n_iter = 0
+CALL print_hydroprofile
+#endif
 
 OPEN(NEWUNIT=u, FILE=TRIM(out_path)//'.grid', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE', &
      ACTION='WRITE')
@@ -67,6 +74,8 @@ END DO
 CALL SYSTEM_CLOCK(c1)
 loop_seconds = DBLE(c1 - c0)/DBLE(crate)
 #ifdef HIPDROP
+n_iter = 1
+CALL print_hydroprofile
 CALL CUMC3D_HIP_SYNC_HOST
 #endif
 

@@ -29,3 +29,31 @@ def load_golden(name):
         pytest.skip(f"golden fixture {name} missing")
     with np.load(path) as d:
         return {k: d[k] for k in d.files}
+
+
+def check_profile(prof, cfg, state, t):
+    """A profile file (read back as a dict) holds print_hydroprofile's
+    datasets of `state` (ghosted-box arrays) at time t
+    (solver/mhd/dataio.f90:36-233): primitive = prim(1:5, 0:n+1, ...),
+    epsilon(0:n+1, ...), bfield = prim(ibx:ibz, -1:n+1, ...), the interfaces
+    xF(-1:n+1) (yF after FIXPOLE), stored with the Fortran dimensions reversed."""
+    from cumc3d_amd.grid import Grid
+    from cumc3d_amd.hydroprofile import written_faces
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    c = (slice(2, nz + 4), slice(2, ny + 4), slice(2, nx + 4))      # cells 0..n+1
+    f = (slice(1, nz + 4), slice(1, ny + 4), slice(1, nx + 4))      # faces -1..n+1
+    faces = written_faces(Grid(cfg))
+    assert prof["time"].tolist() == [t]
+    assert prof["dimension"].tolist() == [float(cfg.n_dim)]
+    assert prof["coordinate"].tolist() == [float(cfg.coordinate_flag + 1)]
+    for name, fa, n in (("x-interface", faces[0], nx), ("y-interface", faces[1], ny), ("z-interface", faces[2], nz)):
+        assert np.array_equal(prof[name], fa[2:n + 5]), name
+    prim = prof["primitive"]
+    assert prim.shape == (nz + 2, ny + 2, nx + 2, 5)
+    for v, name in enumerate(("rho", "vx", "vy", "vz", "p")):
+        assert np.array_equal(prim[..., v], state[name][c], equal_nan=True), name
+    assert np.array_equal(prof["epsilon"], state["eps"][c], equal_nan=True)
+    bf = prof["bfield"]
+    assert bf.shape == (nz + 3, ny + 3, nx + 3, 3)
+    for d, name in enumerate(("bx", "by", "bz")):
+        assert np.array_equal(bf[..., d], state[name][f], equal_nan=True), name

@@ -2,15 +2,21 @@
 hooks, its time loop) with RUNGEKUTTA/FINDDT replaced by
 cumc3d_amd/fortran/rungekutta_hip.f90 over the C-ABI, against the reference
 fixtures bit for bit.  Needs an MI355X and the prebuilt drop-in binaries
-(oracle/_ref/dropin, linked by oracle/build_ref.py)."""
+(oracle/_ref/dropin, linked by oracle/build_ref.py).
+
+The same binaries write two profiles through the drop-in print_hydroprofile
+(cumc3d_amd/fortran/dataio_hip.f90): rkiter-0 from the host arrays after
+INITIAL_UPDATE, rkiter-1 packed on the device after the last step.  Both
+must hold exactly the sections dataio.f90:178-233 writes of that state."""
 import os
 import tempfile
 
 import numpy as np
 import pytest
 
-from conftest import interior_window, load_golden
+from conftest import check_profile, interior_window, load_golden
 from cumc3d_amd import ic
+from oracle import oracle as O
 from oracle.build_ref import DROPIN_CASES
 from oracle.cases import CASES
 from oracle.refio import dropin_binary, run_ref
@@ -41,3 +47,7 @@ def test_dropin_matches_reference(name):
         a, b = got[f][w], gold[f]
         assert np.array_equal(a, b, equal_nan=True), f"{name}: {f} differs, max {np.nanmax(np.abs(a - b)):.3e}"
     assert got["maxdivb"] == float(gold["maxdivb"])
+    check_profile(got["profiles"]["rkiter-1-nm.hdf5"], case.cfg, got, got["t"])
+    o = O.Oracle(case.cfg, st)         # the state after INITIAL_UPDATE
+    check_profile(got["profiles"]["rkiter-0-nm.hdf5"], case.cfg, {f: o.get(f) for f in FIELDS[:9]}, 0.0)
+

@@ -4,7 +4,8 @@ bit for bit.  Needs an MI355X."""
 import numpy as np
 import pytest
 
-from conftest import interior_window, load_golden
+from conftest import check_profile, interior_window, load_golden
+from cumc3d_amd import hydroprofile
 from cumc3d_amd import ic
 from cumc3d_amd.config import Config, orszag_tang_3d, torus_cyl, torus_sph
 from cumc3d_amd.solver import make_solver
@@ -127,3 +128,19 @@ def test_gpu_deterministic():
     a.run(5)
     b.run(5)
     assert_same(a.state(), b.state())
+
+
+@pytest.mark.parametrize("name,precision", [("ot_hlld_ppm", "fp64"), ("sph_torus_ppm", "fp64"),
+                                            ("ot2d_hlld_ppm", "fp64"), ("ot_hlld_ppm", "fp32")])
+def test_gpu_profile_file(name, precision, tmp_path):
+    """print_hydroprofile through the device pack (cumc3d_profile) and
+    libcumc3d_io: the file written for a run holds exactly the state's
+    sections, in every precision (the FP32 split widens its single-precision
+    primitives exactly)."""
+    case = CASES[name]
+    print(f"case {name} {precision} profile", flush=True)
+    s = make_solver(case.cfg.with_(precision=precision), ic.GENERATORS[case.ic](case.cfg))
+    s.run(3)
+    path = hydroprofile.write_profile(s, str(tmp_path / "outfile"), 4)
+    assert path.endswith("outfile/rkiter-4-nm.hdf5")
+    check_profile(hydroprofile.read_profile(path), case.cfg, s.state(), s.t)
