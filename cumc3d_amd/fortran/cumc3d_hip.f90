@@ -108,6 +108,59 @@ INTERFACE
     REAL(C_DOUBLE), INTENT(OUT) :: ms(*)
     INTEGER(C_INT), VALUE :: n
   END FUNCTION
+  ! learned reconstruction selector (no counterpart in the reference)
+  INTEGER(C_INT) FUNCTION cumc3d_selector_init(ctx, capacity, seed) BIND(C, NAME='cumc3d_selector_init')
+    IMPORT :: C_INT, C_PTR
+    TYPE(C_PTR), VALUE :: ctx
+    INTEGER(C_INT), VALUE :: capacity, seed
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION cumc3d_selector_set_params(ctx, w, n) BIND(C, NAME='cumc3d_selector_set_params')
+    IMPORT :: C_INT, C_PTR, C_FLOAT
+    TYPE(C_PTR), VALUE :: ctx
+    REAL(C_FLOAT), INTENT(IN) :: w(*)
+    INTEGER(C_INT), VALUE :: n
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION cumc3d_selector_get_params(ctx, w, n) BIND(C, NAME='cumc3d_selector_get_params')
+    IMPORT :: C_INT, C_PTR, C_FLOAT
+    TYPE(C_PTR), VALUE :: ctx
+    REAL(C_FLOAT), INTENT(OUT) :: w(*)
+    INTEGER(C_INT), VALUE :: n
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION cumc3d_selector_collect(ctx, stride, thr_ppm, thr_weno, seed, counts) &
+      BIND(C, NAME='cumc3d_selector_collect')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR), VALUE :: ctx
+    INTEGER(C_INT), VALUE :: stride, seed
+    REAL(C_DOUBLE), VALUE :: thr_ppm, thr_weno
+    INTEGER(C_INT), INTENT(OUT), OPTIONAL :: counts(*)
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION cumc3d_selector_train(ctx, iters, blocks, lr, class_weight, seed, loss) &
+      BIND(C, NAME='cumc3d_selector_train')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE, C_FLOAT
+    TYPE(C_PTR), VALUE :: ctx
+    INTEGER(C_INT), VALUE :: iters, blocks, seed
+    REAL(C_DOUBLE), VALUE :: lr
+    REAL(C_DOUBLE), INTENT(IN), OPTIONAL :: class_weight(*)
+    REAL(C_FLOAT), INTENT(OUT), OPTIONAL :: loss(*)
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION cumc3d_selector_enable(ctx, enable, every) BIND(C, NAME='cumc3d_selector_enable')
+    IMPORT :: C_INT, C_PTR
+    TYPE(C_PTR), VALUE :: ctx
+    INTEGER(C_INT), VALUE :: enable, every
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION cumc3d_selector_mlp(ctx, features, n, logits) BIND(C, NAME='cumc3d_selector_mlp')
+    IMPORT :: C_INT, C_PTR, C_FLOAT
+    TYPE(C_PTR), VALUE :: ctx
+    REAL(C_FLOAT), INTENT(IN) :: features(*)
+    INTEGER(C_INT), VALUE :: n
+    REAL(C_FLOAT), INTENT(OUT) :: logits(*)
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION cumc3d_selector_mask(ctx, dir, refresh, out) BIND(C, NAME='cumc3d_selector_mask')
+    IMPORT :: C_INT, C_PTR, C_SIGNED_CHAR
+    TYPE(C_PTR), VALUE :: ctx
+    INTEGER(C_INT), VALUE :: dir, refresh
+    INTEGER(C_SIGNED_CHAR), INTENT(OUT) :: out(*)
+  END FUNCTION
   INTEGER(C_SIZE_T) FUNCTION c_strlen(s) BIND(C, NAME='strlen')
     IMPORT :: C_SIZE_T, C_PTR
     TYPE(C_PTR), VALUE :: s
