@@ -378,12 +378,6 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         long v = atol(mt);
         if (v >= 1) x->march_threads = v;
     }
-    x->nblockmax = blocks_needed(x) + 16;
-    if (hipMalloc(&x->blockmax, x->nblockmax * sizeof(unsigned long long)) != hipSuccess) {
-        cleanup();
-        return fail(CUMC3D_ENOMEM, "blockmax");
-    }
-    hipMemset(x->blockmax, 0, x->nblockmax * sizeof(unsigned long long));
     if (d.riemann == CUMC3D_HLL) {
         if (hipMalloc(&x->stale, (size_t)3 * x->G) != hipSuccess) { cleanup(); return fail(CUMC3D_ENOMEM, "stale flags"); }
         if (hipMemset(x->stale, 0, (size_t)3 * x->G) != hipSuccess) { cleanup(); return fail(CUMC3D_EHIP, "memset"); }
@@ -620,7 +614,6 @@ int cumc3d_finalize(cumc3d_ctx *x) {
     if (x->pool) hipFree(x->pool);
     if (x->met) hipFree(x->met);
     if (x->S) hipFree(x->S);
-    if (x->blockmax) hipFree(x->blockmax);
     if (x->stale) hipFree(x->stale);
     void *sel[] = {x->sel_w, x->sel_m, x->sel_v, x->sel_frag, x->mask, x->samp_feat, x->samp_label,
                    x->samp_counts, x->grad_part, x->loss_part, x->loss_dev};

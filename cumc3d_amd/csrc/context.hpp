@@ -22,8 +22,6 @@ struct cumc3d_ctx {
     bool fp32 = false;            // CUMC3D_FP32: working fields are float
     double *met = nullptr;        // grid metrics and weights
     cumc3d::DevScalars *S = nullptr;
-    unsigned long long *blockmax = nullptr;
-    long nblockmax = 0;
     // working-type fields (double, or float in the FP32 split)
     void *P[5], *eps, *cs, *Bc[3], *F[3][7], *E[3], *Ec[3];
     // always double: face field (ping-pong), u_old, conserved state (FP32 split), floors
@@ -139,8 +137,9 @@ template <class W> __global__ void k_init_interior(ArgsT<W> a) {
     checkrho(a, c, rho, v, eps);
     a.P[0][c] = (W)rho; a.P[1][c] = (W)v[0]; a.P[2][c] = (W)v[1]; a.P[3][c] = (W)v[2];
     a.eps[c] = (W)eps;
-    double b[3] = {bavg<0>(a, j, k, a.Bin[0][c], a.Bin[0][c - 1]), bavg<1>(a, j, k, a.Bin[1][c], a.Bin[1][c - a.NX]),
-                   bavg<2>(a, j, k, a.Bin[2][c], a.Bin[2][c - a.NX * a.NY])};
+    double b[3] = {bavg_rt<0>(a, j, k, a.Bin[0][c], a.Bin[0][c - 1]),
+                   bavg_rt<1>(a, j, k, a.Bin[1][c], a.Bin[1][c - a.NX]),
+                   bavg_rt<2>(a, j, k, a.Bin[2][c], a.Bin[2][c - a.NX * a.NY])};
     for (int d = 0; d < 3; ++d) a.Bc[d][c] = (W)b[d];
     cell_emf(a, c, (W)v[0], (W)v[1], (W)v[2], (W)b[0], (W)b[1], (W)b[2]);
     double p, cs;
@@ -166,7 +165,7 @@ template <class W> __global__ __launch_bounds__(256) void k_finddt(ArgsT<W> a) {
         const long c = cidx(a, j, k, l);
         double v[3] = {a.P[1][c], a.P[2][c], a.P[3][c]};
         double b[3] = {a.Bc[0][c], a.Bc[1][c], a.Bc[2][c]};
-        dtloc = cfl_dt(a, j, k, l, a.cs[c], a.P[0][c], v, b);
+        dtloc = cfl_dt_rt(a, j, k, l, a.cs[c], a.P[0][c], v, b);
     }
     dtloc = block_min_f64(dtloc, a.S);
     if (threadIdx.x == 0) atomicMin(&a.S->dtmin_bits, (unsigned long long)__double_as_longlong(dtloc));
@@ -204,6 +203,20 @@ template <class W> __global__ __launch_bounds__(256) void k_divb(ArgsT<W> a, uns
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
+// k_update for the grid's geometry and the stage
+template <class W, int G>
+static void launch_update_g(int stage, dim3 grid, hipStream_t st, const ArgsT<W> &a) {
+    if (stage == 1) hipLaunchKernelGGL((k_update<1, W, G>), grid, dim3(256), 0, st, a);
+    else if (stage == 2) hipLaunchKernelGGL((k_update<2, W, G>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_update<3, W, G>), grid, dim3(256), 0, st, a);
+}
+template <class W>
+static void launch_update(int geo, int stage, dim3 grid, hipStream_t st, const ArgsT<W> &a) {
+    if (geo == 1) launch_update_g<W, 1>(stage, grid, st, a);
+    else if (geo == 2) launch_update_g<W, 2>(stage, grid, st, a);
+    else launch_update_g<W, 0>(stage, grid, st, a);
+}
+
 template <class W> static ArgsT<W> make_args(cumc3d_ctx *x) {
     ArgsT<W> a;
     memset(&a, 0, sizeof(a));
@@ -237,7 +250,6 @@ template <class W> static ArgsT<W> make_args(cumc3d_ctx *x) {
     a.pfl = x->floors[1];
     a.efl = x->floors[2];
     a.S = x->S;
-    a.blockmax = x->blockmax;
     for (int d = 0; d < 3; ++d) a.mask[d] = x->mask ? x->mask + (size_t)d * x->G : nullptr;
     for (int d = 0; d < 3; ++d) a.stale[d] = x->stale ? x->stale + (size_t)d * x->G : nullptr;
     a.dcb = x->sph[0]; a.dcb3 = x->sph[1]; a.xbar = x->sph[2]; a.gsrc = x->sph[3]; a.hbx = x->sph[4];
@@ -270,19 +282,19 @@ static void launch_sweep_rs(cumc3d_ctx *x, const ArgsT<W> &a, int D, hipStream_t
         unsigned grid = (unsigned)((waves * 64 + 255) / 256);
         hipLaunchKernelGGL((k_flux_x<W, R, S>), dim3(grid), dim3(256), 0, st, a);
         if (S == 3 && a.bn_mode == 2)
-            hipLaunchKernelGGL((k_carry<W, R, 0>), dim3(1), dim3(1024), 0, st, a, (int)grid);
+            hipLaunchKernelGGL((k_carry<W, R, 0>), dim3(1), dim3(64), 0, st, a);
     } else if (D == 1) {
         int ns = nseg_for(x, x->nx, x->nz, x->ny);
         unsigned grid = nblk((long)x->nx * x->nz * ns);
         hipLaunchKernelGGL((k_flux_yz<W, R, S, 1>), dim3(grid), dim3(256), 0, st, a, ns);
         if (S == 3 && a.bn_mode == 2)
-            hipLaunchKernelGGL((k_carry<W, R, 1>), dim3(1), dim3(1024), 0, st, a, (int)grid);
+            hipLaunchKernelGGL((k_carry<W, R, 1>), dim3(1), dim3(64), 0, st, a);
     } else {
         int ns = nseg_for(x, x->nx, x->ny, x->nz);
         unsigned grid = nblk((long)x->nx * x->ny * ns);
         hipLaunchKernelGGL((k_flux_yz<W, R, S, 2>), dim3(grid), dim3(256), 0, st, a, ns);
         if (S == 3 && a.bn_mode == 2)
-            hipLaunchKernelGGL((k_carry<W, R, 2>), dim3(1), dim3(1024), 0, st, a, (int)grid);
+            hipLaunchKernelGGL((k_carry<W, R, 2>), dim3(1), dim3(64), 0, st, a);
     }
 }
 
@@ -310,15 +322,6 @@ template <class W> static void launch_sweep(cumc3d_ctx *x, const ArgsT<W> &a, in
     }
 }
 
-static long blocks_needed(cumc3d_ctx *x) {
-    long waves = ((long)(x->nx + 2) * x->ny * x->nz + 62) / 63 + 1;
-    long bx = (waves * 64 + 255) / 256;
-    long by = nblk((long)x->nx * x->nz * nseg_for(x, x->nx, x->nz, x->ny));
-    long bz = nblk((long)x->nx * x->ny * nseg_for(x, x->nx, x->ny, x->nz));
-    long m = bx > by ? bx : by;
-    return m > bz ? m : bz;
-}
-
 // ---------------------------------------------------------------------------
 // stage / step
 // ---------------------------------------------------------------------------
@@ -343,9 +346,7 @@ template <class W> static void run_stage_t(cumc3d_ctx *x, int stage) {
     hipLaunchKernelGGL(k_edge<W>, dim3(nblk((long)(x->nx + 1) * (x->ny + 1) * (x->nz + 1))), dim3(256), 0, st, a);
     if (a.fixpole) hipLaunchKernelGGL(k_fixpole<W>, dim3(nblk(2L * x->nx)), dim3(256), 0, st, a);
     tmark(x, 4);
-    if (stage == 1) hipLaunchKernelGGL((k_update<1, W>), dim3(nblk(nn)), dim3(256), 0, st, a);
-    else if (stage == 2) hipLaunchKernelGGL((k_update<2, W>), dim3(nblk(nn)), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_update<3, W>), dim3(nblk(nn)), dim3(256), 0, st, a);
+    launch_update<W>(x->d.coordinate_flag, stage, dim3(nblk(nn)), st, a);
     tmark(x, 5);
     x->cur = 1 - x->cur;
     ArgsT<W> g = make_args<W>(x);

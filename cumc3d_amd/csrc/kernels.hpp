@@ -24,6 +24,7 @@ struct DevScalars {
     double total_time;
     unsigned long long dtmin_bits;    // CFL min (bit pattern of a positive double)
     double carry[3];                  // HLLD usstar carried per field component (x,y,z)
+    unsigned long long carry_key;     // max serial key of the sweep's double-star interfaces (0: none)
     int nstep;
     int active;                       // 0 once global_time >= total_time: stage kernels exit
     int nonfinite;                    // set when a cell's CFL time step is NaN or not positive
@@ -65,7 +66,6 @@ template <class W> struct ArgsT {
     W *Ec[3];               // cell EMFs ecell after the ghost fill, cells 0..n+1
     const double *rfl, *pfl, *efl;   // torus floors
     DevScalars *S;
-    unsigned long long *blockmax;    // per-block max of double-star interface keys
     const unsigned char *mask[3];    // selector class per cell and sweep direction
     unsigned char *stale[3];         // HLL only: 1 where a sweep assigned no flux (see store_flux_hll)
     // spherical grids (getgrid.f90:68-98), index j+3 / k+3: dx_cb, dx_cb/3,
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(256) void k_flux_x(ArgsT<T> a) {
     if constexpr (S == 3) {
         if (a.bn_mode == 2) {
             key = block_max_u64(key);
-            if (threadIdx.x == 0) a.blockmax[blockIdx.x] = key;
+            if (threadIdx.x == 0 && key) atomicMax(&a.S->carry_key, key);
         }
     }
 }
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void k_flux_yz(ArgsT<T> a, int nseg) {
     if constexpr (S == 3) {
         if (a.bn_mode == 2) {
             key = block_max_u64(key);
-            if (threadIdx.x == 0) a.blockmax[blockIdx.x] = key;
+            if (threadIdx.x == 0 && key) atomicMax(&a.S->carry_key, key);
         }
     }
 }
@@ -439,15 +439,11 @@ __global__ __launch_bounds__(256) void k_flux_yz(ArgsT<T> a, int nseg) {
 // reference leaves behind in usstarL/usstarR and reads back as usstar(ibn) in
 // later sweeps (rsolver_module.f90:704-741).
 template <class T, int R, int D>
-__global__ __launch_bounds__(1024) void k_carry(ArgsT<T> a, int nblocks) {
-    if (!a.S->active) return;
-    unsigned long long key = 0;
-    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
-        unsigned long long v = a.blockmax[i];
-        key = v > key ? v : key;
-    }
-    key = block_max_u64(key);
-    if (threadIdx.x != 0 || key == 0) return;
+__global__ __launch_bounds__(64) void k_carry(ArgsT<T> a) {
+    if (!a.S->active || threadIdx.x != 0) return;
+    const unsigned long long key = a.S->carry_key;
+    a.S->carry_key = 0;     // the next sweep starts from no double-star interface
+    if (key == 0) return;
     unsigned long long q = key - 1;
     int j, k, l;
     if constexpr (D == 0) {
@@ -646,31 +642,33 @@ template <class W> __global__ __launch_bounds__(256) void k_fixpole(ArgsT<W> a) 
 // parameter.h:24 small_num = TINY(1.0D0), the smallest normal double
 constexpr double SMALL_NUM = 2.2250738585072014e-308;
 
-template <class A>
+// G: geometry (coordinate_flag), a template parameter so that each grid type
+// compiles only its own terms (the update kernel's registers depend on it).
+template <int G, class A>
 __device__ __forceinline__ double lB_x(const A &a, int jf, int k, int l, double ez0, double ez1, double ey0, double ey1) {
     double dy = a.ax[1].d[k + 3], dz = a.ax[2].d[l + 3];
-    if (a.coord == 0) return -((ez0 - ez1) / (dy) - (ey0 - ey1) / (dz));
+    if constexpr (G == 0) return -((ez0 - ez1) / (dy) - (ey0 - ey1) / (dz));
     double xF = a.ax[0].xF[jf + 3];
-    if (a.coord == 2)   // spatial.f90:689
+    if constexpr (G == 2)   // spatial.f90:689
         return (-(a.sinf[k + 3] * ez0 - a.sinf[k + 2] * ez1) + (ey0 - ey1) * (dy / dz)) / (xF * a.dcose[k + 3] + SMALL_NUM);
     return -(ez0 - ez1) / (xF * dy) + (ey0 - ey1) / (dz);
 }
-template <class A>
+template <int G, class A>
 __device__ __forceinline__ double lB_y(const A &a, int j, int kf, int l, double ex0, double ex1, double ez0, double ez1) {
     double dx = a.ax[0].d[j + 3], dz = a.ax[2].d[l + 3];
-    if (a.coord == 0) return -((ex0 - ex1) / (dz) - (ez0 - ez1) / (dx));
-    if (a.coord == 2) {   // spatial.f90:692
+    if constexpr (G == 0) return -((ex0 - ex1) / (dz) - (ez0 - ez1) / (dx));
+    if constexpr (G == 2) {   // spatial.f90:692
         double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2], x = a.ax[0].c[j + 3];
         return (-(ex0 - ex1) / (a.xbar[j + 3] * a.sinf[kf + 3] * dz + SMALL_NUM) + (xF * ez0 - xFm * ez1) / (x * dx));
     }
     return -(ex0 - ex1) / (dz) + (ez0 - ez1) / (dx);
 }
-template <class A>
+template <int G, class A>
 __device__ __forceinline__ double lB_z(const A &a, int j, int k, double ey0, double ey1, double ex0, double ex1) {
     double dx = a.ax[0].d[j + 3], dy = a.ax[1].d[k + 3];
-    if (a.coord == 0) return -((ey0 - ey1) / (dx) - (ex0 - ex1) / (dy));
+    if constexpr (G == 0) return -((ey0 - ey1) / (dx) - (ex0 - ex1) / (dy));
     double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2], x = a.ax[0].c[j + 3];
-    if (a.coord == 2)   // spatial.f90:695
+    if constexpr (G == 2)   // spatial.f90:695
         return (-(xF * ey0 - xFm * ey1) / (x * dx) + (ex0 - ex1) / (a.xbar[j + 3] * dy));
     return -(xF * ey0 - xFm * ey1) / (x * dx) + (ex0 - ex1) / (x * dy);
 }
@@ -714,7 +712,8 @@ template <class A> __device__ __forceinline__ void pressure(const A &a, long c, 
 }
 
 // FINDDT per cell (rungekutta.f90:272-311)
-template <class A> __device__ __forceinline__ double cfl_dt(const A &a, int j, int k, int l, double cs, double rho, const double v[3],
+template <int G, class A>
+__device__ __forceinline__ double cfl_dt(const A &a, int j, int k, int l, double cs, double rho, const double v[3],
                                          const double b[3]) {
     double a2 = cs * cs, a4 = a2 * a2;
     double b2x = (b[0] * b[0] / rho), b2y = (b[1] * b[1] / rho), b2z = (b[2] * b[2] / rho);
@@ -724,10 +723,10 @@ template <class A> __device__ __forceinline__ double cfl_dt(const A &a, int j, i
     double cfz = sqrtT(0.5 * (a2 + b2 + sqrtT((a4 + 2.0 * a2 * b2 + b4) - 4.0 * a2 * b2z)));
     double lam = fmax2(fmax2(fabsT(v[0]) + cfx, fabsT(v[1]) + cfy), fabsT(v[2]) + cfz);
     double h = a.ax[0].d[j + 3];
-    if (a.coord == 0) {
+    if constexpr (G == 0) {
         if (a.ndim > 1) h = fmin2(h, a.ax[1].d[k + 3]);
         if (a.ndim > 2) h = fmin2(h, a.ax[2].d[l + 3]);
-    } else if (a.coord == 1) {
+    } else if constexpr (G == 1) {
         if (a.ndim > 1 && a.ny > 1) h = fmin2(h, a.ax[0].c[j + 3] * a.ax[1].d[k + 3]);
         if (a.ndim > 2) h = fmin2(h, a.ax[2].d[l + 3]);
     } else {
@@ -761,31 +760,46 @@ __device__ __forceinline__ double block_min_f64(double v, DevScalars *S) {
 }
 
 // geometry-weighted cell average of face field component D (fromrvetou.f90:153-195)
-template <int D, class A>
+template <int D, int G, class A>
 __device__ __forceinline__ double bavg(const A &a, int j, int k, double up, double lo) {
-    if (D == 0 && a.coord == 1) {
+    if constexpr (D == 0 && G == 1) {
         double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2];
         return (xF * up + xFm * lo) / (xF + xFm);
     }
-    if (D == 0 && a.coord == 2) {
+    if constexpr (D == 0 && G == 2) {
         double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2];
         return a.hbx[j + 3] * (xF * xF * up + xFm * xFm * lo);
     }
-    if (D == 1 && a.coord == 2) return a.hby[k + 3] * (a.sinf[k + 3] * up + a.sinf[k + 2] * lo);
+    if constexpr (D == 1 && G == 2) return a.hby[k + 3] * (a.sinf[k + 3] * up + a.sinf[k + 2] * lo);
     return 0.5 * (up + lo);
+}
+
+// the same with the geometry chosen at run time (kernels that run once per step or less)
+template <int D, class A>
+__device__ __forceinline__ double bavg_rt(const A &a, int j, int k, double up, double lo) {
+    if (a.coord == 1) return bavg<D, 1>(a, j, k, up, lo);
+    if (a.coord == 2) return bavg<D, 2>(a, j, k, up, lo);
+    return bavg<D, 0>(a, j, k, up, lo);
+}
+template <class A>
+__device__ __forceinline__ double cfl_dt_rt(const A &a, int j, int k, int l, double cs, double rho, const double v[3],
+                                            const double b[3]) {
+    if (a.coord == 1) return cfl_dt<1>(a, j, k, l, cs, rho, v, b);
+    if (a.coord == 2) return cfl_dt<2>(a, j, k, l, cs, rho, v, b);
+    return cfl_dt<0>(a, j, k, l, cs, rho, v, b);
 }
 
 // One interior cell: FLUX_DIFF + sources + l_operator + RK combine +
 // FROMUTORVE + CUSTOM_CHECKRHO + FROMRVETOU + FINDPRESSURE; returns the
 // cell's CFL time step in stage 3.
-template <int STAGE, class W>
+template <int STAGE, class W, int G>
 __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, int l, double dt) {
     constexpr bool split = sizeof(W) == 4;   // FP32 split: U holds the conserved state
     const long c = cidx(a, j, k, l), sy = a.NX, sz = a.NX * a.NY;
     // stage-start primitives: FROMRVETOU input (fromrvetou.f90:428-438) in the
     // FP64 mode, source terms only in the FP32 split
     double rho = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
-    if (!split || a.coord != 0 || a.model != 0 || a.rt_gravity) {
+    if (!split || G != 0 || a.model != 0 || a.rt_gravity) {
         rho = a.P[0][c]; vx = a.P[1][c]; vy = a.P[2][c]; vz = a.P[3][c];
         bx = a.Bc[0][c]; by = a.Bc[1][c]; bz = a.Bc[2][c];
     }
@@ -808,11 +822,11 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
     }
     // sources: GET_SOURCE (spatial.f90:195-226) + CUSTOM_SOURCE
     double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (a.coord == 1) {
+    if constexpr (G == 1) {
         double bsq = bx * bx + by * by + bz * bz, xj = a.ax[0].c[j + 3];
         sc[1] = sc[1] + (a.P[4][c] + rho * vy * vy + 0.5 * bsq - by * by) / (xj);
         sc[2] = sc[2] - (rho * vx * vy - bx * by) / (xj);
-    } else if (a.coord == 2) {   // spatial.f90:227-256
+    } else if constexpr (G == 2) {   // spatial.f90:227-256
         double bsq = bx * bx + by * by + bz * bz, p = a.P[4][c];
         double g = a.gsrc[j + 3], cot = a.cot[k + 3];
         sc[1] = sc[1] + (2.0 * p + bx * bx + rho * (vy * vy + vz * vz)) * g;
@@ -847,9 +861,9 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
         for (int i = 0; i < 5; ++i) {
             const double f1 = a.F[0][i][c], f0 = a.F[0][i][c - 1];
             double df;
-            if (a.coord == 1)
+            if constexpr (G == 1)
                 df = (a.ax[0].xF[j + 3] * f1 - a.ax[0].xF[j + 2] * f0) / (a.ax[0].c[j + 3] * dx);
-            else if (a.coord == 2) {
+            else if constexpr (G == 2) {
                 double xF = a.ax[0].xF[j + 3], xFm = a.ax[0].xF[j + 2];
                 df = (xF * xF * f1 - xFm * xFm * f0) / a.dcb3[j + 3];
             } else df = (f1 - f0) / (dx);
@@ -859,9 +873,11 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
             double dy = a.ax[1].d[k + 3];
             for (int i = 0; i < 5; ++i) {
                 const double f1 = a.F[1][i][c], f0 = a.F[1][i][c - sy];
-                double df = (a.coord == 1) ? (f1 - f0) / (a.ax[0].c[j + 3] * dy)
-                            : (a.coord == 2) ? (a.sinf[k + 3] * f1 - a.sinf[k + 2] * f0) / (a.xbar[j + 3] * a.dcose[k + 3])
-                                             : (f1 - f0) / (dy);
+                double df;
+                if constexpr (G == 1) df = (f1 - f0) / (a.ax[0].c[j + 3] * dy);
+                else if constexpr (G == 2)
+                    df = (a.sinf[k + 3] * f1 - a.sinf[k + 2] * f0) / (a.xbar[j + 3] * a.dcose[k + 3]);
+                else df = (f1 - f0) / (dy);
                 L[i] = L[i] - df;
             }
         }
@@ -869,7 +885,7 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
             double dz = a.ax[2].d[l + 3];
             for (int i = 0; i < 5; ++i) {
                 const double f1 = a.F[2][i][c], f0 = a.F[2][i][c - sz];
-                if (a.coord == 2)
+                if constexpr (G == 2)
                     L[i] = L[i] - (f1 - f0) * a.ax[1].d[k + 3] / (a.xbar[j + 3] * dz * a.dcose[k + 3]);
                 else L[i] = L[i] - (f1 - f0) / dz;
             }
@@ -887,10 +903,10 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
     {
         const long cl[3] = {c - 1, c - sy, c - sz};
         const int fidx[3] = {j, k, l};
-        const double Lup[3] = {lB_x(a, j, k, l, ez_c, ez_my, ey_c, ey_mz), lB_y(a, j, k, l, ex_c, ex_mz, ez_c, ez_mx),
-                               lB_z(a, j, k, ey_c, ey_mx, ex_c, ex_my)};
-        const double Llo[2] = {lB_x(a, j - 1, k, l, ez_mx, ez_mxmy, ey_mx, ey_mxmz),
-                               lB_y(a, j, k - 1, l, ex_my, ex_mymz, ez_my, ez_mxmy)};
+        const double Lup[3] = {lB_x<G>(a, j, k, l, ez_c, ez_my, ey_c, ey_mz), lB_y<G>(a, j, k, l, ex_c, ex_mz, ez_c, ez_mx),
+                               lB_z<G>(a, j, k, ey_c, ey_mx, ex_c, ex_my)};
+        const double Llo[2] = {lB_x<G>(a, j - 1, k, l, ez_mx, ez_mxmy, ey_mx, ey_mxmz),
+                               lB_y<G>(a, j, k - 1, l, ex_my, ex_mymz, ez_my, ez_mxmy)};
         for (int d = 0; d < 3; ++d) {
             double bup = a.Bin[d][c];
             double uoup;
@@ -908,7 +924,7 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
             } else {
                 uolo = a.Uo[5 + d][cl[d]];
             }
-            const double llo = d < 2 ? Llo[d] : lB_z(a, j, k, ey_mz, ey_mxmz, ex_mz, ex_mymz);
+            const double llo = d < 2 ? Llo[d] : lB_z<G>(a, j, k, ey_mz, ey_mxmz, ex_mz, ex_mymz);
             bn_lo[d] = rk<STAGE>(uolo, blo, dt, llo);
             // the lowest face (index 0) keeps its stored value in prim
             // (FROMUTORVE copies faces 1..n only, fromrvetou.f90:201-207)
@@ -916,8 +932,8 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
         }
     }
     // FROMUTORVE (fromrvetou.f90:153-211)
-    double bce[3] = {bavg<0>(a, j, k, bn_up[0], bn_lo[0]), bavg<1>(a, j, k, bn_up[1], bn_lo[1]),
-                     bavg<2>(a, j, k, bn_up[2], bn_lo[2])};
+    double bce[3] = {bavg<0, G>(a, j, k, bn_up[0], bn_lo[0]), bavg<1, G>(a, j, k, bn_up[1], bn_lo[1]),
+                     bavg<2, G>(a, j, k, bn_up[2], bn_lo[2])};
     double rn = un[0];
     double v[3] = {un[1] / rn, un[2] / rn, un[3] / rn};
     double bsq = bce[0] * bce[0] + bce[1] * bce[1] + bce[2] * bce[2];
@@ -926,8 +942,8 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
     checkrho(a, c, rn, v, en);
     if (STAGE == 3) checkrho(a, c, rn, v, en);
     // FROMRVETOU bcell (fromrvetou.f90:380-422)
-    double bcf[3] = {bavg<0>(a, j, k, bn_up[0], bprim_lo[0]), bavg<1>(a, j, k, bn_up[1], bprim_lo[1]),
-                     bavg<2>(a, j, k, bn_up[2], bprim_lo[2])};
+    double bcf[3] = {bavg<0, G>(a, j, k, bn_up[0], bprim_lo[0]), bavg<1, G>(a, j, k, bn_up[1], bprim_lo[1]),
+                     bavg<2, G>(a, j, k, bn_up[2], bprim_lo[2])};
     double p, cs;
     pressure(a, c, rn, en, p, cs);
     a.P[0][c] = (W)rn;
@@ -954,14 +970,14 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
         a.U[3][c] = v[2] * rn;
         a.U[4][c] = rn * (en + 0.5 * vsq2) + 0.5 * bsq2;
     }
-    return STAGE == 3 ? cfl_dt(a, j, k, l, cs, rn, v, bcf) : 1.0e5;
+    return STAGE == 3 ? cfl_dt<G>(a, j, k, l, cs, rn, v, bcf) : 1.0e5;
 }
 
 // One thread per interior cell, x fastest.  (Measured on MI355X at 256^3:
 // this beats a z march carrying the plane below in registers, whose extra
 // live values cut occupancy, 64 x 4 cell tiles, and an XCD-grouped block
 // order; rows in plain round-robin block order stream best.)
-template <int STAGE, class W>
+template <int STAGE, class W, int G>
 __global__ __launch_bounds__(256) void k_update(ArgsT<W> a) {
     const long nn = (long)a.nx * a.ny * a.nz;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -977,7 +993,7 @@ __global__ __launch_bounds__(256) void k_update(ArgsT<W> a) {
         return;
     }
     double dtloc = 1.0e5;
-    if (live) dtloc = update_cell<STAGE, W>(a, j, k, l, a.S->dt);
+    if (live) dtloc = update_cell<STAGE, W, G>(a, j, k, l, a.S->dt);
     if constexpr (STAGE == 3) {
         dtloc = block_min_f64(dtloc, a.S);
         if (threadIdx.x == 0) atomicMin(&a.S->dtmin_bits, (unsigned long long)__double_as_longlong(dtloc));
