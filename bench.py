@@ -71,8 +71,8 @@ def kernel_bytes(cfg, name: str, w: int) -> float:
         if split:
             # U, u_old, face B (+ lower faces) in double; fluxes, edge EMFs
             # and the primitives written back in float (the stage-start
-            # primitives are read only for the cylindrical/torus/RT sources)
-            prim_in = 8 if (cfg.coordinate_flag == 1 or cfg.model == "torus" or cfg.rt_gravity) else 0
+            # primitives are read only for the curvilinear/torus/RT sources)
+            prim_in = 8 if (cfg.coordinate_flag != 0 or cfg.model != "riemann" or cfg.rt_gravity) else 0
             return n * (8 * (5 + 8 + 3 + 3 + 5) + w * (prim_in + 15 + 3 + 5 + 1 + 1 + 3 + 3))
         # reads: prim 5, bcell 3, eps, u_old 8, face B 3, fluxes 15, edge EMFs 3;
         # writes: prim 5, eps, cs, bcell 3, face B 3, cell EMFs 3
@@ -109,7 +109,7 @@ def measured_traffic(kernel: str, w: int, n: int):
 
 def make_workload(name: str, n: int):
     from cumc3d_amd import ic
-    from cumc3d_amd.config import blast_3d, orszag_tang_3d, turbulence_3d
+    from cumc3d_amd.config import blast_3d, orszag_tang_3d, torus_cyl, torus_sph, turbulence_3d
     if name == "ot":
         cfg = orszag_tang_3d(n, recon="weno", riemann="hlld")
         return cfg, ic.orszag_tang(cfg), f"3D periodic Orszag-Tang {n}^3, HLLD + WENO-Z"
@@ -119,7 +119,38 @@ def make_workload(name: str, n: int):
     if name == "turbulence":
         cfg = turbulence_3d(n)
         return cfg, ic.turbulence(cfg), f"3D periodic decaying turbulence {n}^3, HLLD + WENO-Z"
+    if name == "torus_sph":
+        cfg = torus_sph(n, 3 * n // 4, n // 2, recon="weno", riemann="hll")
+        return (cfg, ic.torus_sph(cfg),
+                f"spherical torus (r, theta, phi) {cfg.nx}x{cfg.ny}x{cfg.nz} with FIXPOLE, HLL + WENO-Z")
+    if name == "torus_cyl":
+        cfg = torus_cyl(n, n // 2, n, recon="ppm")
+        return cfg, ic.torus(cfg), f"cylindrical torus (r, phi, z) {cfg.nx}x{cfg.ny}x{cfg.nz}, HLLD + PPM"
     raise SystemExit(f"unknown workload {name}")
+
+
+def profile_io(cfg, st, variant: str, sel_w=None, steps: int = 3):
+    """Time print_hydroprofile's path at this size: cumc3d_profile (device
+    pack + copy to the host) and the HDF5 write through libcumc3d_io."""
+    import tempfile
+    from cumc3d_amd import hydroprofile
+    s = make_variant(cfg, st, variant, sel_w)
+    s.run(steps, total_time=1e300, sync_each_step=False)
+    t0 = time.perf_counter()
+    p = s.profile()
+    t1 = time.perf_counter()
+    nbytes = sum(a.nbytes for a in p.values())
+    with tempfile.TemporaryDirectory() as d:
+        faces = hydroprofile.written_faces(s.grid)
+        path = hydroprofile.profile_name(d, 1)
+        t2 = time.perf_counter()
+        hydroprofile.write_arrays(path, cfg, faces, s.t, p["primitive"], p["epsilon"], p["bfield"])
+        t3 = time.perf_counter()
+        size = os.path.getsize(path)
+    del s
+    return {"bytes": nbytes, "file_bytes": size, "pack_and_copy_ms": 1e3 * (t1 - t0),
+            "pack_and_copy_GBs": nbytes / (t1 - t0) / 1e9, "hdf5_write_ms": 1e3 * (t3 - t2),
+            "hdf5_write_GBs": size / (t3 - t2) / 1e9}
 
 
 def train_selector(n: int, workload: str, thr_ppm: float, thr_weno: float, seed: int = 1, iters: int = 600):
@@ -227,6 +258,8 @@ def divb_norm(cfg, st):
     lowest face of each axis stays frozen in the reference, so cells 1 are left
     out)."""
     import numpy as np
+    if cfg.coordinate_flag != 0:
+        return None     # the curvilinear divergences are checked by the tests
     bx, by, bz = st["bx"], st["by"], st["bz"]
     h = (cfg.x_end - cfg.x_start) / cfg.nx
     i = (slice(4, -3),) * 3
@@ -313,7 +346,9 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--workload", default="ot", choices=["ot", "blast", "turbulence"])
+    ap.add_argument("--workload", default="ot", choices=["ot", "blast", "turbulence", "torus_sph", "torus_cyl"])
+    ap.add_argument("--profile-io", action="store_true",
+                    help="also time the profile output path (device pack, copy, HDF5 write)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-l1", action="store_true")
     ap.add_argument("--train-n", type=int, default=64)
@@ -371,6 +406,11 @@ def main(argv=None):
         print(f"[bench] L1 against FP64 after {nacc} steps", file=sys.stderr, flush=True)
         err = l1_errors(cfg, st, nacc, sel_w)
 
+    io = None
+    if args.profile_io:
+        print("[bench] profile output path", file=sys.stderr, flush=True)
+        io = profile_io(cfg, st, main_var, sel_w)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline (reference Fortran)", file=sys.stderr, flush=True)
@@ -383,7 +423,7 @@ def main(argv=None):
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64 (bf16 selector)",
-        "data": f"synthetic (seeded IC, cumc3d_amd.ic.{ {'ot': 'orszag_tang'}.get(args.workload, args.workload) })",
+        "data": f"synthetic (seeded IC, cumc3d_amd.ic.{ {'ot': 'orszag_tang', 'torus_cyl': 'torus'}.get(args.workload, args.workload) })",
         "config": {"workload": label + ", FP32 split + learned selector (FP64 accumulators)", "zones": zones,
                    "riemann": cfg.riemann, "recon": "selector: TVD-MC / PPM / WENO-Z per cell"},
         "variants": {v: {"value": rate(v), "ms_per_step": 1e3 * res[v][0] / args.steps} for v in res},
@@ -393,6 +433,8 @@ def main(argv=None):
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if io is not None:
+        line["output"] = io
     print(json.dumps(line))
     return 0
 

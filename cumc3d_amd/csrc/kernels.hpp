@@ -207,6 +207,22 @@ __device__ __forceinline__ void recon_stencil(const ArgsT<T> &a, long c, int i, 
             load_w<T>(a.ax[D].wsel[2] + (long)(i + 3) * NW, w, 10);
             recon_cell<T, 4>(w, s, (T)a.g1, fm, fp);
         }
+        // positivity fallback (the selector's own, no reference counterpart):
+        // a PPM or WENO-Z face with non-positive density or pressure gives
+        // the Riemann solver a NaN sound speed; such a cell takes its TVD-MC
+        // faces instead, which stay within the neighbouring cell values
+        if (cls > 0) {
+            const bool ok = fm.rho > T(0) && fp.rho > T(0) && fm.p > T(0) && fp.p > T(0);
+            if (__ballot(!ok)) {
+                Face<T> gm, gp;
+                load_w<T>(a.ax[D].wsel[0] + (long)(i + 3) * NW, w, 2);
+                recon_cell<T, 2>(w, s, (T)a.g1, gm, gp);
+                if (!ok) {
+                    fm = gm;
+                    fp = gp;
+                }
+            }
+        }
     } else {
         T w[NW];
         load_w<T>(a.ax[D].w + (long)(i + 3) * NW, w, nweights<R>());
