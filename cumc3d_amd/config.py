@@ -12,14 +12,14 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import Tuple
+from typing import Optional, Tuple
 
 # Riemann solvers (solver/mhd/parameter.h:91-101 LF_flag/HLL_flag/HLLC_flag/HLLD_flag)
 RIEMANN = {"lf": 0, "hll": 1, "hllc": 2, "hlld": 3}
 # Reconstructions (parameter.h:103-115 tvdmm/tvdmc/tvdvl/ppmc/weno flags)
 RECON = {"tvdmm": 0, "tvdvl": 1, "tvdmc": 2, "ppm": 3, "weno": 4}
-# In-step model hooks (model/riemann, model/torus-cyn, model/torus-sph)
-MODELS = {"riemann": 0, "torus": 1, "torus_sph": 2}
+# In-step model hooks (model/riemann, model/torus-cyn, model/torus-sph, model/collapsar)
+MODELS = {"riemann": 0, "torus": 1, "torus_sph": 2, "collapsar": 3}
 # Arithmetic of the spatial operator
 PRECISION = {"fp64": 0, "fp32": 1}
 
@@ -54,6 +54,18 @@ class Config:
     # theta = 0 and pi with sin = 0 (getgrid.f90:91-98), and the radial edge
     # EMFs on them are averaged over phi (mhd_module.f90:437-461)
     fixpole: bool = True
+    # Face coordinates xF, yF, zF (-3..n+3, n+7 values each) of a custom grid
+    # (CUSTOM_GRID, e.g. model/collapsar/src/custom.f90:68-155 reading
+    # ./profile/*_grid.dat); None on an axis keeps the uniform faces of
+    # getgrid.f90:26-34.
+    faces: Optional[Tuple[Optional[Tuple[float, ...]], ...]] = None
+    # Self-gravity of the collapsar model (model/collapsar/src/param.h:37-44,
+    # def.f90:31): potential solved every n_pot steps by red-black SOR with
+    # weight sor_omega, to relative tolerance sor_tolerance, at most relax_max sweeps
+    n_pot: int = 10
+    relax_max: int = 100000
+    sor_tolerance: float = 1.0e-10
+    sor_omega: float = 1.90
 
     def __post_init__(self):
         self.boundary_flag = tuple(int(b) for b in self.boundary_flag)
@@ -73,6 +85,21 @@ class Config:
             raise ValueError("the torus-cyn model needs cylindrical coordinates")
         if self.model == "torus_sph" and self.coordinate_flag != 2:
             raise ValueError("the torus-sph model needs spherical coordinates")
+        if self.model == "collapsar" and self.coordinate_flag != 2:
+            raise ValueError("the collapsar model needs spherical coordinates")
+        if self.n_pot < 1 or self.relax_max < 1:
+            raise ValueError("n_pot and relax_max must be positive")
+        if self.faces is not None:
+            if len(self.faces) != 3:
+                raise ValueError("faces needs (xF, yF, zF)")
+            faces = []
+            for f, n in zip(self.faces, (self.nx, self.ny, self.nz)):
+                if f is not None:
+                    f = tuple(float(v) for v in f)
+                    if len(f) != n + 7:
+                        raise ValueError("a face array needs n+7 entries (-3..n+3)")
+                faces.append(f)
+            self.faces = tuple(faces)
         if self.n_dim == 1 and (self.ny != 1 or self.nz != 1):
             raise ValueError("n_dim=1 needs ny=nz=1")
         if self.n_dim == 2 and self.nz != 1:
@@ -142,3 +169,32 @@ def torus_sph(nr: int = 128, nth: int = 96, nphi: int = 64, recon: str = "ppm",
                   z_start=0.0, z_end=2.0 * math.pi, cfl=0.4, ggas=5.0 / 3.0,
                   riemann=riemann, recon=recon, model="torus_sph", r_sh=1.0,
                   boundary_flag=(1, 1, 3, 3, 0, 0))
+
+
+def log_faces(n: int, r_in: float, r_out: float) -> Tuple[float, ...]:
+    """Radial faces xF(-3..n+3) growing by a constant ratio from xF(0) = r_in
+    to xF(n) = r_out (a logarithmic grid like the collapsar's r_grid.dat)."""
+    q = (r_out / r_in) ** (1.0 / n)
+    return tuple(r_in * q ** j for j in range(-3, n + 4))
+
+
+def collapsar(nr: int = 156, nth: int = 44, nphi: int = 128, recon: str = "ppm",
+              riemann: str = "hlld", n_dim: int = 3, faces=None, ggas: float = 4.0 / 3.0,
+              r_in: float = 32.0, r_out: float = 6.4e4) -> Config:
+    """Collapsar: a rotating, magnetised progenitor around a black hole, with
+    self-gravity (model/collapsar).  Spherical (r, theta, phi) with the axis
+    at both poles, periodic in phi, outflow in r, on a custom grid (CUSTOM_GRID,
+    model/collapsar/src/custom.f90:68-155).  The default size is the
+    reference's shipped progenitor grid (model/collapsar/profile: 156 radial,
+    44 polar, 128 azimuthal cells).  Without `faces` the radial faces are
+    logarithmic from r_in to r_out and theta, phi are uniform over [0, pi] and
+    [0, 2 pi]; cumc3d_amd.ic.read_collapsar_profile gives the faces of a
+    profile directory."""
+    nz = nphi if n_dim == 3 else 1
+    if faces is None:
+        faces = (log_faces(nr, r_in, r_out), None, None)
+    return Config(nx=nr, ny=nth, nz=nz, n_dim=n_dim, coordinate_flag=2,
+                  x_start=r_in, x_end=r_out, y_start=0.0, y_end=math.pi,
+                  z_start=0.0, z_end=2.0 * math.pi, cfl=0.4, ggas=ggas,
+                  riemann=riemann, recon=recon, model="collapsar",
+                  boundary_flag=(1, 1, 3, 3, 0, 0), faces=faces)

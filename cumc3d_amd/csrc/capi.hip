@@ -11,6 +11,7 @@
 
 #include "../../include/cumc3d_hip.h"
 #include "context.hpp"
+#include "gravity.hpp"
 
 using namespace cumc3d;
 
@@ -31,7 +32,7 @@ static int fail(int code, const std::string &msg) {
 // dt = MIN(1e5, cell minimum), clipped to total_time (main.f90:384-389); a
 // step that starts at or past total_time gets dt = 0 and is a no-op in time.
 __global__ void k_dt_finalize(DevScalars *S) {
-    S->active = S->t < S->total_time;
+    S->active = S->t < S->total_time && !S->grav_stop;
     if (!S->active) {
         // keep the CFL minimum of the current state for a later continuation
         S->dt = 0.0;
@@ -45,7 +46,7 @@ __global__ void k_dt_finalize(DevScalars *S) {
 }
 
 __global__ void k_time_advance(DevScalars *S, double *dtlog, int slot) {
-    if (S->t < S->total_time) {
+    if (S->active) {   // the step ran: t < total_time and no model STOP during it
         S->t = S->t + S->dt;
         S->nstep += 1;
     }
@@ -54,7 +55,7 @@ __global__ void k_time_advance(DevScalars *S, double *dtlog, int slot) {
 
 __global__ void k_set_dt(DevScalars *S, double dt) {
     S->dt = dt;
-    S->active = 1;
+    S->active = !S->grav_stop;
 }
 
 // ---------------------------------------------------------------------------
@@ -147,10 +148,89 @@ static void sel_refresh(cumc3d_ctx *x) {
     if (x->timing && x->tstage < TSTAGES) x->sel_timed[x->tstage] = 1;
 }
 
-static void run_stage(cumc3d_ctx *x, int stage) {
+// ---------------------------------------------------------------------------
+// collapsar self-gravity (model/collapsar/src/custom.f90:416-658)
+// ---------------------------------------------------------------------------
+static SorArgs sor_args(cumc3d_ctx *x) {
+    SorArgs a;
+    memset(&a, 0, sizeof(a));
+    a.nx = x->nx; a.ny = x->ny; a.nz = x->nz;
+    a.NX = x->NX; a.NY = x->NY;
+    a.phi = x->phi;
+    a.err = x->sor_err;
+    a.rho = (const double *)x->P[0];
+    a.ajp1 = x->ajp1; a.ajm1 = x->ajm1; a.bkp1 = x->bkp1; a.bkm1 = x->bkm1;
+    a.clp1 = x->clp1; a.clm1 = x->clm1; a.epsc = x->epsc;
+    a.part_nan = x->part_nan;
+    a.part_max = x->part_max;
+    a.nerr = x->sor_nerr;
+    a.relax_max = x->grav_p.relax_max;
+    a.pa = x->grav_p.prim_a[0];
+    a.omega = x->grav_p.omega;
+    a.tol = x->grav_p.tolerance;
+    a.pi4 = 4.0 * (4.0 * std::atan(1.0));   // 4 pi, pi = 4 atan(1) (parameter.h:21)
+    a.S = x->S;
+    return a;
+}
+
+// The RBSOR loop (custom.f90:510-647): sweeps queued in growing batches, the
+// end flag read between batches.  A solve that converged on sweep relax_max
+// sets grav_error (the reference STOPs there); one that ran out continues.
+static int sor_solve(cumc3d_ctx *x, bool zero_guess) {
+    hipStream_t st = x->stream;
+    if (zero_guess) hipLaunchKernelGGL(k_fill0, dim3(nblk(x->G)), dim3(256), 0, st, x->phi, x->G);
+    hipLaunchKernelGGL(k_sor_begin, dim3(1), dim3(1), 0, st, x->S);
+    const SorArgs a = sor_args(x);
+    const long hx = (x->nx + 1) / 2;
+    const dim3 gs(nblk(hx * x->ny * x->nz));
+    const dim3 gp((unsigned)a.nerr + nblk((long)x->ny * x->nz + (long)x->nx * x->nz + (long)x->nx * x->ny));
+    long queued = 0;
+    int batch = 4, flags[3] = {0, 0, 0};
+    for (;;) {
+        for (int i = 0; i < batch; ++i) {
+            hipLaunchKernelGGL(k_sor_sweep<0>, gs, dim3(256), 0, st, a);
+            hipLaunchKernelGGL(k_sor_sweep<1>, gs, dim3(256), 0, st, a);
+            hipLaunchKernelGGL(k_sor_post, gp, dim3(256), 0, st, a);
+        }
+        queued += batch;
+        HIPCHK(hipMemcpyAsync(flags, &x->S->sor_state, sizeof(flags), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (flags[0]) break;
+        if (queued > (long)x->grav_p.relax_max + 1)   // the device caps the sweeps at relax_max
+            return fail(CUMC3D_EHIP, "Poisson solve: sweep count not reported by the device");
+        if (batch < 256) batch *= 2;
+    }
+    if (flags[2] == x->grav_p.relax_max) x->grav_error = 1;
+    return 0;
+}
+
+// UPDATE(p)'s CUSTOM_UPDATE after the stage's update and ghost fill
+static int grav_stage(cumc3d_ctx *x, int stage) {
+    if (x->d.model != CUMC3D_MODEL_COLLAPSAR) return 0;
+    const ArgsT<double> a = make_args<double>(x);
+    const double c = x->grav_p.c_code;
+    if (stage == 1) hipLaunchKernelGGL(k_accrete<1>, dim3(1), dim3(64), 0, x->stream, a, c);
+    else if (stage == 2) hipLaunchKernelGGL(k_accrete<2>, dim3(1), dim3(64), 0, x->stream, a, c);
+    else hipLaunchKernelGGL(k_accrete<3>, dim3(1), dim3(64), 0, x->stream, a, c);
+    if (stage == 3 && x->cur_step % x->grav_p.n_pot == 0) return sor_solve(x, false);
+    return 0;
+}
+
+// after a sync: the reference's STOPs as errors
+static int grav_check(cumc3d_ctx *x) {
+    if (x->d.model != CUMC3D_MODEL_COLLAPSAR) return 0;
+    int stop = 0;
+    HIPCHK(hipMemcpy(&stop, &x->S->grav_stop, sizeof(stop), hipMemcpyDeviceToHost));
+    if (stop) return fail(CUMC3D_ESTOP, "Black hole event horizon exceeds computational grid");
+    if (x->grav_error) return fail(CUMC3D_ESTOP, "Convergence error in poisson solver");
+    return 0;
+}
+
+static int run_stage(cumc3d_ctx *x, int stage) {
     if (stage == 1) sel_refresh(x);
     if (x->fp32) f32::run_stage(x, stage);
     else run_stage_t<double>(x, stage);
+    return grav_stage(x, stage);
 }
 
 // per-kernel-class device times from the recorded events (after a sync)
@@ -193,6 +273,7 @@ const char *cumc3d_error_string(int code) {
         case CUMC3D_ENOMEM: return "device allocation failed";
         case CUMC3D_ESTATE: return "call out of order";
         case CUMC3D_ENONFINITE: return "non-finite or non-positive time step";
+        case CUMC3D_ESTOP: return "the model stopped the run";
         default: return "unknown error";
     }
 }
@@ -212,7 +293,14 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         return fail(CUMC3D_EINVAL, "coordinate_flag must be 0 (Cartesian), 1 (cylindrical) or 2 (spherical)");
     if (d.riemann < 0 || d.riemann > 3) return fail(CUMC3D_EINVAL, "unknown Riemann solver");
     if (d.recon < 0 || d.recon > 4) return fail(CUMC3D_EINVAL, "unknown reconstruction");
-    if (d.model < 0 || d.model > 2) return fail(CUMC3D_EINVAL, "unknown model");
+    if (d.model < 0 || d.model > 3) return fail(CUMC3D_EINVAL, "unknown model");
+    if (d.model == CUMC3D_MODEL_COLLAPSAR) {
+        if (d.coordinate_flag != 2) return fail(CUMC3D_EINVAL, "the collapsar model needs spherical coordinates");
+        if (!d.gravity) return fail(CUMC3D_EINVAL, "the collapsar model needs cumc3d_desc.gravity");
+        if (d.precision != CUMC3D_FP64) return fail(CUMC3D_EINVAL, "the collapsar model runs in FP64 only");
+        if (d.gravity->relax_max < 1 || d.gravity->n_pot < 1)
+            return fail(CUMC3D_EINVAL, "gravity relax_max and n_pot must be positive");
+    }
     if (d.model == CUMC3D_MODEL_TORUS && d.coordinate_flag != 1)
         return fail(CUMC3D_EINVAL, "the torus-cyn model needs cylindrical coordinates");
     if (d.model == CUMC3D_MODEL_TORUS_SPH && d.coordinate_flag != 2)
@@ -242,6 +330,9 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
     cumc3d_ctx *x = new cumc3d_ctx();
     x->d = d;
     x->d.metrics = nullptr;   // caller-owned: read during init only
+    x->d.gravity = nullptr;
+    if (d.model == CUMC3D_MODEL_COLLAPSAR) x->grav_p = *d.gravity;
+    x->xF0 = d.xF[3];
     x->nx = d.nx; x->ny = d.ny; x->nz = d.nz;
     x->NX = d.nx + 6; x->NY = d.ny + 6; x->NZ = d.nz + 6;
     x->G = x->NX * x->NY * x->NZ;
@@ -315,6 +406,40 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         soff = (long)hm.size();
         for (int i = 0; i < 10; ++i) hm.insert(hm.end(), t[i].begin(), t[i].end());
     }
+    // get_poisson / poisson_coef (model/collapsar/src/poisson.f90:6-64) from
+    // the cell centres, in its operation order; libm tan and sin
+    std::vector<double> pc;
+    long pco[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (d.model == CUMC3D_MODEL_COLLAPSAR) {
+        const int nx = d.nx, ny = d.ny, nz = d.nz;
+        const long N = (long)nx * ny * nz, nb = (long)(nx + 1) * (ny + 1);
+        const long len[7] = {nx + 1, nx + 1, nb, nb, N, N, N};
+        for (int i = 0; i < 7; ++i) {
+            pco[i] = (long)pc.size();
+            pc.resize(pc.size() + len[i], 0.0);
+        }
+        double *ajp1 = &pc[pco[0]], *ajm1 = &pc[pco[1]], *bkp1 = &pc[pco[2]], *bkm1 = &pc[pco[3]];
+        double *clp1 = &pc[pco[4]], *clm1 = &pc[pco[5]], *epsc = &pc[pco[6]];
+        const std::vector<double> &X = centres[0], &Y = centres[1], &Z = centres[2];
+        for (int i = 1; i <= nx; ++i)
+            for (int j = 1; j <= ny; ++j)
+                for (int k = 1; k <= nz; ++k) {
+                    const double xm1 = X[i + 2], xc = X[i + 3], xp1 = X[i + 4];
+                    const double ym1 = Y[j + 2], yc = Y[j + 3], yp1 = Y[j + 4];
+                    const double zm1 = Z[k + 2], zc = Z[k + 3], zp1 = Z[k + 4];
+                    const double t = std::tan(yc), sn = std::sin(yc);
+                    const long q = (long)(i - 1) + (long)nx * ((long)(j - 1) + (long)ny * (k - 1));
+                    epsc[q] = 2.0 * (xp1 + xm1 - 3.0 * xc) / xc / (xp1 - xc) / (xc - xm1) +
+                              (yp1 + ym1 - 2.0 * yc - 2.0 * t) / (xc * xc) / t / (yp1 - yc) / (yc - ym1) -
+                              2.0 / (xc * xc) / (sn * sn) / (zp1 - zc) / (zc - zm1);
+                    ajp1[i] = 2.0 * (2.0 * xc - xm1) / (xp1 - xm1) / (xp1 - xc) / xc;
+                    ajm1[i] = 2.0 * (2.0 * xc - xp1) / (xp1 - xm1) / (xc - xm1) / xc;
+                    bkp1[(long)j * (nx + 1) + i] = (2.0 * t + yc - ym1) / (xc * xc) / t / (yp1 - yc) / (yp1 - ym1);
+                    bkm1[(long)j * (nx + 1) + i] = (2.0 * t + yc - yp1) / (xc * xc) / t / (yp1 - ym1) / (yc - ym1);
+                    clp1[q] = 2.0 / (xc * xc) / (sn * sn) / (zp1 - zm1) / (zp1 - zc);
+                    clm1[q] = 2.0 / (xc * xc) / (sn * sn) / (zp1 - zm1) / (zc - zm1);
+                }
+    }
     int rc = 0;
     auto cleanup = [&]() { cumc3d_finalize(x); };
     if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -373,6 +498,7 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
     s0.total_time = 1.0e300;
     s0.active = 1;
     s0.dtmin_bits = (unsigned long long)0x7E37E43C8800759CULL;   // bits of 1e300
+    s0.m_bh = s0.mbh_old = x->grav_p.m_bh;
     if (hipMemcpy(x->S, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess) { cleanup(); return fail(CUMC3D_EHIP, "scalars"); }
     if (const char *mt = getenv("CUMC3D_MARCH_THREADS")) {
         long v = atol(mt);
@@ -382,13 +508,34 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         if (hipMalloc(&x->stale, (size_t)3 * x->G) != hipSuccess) { cleanup(); return fail(CUMC3D_ENOMEM, "stale flags"); }
         if (hipMemset(x->stale, 0, (size_t)3 * x->G) != hipSuccess) { cleanup(); return fail(CUMC3D_EHIP, "memset"); }
     }
+    if (d.model == CUMC3D_MODEL_COLLAPSAR) {
+        // phi (G), error terms (N), the coefficients, the error blocks' partials
+        const long N = (long)d.nx * d.ny * d.nz;
+        x->sor_nerr = (int)nblk(N);
+        const size_t bytes = ((size_t)x->G + N + pc.size() + 2 * (size_t)x->sor_nerr) * sizeof(double);
+        if (hipMalloc(&x->grav, bytes) != hipSuccess) { cleanup(); return fail(CUMC3D_ENOMEM, "gravity arrays"); }
+        if (hipMemset(x->grav, 0, bytes) != hipSuccess) { cleanup(); return fail(CUMC3D_EHIP, "memset"); }
+        double *g = (double *)x->grav;
+        x->phi = g;
+        x->sor_err = g + x->G;
+        double *c0 = x->sor_err + N;
+        if (hipMemcpy(c0, pc.data(), pc.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            cleanup();
+            return fail(CUMC3D_EHIP, "poisson coefficient upload");
+        }
+        x->ajp1 = c0 + pco[0]; x->ajm1 = c0 + pco[1]; x->bkp1 = c0 + pco[2]; x->bkm1 = c0 + pco[3];
+        x->clp1 = c0 + pco[4]; x->clm1 = c0 + pco[5]; x->epsc = c0 + pco[6];
+        x->part_max = c0 + pc.size();
+        x->part_nan = (long long *)(x->part_max + x->sor_nerr);
+    }
     *out = x;
     return rc;
 }
 
 int cumc3d_upload(cumc3d_ctx *x, const double *prim, const double *eps, const double *floors3) {
     if (!x || !prim || !eps) return fail(CUMC3D_EINVAL, "null argument");
-    if (x->d.model != CUMC3D_MODEL_RIEMANN && !floors3) return fail(CUMC3D_EINVAL, "torus models need floor arrays");
+    if ((x->d.model == CUMC3D_MODEL_TORUS || x->d.model == CUMC3D_MODEL_TORUS_SPH) && !floors3)
+        return fail(CUMC3D_EINVAL, "torus models need floor arrays");
     double *tmp = nullptr;
     HIPCHK(hipMalloc(&tmp, (size_t)9 * x->G * sizeof(double)));
     HIPCHK(hipMemcpyAsync(tmp, prim, (size_t)8 * x->G * sizeof(double), hipMemcpyHostToDevice, x->stream));
@@ -421,8 +568,17 @@ int cumc3d_initial_update(cumc3d_ctx *x) {
         hipLaunchKernelGGL(k_init_interior<double>, dim3(gi), dim3(256), 0, x->stream, make_args<double>(x));
         hipLaunchKernelGGL(k_ghost<double>, dim3(nblk(ghost_count(*x))), dim3(256), 0, x->stream, make_args<double>(x));
     }
+    x->nstep_total = 0;
+    x->grav_error = 0;
+    if (x->d.model == CUMC3D_MODEL_COLLAPSAR) {   // UPDATE(0): CUSTOM_UPDATE(0)
+        hipLaunchKernelGGL(k_grav_start, dim3(1), dim3(1), 0, x->stream, x->S, x->grav_p.m_bh, x->grav_p.c_code,
+                           x->xF0);
+        int rc = sor_solve(x, true);
+        if (rc) return rc;
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(x->stream));
+    if (int rc = grav_check(x)) return rc;
     x->initialized = true;
     x->dt_pending = false;
     return 0;
@@ -449,10 +605,15 @@ int cumc3d_finddt(cumc3d_ctx *x, double *dt) {
 int cumc3d_rk3_step(cumc3d_ctx *x, double dt) {
     if (!x) return fail(CUMC3D_EINVAL, "null context");
     if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
+    if (int rc = grav_check(x)) return rc;
     hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, x->stream, x->S, dt);
-    for (int s = 1; s <= 3; ++s) run_stage(x, s);
+    x->cur_step = x->nstep_total;
+    for (int s = 1; s <= 3; ++s)
+        if (int rc = run_stage(x, s)) return rc;
+    x->nstep_total += 1;
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(x->stream));
+    if (int rc = grav_check(x)) return rc;
     // stage 3 reduced the CFL minimum of the new state (FINDDT of the next step)
     x->dt_pending = true;
     return 0;
@@ -487,14 +648,15 @@ int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_st
         HIPCHK(hipEventCreate(&s1));
         HIPCHK(hipEventRecord(s0, x->stream));
     }
-    int n = 0;
+    int n = 0, grc = 0;
     double tcur = *t;
-    for (; n < max_steps; ++n) {
+    for (; n < max_steps && !grc; ++n) {
         if (sync_each_step && !(tcur < total_time)) break;
         // FINDDT: the previous stage 3 already reduced the minimum
         if (!x->dt_pending) launch_finddt(x);
         hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, x->stream, x->S);
-        for (int s = 1; s <= 3; ++s) run_stage(x, s);
+        x->cur_step = x->nstep_total + n;
+        for (int s = 1; s <= 3 && !grc; ++s) grc = run_stage(x, s);
         x->dt_pending = true;
         hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(1), 0, x->stream, x->S, dlog, n);
         if (sync_each_step) {
@@ -518,19 +680,21 @@ int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_st
     HIPCHK(hipMemcpy(&h, x->S, sizeof(h), hipMemcpyDeviceToHost));
     *t = h.t;
     int done = h.nstep;
+    x->nstep_total += done;
     if (steps_done) *steps_done = done;
     if (dts && dlog) {
         HIPCHK(hipMemcpy(dts, dlog, (size_t)(done < max_steps ? done : max_steps) * sizeof(double),
                          hipMemcpyDeviceToHost));
     }
     if (dlog) HIPCHK(hipFree(dlog));
+    if (grc) return grc;
     if (h.nonfinite) {
         h.nonfinite = 0;
         HIPCHK(hipMemcpy(x->S, &h, sizeof(h), hipMemcpyHostToDevice));
         return fail(CUMC3D_ENONFINITE, "a cell's CFL time step was NaN or non-positive (NaN or negative energy in the state)");
     }
     if (!std::isfinite(h.dt) || h.dt < 0.0) return fail(CUMC3D_ENONFINITE, "time step became non-finite");
-    return 0;
+    return grav_check(x);
 }
 
 int cumc3d_download(cumc3d_ctx *x, double *prim, double *eps, double *bcell, double *cs) {
@@ -593,6 +757,19 @@ int cumc3d_profile(cumc3d_ctx *x, double *primitive, double *epsilon, double *bf
     return 0;
 }
 
+int cumc3d_gravity(cumc3d_ctx *x, double *phi, double *m_bh, double *r_sh, int *iterations) {
+    if (!x) return fail(CUMC3D_EINVAL, "null context");
+    if (x->d.model != CUMC3D_MODEL_COLLAPSAR) return fail(CUMC3D_EINVAL, "not a collapsar run");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (phi) HIPCHK(hipMemcpy(phi, x->phi, (size_t)x->G * sizeof(double), hipMemcpyDeviceToHost));
+    DevScalars h;
+    HIPCHK(hipMemcpy(&h, x->S, sizeof(h), hipMemcpyDeviceToHost));
+    if (m_bh) *m_bh = h.m_bh;
+    if (r_sh) *r_sh = h.r_sh;
+    if (iterations) *iterations = h.sor_iter;
+    return 0;
+}
+
 int cumc3d_divb(cumc3d_ctx *x, double *maxdivb) {
     if (!x || !maxdivb) return fail(CUMC3D_EINVAL, "null argument");
     unsigned long long *d = nullptr, h = 0;
@@ -615,6 +792,7 @@ int cumc3d_finalize(cumc3d_ctx *x) {
     if (x->met) hipFree(x->met);
     if (x->S) hipFree(x->S);
     if (x->stale) hipFree(x->stale);
+    if (x->grav) hipFree(x->grav);
     void *sel[] = {x->sel_w, x->sel_m, x->sel_v, x->sel_frag, x->mask, x->samp_feat, x->samp_label,
                    x->samp_counts, x->grad_part, x->loss_part, x->loss_dev};
     for (void *q : sel)

@@ -51,6 +51,19 @@ struct cumc3d_ctx {
     int tstage = 0;               // stages recorded in the current run
     double tsum[8] = {0};
     long tcnt[8] = {0};
+    // collapsar self-gravity (model/collapsar/src/def.f90:16-36): one
+    // allocation holding the potential, the per-cell SOR error terms, the
+    // poisson_coef coefficients and the error blocks' partials
+    cumc3d_gravity_params grav_p{};
+    void *grav = nullptr;
+    double *phi = nullptr, *sor_err = nullptr, *ajp1 = nullptr, *ajm1 = nullptr, *bkp1 = nullptr, *bkm1 = nullptr;
+    double *clp1 = nullptr, *clm1 = nullptr, *epsc = nullptr, *part_max = nullptr;
+    long long *part_nan = nullptr;
+    int sor_nerr = 0;
+    double xF0 = 0.0;             // xF(0), the inner radial face (horizon check)
+    long nstep_total = 0;         // steps since the initial update (main.f90's n_step)
+    long cur_step = 0;            // n_step of the step being queued
+    int grav_error = 0;           // 1: a solve converged on its last allowed sweep (the reference STOPs)
 };
 
 namespace cumc3d {
@@ -255,6 +268,9 @@ template <class W> static ArgsT<W> make_args(cumc3d_ctx *x) {
     a.dcb = x->sph[0]; a.dcb3 = x->sph[1]; a.xbar = x->sph[2]; a.gsrc = x->sph[3]; a.hbx = x->sph[4];
     a.sinf = x->sph[5]; a.dcose = x->sph[6]; a.cot = x->sph[7]; a.hby = x->sph[8]; a.sine = x->sph[9];
     a.fixpole = x->d.coordinate_flag == 2 && x->d.fixpole;
+    a.phi = x->phi;
+    for (int i = 0; i < 4; ++i) a.pa[i] = x->grav_p.prim_a[i];
+    a.eps_a = x->grav_p.eps_a;
     return a;
 }
 

@@ -29,6 +29,15 @@ struct DevScalars {
     int active;                       // 0 once global_time >= total_time: stage kernels exit
     int nonfinite;                    // set when a cell's CFL time step is NaN or not positive
     int pad;
+    // collapsar (model/collapsar/src/def.f90:34-39, custom.f90:416-649)
+    double m_bh, mbh_old, r_sh;       // black-hole mass, its value at step start, Schwarzschild radius
+    double sor_err;                   // abserror of the last SOR sweep pair
+    unsigned sor_ctr;                 // error blocks of the running post kernel that finished
+    int sor_state;                    // 0 while the Poisson solve iterates, 1 once it ended
+    int sor_n;                        // sweeps done in the running solve
+    int sor_iter;                     // the DO variable n when the solve ended
+    int grav_stop;                    // 1: r_sh reached xF(0) (the reference STOPs)
+    int pad2;
 };
 
 struct Axis {
@@ -73,6 +82,9 @@ template <class W> struct ArgsT {
     // 0.5 dy/dcose, sine along theta.  Null on other geometries.
     const double *dcb, *dcb3, *xbar, *gsrc, *hbx;
     const double *sinf, *dcose, *cot, *hby, *sine;
+    // collapsar: potential over the ghosted box, atmosphere prim_a(irho:ivz) and eps_a
+    const double *phi;
+    double pa[4], eps_a;
 };
 
 // Load field[idx] as scalar base + 32-bit byte offset (the global_load saddr
@@ -93,19 +105,28 @@ template <class A> __device__ __forceinline__ long cidx(const A &a, int j, int k
 
 // Ghost-cell source along one axis (BOUNDARYP_NM, boundary.f90:423-745):
 // flag 0 periodic, 1 outflow, >=2 parity mirror.  side: -1 interior, 0 inner, 1 outer.
+// The source is clamped into 1..n: on an axis with fewer cells than ghost
+// layers (the inactive axes of 1D/2D runs, n = 1) the reference's sequential
+// fill copies ghost from ghost, and for periodic and outflow flags the chain
+// ends at the same interior cell; reading that cell directly keeps a
+// one-pass fill free of reads of ghosts it is writing.  (A mirror chain on
+// such an axis reads the previous fill's outer ghost in the reference; those
+// outer layers of an inactive axis feed no computation.)
 __device__ __forceinline__ int gmap(int i, int n, int fin, int fout, int &side) {
+    int r;
     if (i < 1) {
         side = 0;
         int m = 1 - i;
-        return fin == 0 ? n + 1 - m : (fin == 1 ? 1 : m);
-    }
-    if (i > n) {
+        r = fin == 0 ? n + 1 - m : (fin == 1 ? 1 : m);
+    } else if (i > n) {
         side = 1;
         int m = i - n;
-        return fout == 0 ? m : (fout == 1 ? n : n + 1 - m);
+        r = fout == 0 ? m : (fout == 1 ? n : n + 1 - m);
+    } else {
+        side = -1;
+        return i;
     }
-    side = -1;
-    return i;
+    return r < 1 ? 1 : (r > n ? n : r);
 }
 
 // parity factor of variable var for a ghost on (axis, side); constant array
@@ -689,6 +710,13 @@ __device__ __forceinline__ double lB_z(const A &a, int j, int k, double ey0, dou
     return -(xF * ey0 - xFm * ey1) / (x * dx) + (ex0 - ex1) / (x * dy);
 }
 
+// first_derivative of model/collapsar/src/custom.f90:402-409: the
+// three-point derivative at xc on a non-uniform grid
+__device__ __forceinline__ double first_derivative(double xm1, double xc, double xp1, double fm1, double fc, double fp1) {
+    double h2 = xp1 - xc, h1 = xc - xm1;
+    return ((fp1 - fc) * h1 * h1 + (fc - fm1) * h2 * h2) / (h1 * h2 * (h1 + h2));
+}
+
 template <int STAGE>
 __device__ __forceinline__ double rk(double uo, double u, double dt, double L) {
     if constexpr (STAGE == 1) return uo + dt * L;
@@ -697,10 +725,19 @@ __device__ __forceinline__ double rk(double uo, double u, double dt, double L) {
 }
 
 // CUSTOM_CHECKRHO of the model (model/riemann/src/custom.f90:115-121,
-// model/torus-cyn/src/custom.f90:254-271)
+// model/torus-cyn/src/custom.f90:254-271, model/collapsar/src/custom.f90:295-320)
 template <class A> __device__ __forceinline__ void checkrho(const A &a, long c, double &rho, double v[3], double &eps) {
     if (a.model == 0) {
         eps = fmax2(eps, 0.0);
+    } else if (a.model == 3) {   // reset to the atmosphere prim_a, eps_a
+        double diff = rho - a.pa[0];
+        double f = fmax2(fsign1(diff), 0.0);
+        rho = f * rho + (1.0 - f) * a.pa[0];
+        v[0] = f * v[0] + (1.0 - f) * a.pa[1];
+        v[1] = f * v[1] + (1.0 - f) * a.pa[2];
+        v[2] = f * v[2] + (1.0 - f) * a.pa[3];
+        eps = f * eps + (1.0 - f) * a.eps_a;
+        if (eps < 0.0) eps = a.eps_a;
     } else {
         double diff = rho - a.rfl[c];
         double f = fmax2(fsign1(diff), 0.0);
@@ -712,10 +749,11 @@ template <class A> __device__ __forceinline__ void checkrho(const A &a, long c, 
 }
 
 // FINDPRESSURE of the model (model/riemann/src/findpressure.f90:17-28,
-// model/torus-cyn/src/findpressure.f90:97-112)
+// model/torus-cyn/src/findpressure.f90:97-112; the collapsar's,
+// model/collapsar/src/findpressure.f90:27-38, is the ideal gas of riemann's)
 template <class A> __device__ __forceinline__ void pressure(const A &a, long c, double rho, double eps, double &p, double &cs) {
     double dpdrho, dpdeps;
-    if (a.model == 0 || rho > a.rfl[c]) {
+    if (a.model == 0 || a.model == 3 || rho > a.rfl[c]) {
         p = rho * eps * a.g1;
         dpdrho = eps * a.g1;
         dpdeps = rho * a.g1;
@@ -849,7 +887,23 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
         sc[2] = sc[2] + (p + rho * vz * vz + 0.5 * bsq - bz * bz) * g * cot - (rho * vx * vy - bx * by) * g;
         sc[3] = sc[3] - (rho * vx * vz - bx * bz) * g - (rho * vy * vz - by * bz) * g * cot;
     }
-    if (a.model == 2) {   // model/torus-sph/src/custom.f90:283-296
+    if (G == 2 && a.model == 3) {   // model/collapsar/src/custom.f90:361-387
+        // the matter potential's three-point gradient on the non-uniform
+        // grid (first_derivative, :402-409) plus the black hole's pull
+        double diff = rho - a.pa[0];
+        double f = fmax2(fsign1(diff), 0.0);
+        const double *phi = a.phi, *xc = a.ax[0].c, *yc = a.ax[1].c, *zc = a.ax[2].c;
+        const double pc = phi[c];
+        double dphidx = first_derivative(xc[j + 2], xc[j + 3], xc[j + 4], phi[c - 1], pc, phi[c + 1]);
+        double dphidy = first_derivative(yc[k + 2], yc[k + 3], yc[k + 4], phi[c - sy], pc, phi[c + sy]);
+        double dphidz = first_derivative(zc[l + 2], zc[l + 3], zc[l + 4], phi[c - sz], pc, phi[c + sz]);
+        const double xj = xc[j + 3], sk = a.sine[k + 3], rs = a.S->r_sh;
+        dphidx = dphidx + a.S->m_bh / ((xj - rs) * (xj - rs));
+        sc[1] = sc[1] - f * rho * dphidx;
+        sc[2] = sc[2] - f * rho * dphidy / xj;
+        sc[3] = sc[3] - f * rho * dphidz / xj / sk;
+        sc[4] = sc[4] - f * rho * (vx * dphidx + vy * dphidy / xj + vz * dphidz / xj / sk);
+    } else if (a.model == 2) {   // model/torus-sph/src/custom.f90:283-296
         double diff = rho - a.rfl[c];
         double f = fmax2(fsign1(diff), 0.0);
         double xj = a.ax[0].c[j + 3];
@@ -1098,6 +1152,7 @@ template <class W> __global__ __launch_bounds__(256) void k_ghost(ArgsT<W> a) {
             else if (f4 == 1) ey = a.ny;
             else if (f4 >= 2) ey = a.ny + 1 - m;
             else { ey = k; eps_y = false; }
+            if (eps_y) ey = ey < 1 ? 1 : (ey > a.ny ? a.ny : ey);   // as gmap's clamp
         }
         eps_same_cell = (gx || eps_y || gz) && ey == kk;
         if (gx || eps_y || gz) a.eps[c] = a.eps[cidx(a, jj, ey, ll)];

@@ -7,11 +7,13 @@ USE ISO_C_BINDING
 IMPLICIT NONE
 
 INTEGER(C_INT), PARAMETER :: CUMC3D_OK = 0, CUMC3D_EINVAL = 1, CUMC3D_EHIP = 2, &
-                             CUMC3D_ENOMEM = 3, CUMC3D_ESTATE = 4, CUMC3D_ENONFINITE = 5
+                             CUMC3D_ENOMEM = 3, CUMC3D_ESTATE = 4, CUMC3D_ENONFINITE = 5, &
+                             CUMC3D_ESTOP = 6
 INTEGER(C_INT), PARAMETER :: CUMC3D_LF = 0, CUMC3D_HLL = 1, CUMC3D_HLLC = 2, CUMC3D_HLLD = 3
 INTEGER(C_INT), PARAMETER :: CUMC3D_TVDMM = 0, CUMC3D_TVDVL = 1, CUMC3D_TVDMC = 2, &
                              CUMC3D_PPM = 3, CUMC3D_WENO = 4
-INTEGER(C_INT), PARAMETER :: CUMC3D_MODEL_RIEMANN = 0, CUMC3D_MODEL_TORUS = 1, CUMC3D_MODEL_TORUS_SPH = 2
+INTEGER(C_INT), PARAMETER :: CUMC3D_MODEL_RIEMANN = 0, CUMC3D_MODEL_TORUS = 1, CUMC3D_MODEL_TORUS_SPH = 2, &
+                             CUMC3D_MODEL_COLLAPSAR = 3
 INTEGER(C_INT), PARAMETER :: CUMC3D_FP64 = 0, CUMC3D_FP32 = 1
 INTEGER(C_INT), PARAMETER :: CUMC3D_BN_ZERO = 0, CUMC3D_BN_FIELD = 1, CUMC3D_BN_CARRY = 2
 
@@ -21,6 +23,12 @@ TYPE, BIND(C) :: cumc3d_grid_metrics
   TYPE(C_PTR) :: dx_cb, xbar
   TYPE(C_PTR) :: dcose, dsine, sine
   TYPE(C_PTR) :: sinf
+END TYPE
+
+TYPE, BIND(C) :: cumc3d_gravity_params
+  REAL(C_DOUBLE) :: prim_a(4)
+  REAL(C_DOUBLE) :: eps_a, m_bh, c_code, omega, tolerance
+  INTEGER(C_INT) :: relax_max, n_pot
 END TYPE
 
 TYPE, BIND(C) :: cumc3d_desc
@@ -33,6 +41,7 @@ TYPE, BIND(C) :: cumc3d_desc
   TYPE(C_PTR) :: xF, yF, zF
   INTEGER(C_INT) :: fixpole
   TYPE(C_PTR) :: metrics
+  TYPE(C_PTR) :: gravity
 END TYPE
 
 INTERFACE
@@ -80,6 +89,12 @@ INTERFACE
     IMPORT :: C_INT, C_PTR, C_DOUBLE
     TYPE(C_PTR), VALUE :: ctx
     REAL(C_DOUBLE), INTENT(OUT), OPTIONAL :: primitive(*), epsilon(*), bfield(*)
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION cumc3d_gravity(ctx, phi, m_bh, r_sh, iterations) BIND(C, NAME='cumc3d_gravity')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR), VALUE :: ctx
+    REAL(C_DOUBLE), INTENT(OUT), OPTIONAL :: phi(*), m_bh, r_sh
+    INTEGER(C_INT), INTENT(OUT), OPTIONAL :: iterations
   END FUNCTION
   INTEGER(C_INT) FUNCTION cumc3d_divb(ctx, maxdivb) BIND(C, NAME='cumc3d_divb')
     IMPORT :: C_INT, C_PTR, C_DOUBLE

@@ -12,8 +12,13 @@
 !
 ! Build: compile with -cpp next to the solver's modules (and the model's
 ! param.h), with the solver's own -DFIXPOLE, -DCUMC3D_TORUS for
-! model/torus-cyn, -DCUMC3D_TORUS -DCUMC3D_TORUS_SPH for model/torus-sph;
-! link libcumc3d_hip.so.
+! model/torus-cyn, -DCUMC3D_TORUS -DCUMC3D_TORUS_SPH for model/torus-sph,
+! -DCUMC3D_COLLAPSAR for model/collapsar; link libcumc3d_hip.so.
+!
+! The collapsar's CUSTOM_DEF values (prim_a, eps_a, m_bh, vel2code) and its
+! param.h Poisson settings go over in cumc3d_gravity_params; the device runs
+! CUSTOM_UPDATE itself (accretion, horizon check, the SOR potential), and
+! CUMC3D_HIP_SYNC_HOST brings phi, m_bh and r_sh back with the state.
 !
 ! The grid goes over as the module's GETGRID arrays (face coordinates plus
 ! dx, x, dx_cb, dcose, sinf, ...), so CUSTOM_GRID and FIXPOLE (which moves
@@ -26,6 +31,7 @@ TYPE(C_PTR), SAVE :: hip_ctx = C_NULL_PTR
 LOGICAL, SAVE :: hip_attached = .false.
 REAL(C_DOUBLE), ALLOCATABLE, TARGET, SAVE :: hip_grid(:)
 TYPE(cumc3d_grid_metrics), TARGET, SAVE :: hip_metrics
+TYPE(cumc3d_gravity_params), TARGET, SAVE :: hip_gravity
 
 CONTAINS
 
@@ -33,7 +39,7 @@ CONTAINS
 ! module state (setup.f90:340-395 allocations, parameter.h flags).
 SUBROUTINE CUMC3D_HIP_ATTACH
   USE DEFINITION
-#ifdef CUMC3D_TORUS
+#if defined(CUMC3D_TORUS) || defined(CUMC3D_COLLAPSAR)
   USE CUSTOM_DEF
 #endif
   IMPLICIT NONE
@@ -61,7 +67,23 @@ SUBROUTINE CUMC3D_HIP_ATTACH
   d%precision = CUMC3D_FP64
   d%cfl = cfl
   d%ggas = ggas
-#if defined(CUMC3D_TORUS_SPH)
+  d%gravity = C_NULL_PTR
+#if defined(CUMC3D_COLLAPSAR)
+  d%model = CUMC3D_MODEL_COLLAPSAR
+  d%rt_gravity = 0
+  d%r_sh = 0.0D0
+  ! CUSTOM_DEF after GET_MODEL (model/collapsar/src/model.f90:44-52,238-245)
+  ! and param.h:37-44; omega_weight is def.f90:31
+  hip_gravity%prim_a = prim_a(irho:ivz)
+  hip_gravity%eps_a = eps_a
+  hip_gravity%m_bh = m_bh
+  hip_gravity%c_code = clight*vel2code
+  hip_gravity%omega = omega_weight
+  hip_gravity%tolerance = tolerance
+  hip_gravity%relax_max = relax_max
+  hip_gravity%n_pot = n_pot
+  d%gravity = C_LOC(hip_gravity)
+#elif defined(CUMC3D_TORUS_SPH)
   d%model = CUMC3D_MODEL_TORUS_SPH
   d%rt_gravity = 0
   d%r_sh = r_sh
@@ -146,13 +168,20 @@ SUBROUTINE CUMC3D_HIP_ATTACH
   hip_attached = .true.
 END SUBROUTINE
 
-! prim, epsilon, bcell and cs of the host modules from the device state.
+! prim, epsilon, bcell and cs of the host modules from the device state
+! (for the collapsar also phi, m_bh and r_sh of CUSTOM_DEF).
 SUBROUTINE CUMC3D_HIP_SYNC_HOST
   USE DEFINITION
   USE MHD_MODULE, ONLY : bcell
+#ifdef CUMC3D_COLLAPSAR
+  USE CUSTOM_DEF, ONLY : phi, m_bh, r_sh
+#endif
   IMPLICIT NONE
   IF (.NOT. hip_attached) RETURN
   CALL cumc3d_check(cumc3d_download(hip_ctx, prim, epsilon, bcell, cs), 'cumc3d_download')
+#ifdef CUMC3D_COLLAPSAR
+  CALL cumc3d_check(cumc3d_gravity(hip_ctx, phi, m_bh, r_sh), 'cumc3d_gravity')
+#endif
 END SUBROUTINE
 
 SUBROUTINE CUMC3D_HIP_DETACH

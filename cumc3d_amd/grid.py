@@ -52,6 +52,10 @@ class Axis:
 class Grid:
     def __init__(self, cfg: Config, xfaces=None, yfaces=None, zfaces=None):
         self.cfg = cfg
+        if cfg.faces is not None:                              # CUSTOM_GRID faces
+            xfaces = xfaces if xfaces is not None else cfg.faces[0]
+            yfaces = yfaces if yfaces is not None else cfg.faces[1]
+            zfaces = zfaces if zfaces is not None else cfg.faces[2]
         self.x = Axis(cfg.x_start, cfg.x_end, cfg.nx, xfaces)
         self.y = Axis(cfg.y_start, cfg.y_end, cfg.ny, yfaces)
         self.z = Axis(cfg.z_start, cfg.z_end, cfg.nz, zfaces)

@@ -8,6 +8,7 @@ ghosted box (-2..n+3 per axis, solver/mhd/setup.f90:348-351):
                             (prim(ibx..ibz), solver/mhd/fromrvetou.f90:54-56)
     eps                     specific internal energy (epsilon)
     rho_floor, p_floor, eps_floor   (torus models only, model/torus-cyn/src/def.f90)
+    gravity                 (collapsar only) prim_a(1:8), eps_a, m_bh, vel2code
 
 The generators build the five measurement workloads; the Brio-Wu tube follows
 model/riemann/src/riemann1d.f90:229-250.  Random perturbations use a counter
@@ -17,7 +18,9 @@ sees the same numbers.
 ``write_ic`` emits the file read by the reference oracle's file-fed GET_MODEL
 (oracle/ref_harness/ic_plain.f90) and by the C-ABI upload path: a stream of
 int32[6] header, float64[2] (ggas, total_time), prim in Fortran order
-(8, nx+6, ny+6, nz+6), epsilon, and optionally the three floor arrays.
+(8, nx+6, ny+6, nz+6), epsilon, then the three floor arrays when bit 0 of
+header[5] is set and the collapsar's gravity record (prim_a(1:8), eps_a,
+m_bh, vel2code) when bit 1 is.
 """
 from __future__ import annotations
 
@@ -30,6 +33,8 @@ from .grid import Grid
 
 PRIM_ORDER = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz")
 MAGIC = 0x4D433344  # 'MC3D'
+# st["gravity"] of the collapsar: prim_a(1:8), eps_a, m_bh, vel2code
+GRAVITY_LEN = 11
 
 
 # ---------------------------------------------------------------------------
@@ -70,16 +75,18 @@ def empty_state(cfg: Config) -> dict:
 # ---------------------------------------------------------------------------
 
 def write_ic(path: str, cfg: Config, st: dict) -> None:
-    has_floor = int("rho_floor" in st)
+    has_floor = int("rho_floor" in st) + 2 * int("gravity" in st)
     prim = np.stack([st[n] for n in PRIM_ORDER], axis=-1)      # (nz+6, ny+6, nx+6, 8)
     with open(path, "wb") as f:
         np.array([MAGIC, 1, cfg.nx, cfg.ny, cfg.nz, has_floor], dtype=np.int32).tofile(f)
         np.array([cfg.ggas, cfg.total_time], dtype=np.float64).tofile(f)
         np.ascontiguousarray(prim, dtype=np.float64).tofile(f)
         np.ascontiguousarray(st["eps"], dtype=np.float64).tofile(f)
-        if has_floor:
+        if has_floor & 1:
             for n in ("rho_floor", "p_floor", "eps_floor"):
                 np.ascontiguousarray(st[n], dtype=np.float64).tofile(f)
+        if has_floor & 2:
+            np.ascontiguousarray(st["gravity"], dtype=np.float64).tofile(f)
 
 
 def read_ic(path: str):
@@ -94,9 +101,11 @@ def read_ic(path: str):
         prim = np.fromfile(f, dtype=np.float64, count=8 * g).reshape(shape + (8,))
         st = {n: np.ascontiguousarray(prim[..., i]) for i, n in enumerate(PRIM_ORDER)}
         st["eps"] = np.fromfile(f, dtype=np.float64, count=g).reshape(shape)
-        if has_floor:
+        if has_floor & 1:
             for n in ("rho_floor", "p_floor", "eps_floor"):
                 st[n] = np.fromfile(f, dtype=np.float64, count=g).reshape(shape)
+        if has_floor & 2:
+            st["gravity"] = np.fromfile(f, dtype=np.float64, count=GRAVITY_LEN)
     return (nx, ny, nz, float(ggas), float(total_time)), st
 
 
@@ -440,6 +449,113 @@ def sph_blast(cfg: Config, seed: int = 20261202, centre=(1.0, 0.35, 1.0), radius
     return st
 
 
+# model/collapsar/profile/par.dat: ggas, black-hole mass (code units) and the
+# cgs-to-code factors of mass, length and time (G = c = M_sun = 1)
+COLLAPSAR_UNITS = {"m_bh": 1.9027781052250767, "mass": 5.028992139685286e-34,
+                   "length": 6.771995159595613e-06, "time": 203019.30744592712}
+CLIGHT = 2.99792458e10                                       # model/collapsar/src/param.h:10
+
+
+def read_collapsar_profile(path: str) -> dict:
+    """The collapsar's profile directory (model/collapsar/profile): face
+    coordinates r_grid.dat, th_grid.dat, phi_grid.dat (one value per line,
+    -3..n+3; model/collapsar/src/custom.f90:68-155), the radial rho, p of
+    hydro.dat over -2..nx+3 and par.dat (ggas, m_bh, mass, length and time
+    factors; model/collapsar/src/model.f90:44-99)."""
+    import os
+
+    def col(name, ncol=1):
+        with open(os.path.join(path, name)) as f:
+            rows = [ln.split() for ln in f if ln.strip()]
+        return np.array([[float(v) for v in r[:ncol]] for r in rows])
+
+    par = col("par.dat", 5)[0]
+    hydro = col("hydro.dat", 2)
+    out = {"xF": col("r_grid.dat")[:, 0], "yF": col("th_grid.dat")[:, 0], "rho": hydro[:, 0], "p": hydro[:, 1],
+           "ggas": float(par[0]), "m_bh": float(par[1]), "mass": float(par[2]), "length": float(par[3]),
+           "time": float(par[4])}
+    if os.path.exists(os.path.join(path, "phi_grid.dat")):
+        out["zF"] = col("phi_grid.dat")[:, 0]
+    if len(out["rho"]) != len(out["xF"]) - 1:
+        raise ValueError("hydro.dat needs nx+6 rows for nx+7 radial faces")
+    return out
+
+
+def collapsar(cfg: Config, seed: int = 20261202, profile: dict | None = None, rho_env: float = 2.0e-11,
+              rho_atm: float = 2.6656378907450677e-15, k_env: float = 1.2, k_atm: float = 2.6,
+              slope: float = 2.5, cavity: float = 3.0, b_gauss: float = 1.0e10, w_cgs: float = 10.0,
+              noise: float = 1.0e-4) -> dict:
+    """Initial model of model/collapsar/src/model.f90:6-252 on cfg's grid:
+    the radial rho, p profile copied to every (theta, phi); epsilon =
+    p/rho/(ggas-1) taken before the density noise; differential rotation
+    v_phi = w_0 s_core^2/(s_core^2 + s^2) s (rotation_rule 2) where rho
+    exceeds the atmosphere rho(nx), with s_core the first cell above it; the
+    uniform field of A_phi = xF sinf/2 scaled to b_0 (1e10 G in code units);
+    1e-4 relative density noise in the star.  The atmosphere prim_a = (rho,
+    0, 0, 0, p) of cell nx, eps_a, m_bh and vel2code go into st["gravity"].
+
+    `profile` is a read_collapsar_profile() dict on cfg's faces; without it the
+    radial profile is synthetic: an envelope rho_env (r/r_c)^-slope with
+    p = k_env rho^(4/3) from r_c = cavity * xF(0) out to where it meets the
+    atmosphere rho_atm (p = k_atm rho_atm^(4/3)), atmosphere inside r_c."""
+    assert cfg.model == "collapsar" and cfg.coordinate_flag == 2
+    g = Grid(cfg)
+    m = g.spherical()
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    gam = cfg.ggas
+    units = dict(COLLAPSAR_UNITS)
+    if profile is not None:
+        rho1, p1 = np.asarray(profile["rho"], float), np.asarray(profile["p"], float)
+        units.update({k: profile[k] for k in ("m_bh", "mass", "length", "time")})
+    else:
+        r = g.x.c                                            # x(-2..nx+3)
+        r_c = cavity * g.x.faces[3]
+        env = rho_env * (r / r_c) ** (-slope)
+        star = (r >= r_c) & (env > rho_atm)
+        star[nx + 2:] = False                                # cell nx and beyond: atmosphere
+        rho1 = np.where(star, env, rho_atm)
+        p1 = np.where(star, k_env * rho1 ** (4.0 / 3.0), k_atm * rho_atm ** (4.0 / 3.0))
+    if rho1.shape != (nx + 6,):
+        raise ValueError("the radial profile needs nx+6 values (-2..nx+3)")
+    vel2code = units["length"] / units["time"]
+    gauss2code = (units["mass"] / units["length"]) ** 0.5 / units["time"]
+    st = empty_state(cfg)
+    st["rho"][:] = rho1[None, None, :]
+    st["p"][:] = p1[None, None, :]
+    st["eps"][:] = st["p"] / st["rho"] / (gam - 1.0)
+    rho_nx = rho1[nx + 2]
+    above = rho1 > rho_nx
+    s_core = None
+    for j in range(1, nx + 1):                               # model.f90:106-112
+        if rho1[j + 1] <= rho_nx and rho1[j + 2] > rho_nx:
+            s_core = g.x.c[j + 2]
+            break
+    I = (slice(3, nz + 3), slice(3, ny + 3), slice(3, nx + 3))
+    if s_core is not None:
+        w_0 = w_cgs / units["time"]
+        s_eq = g.x.c[3:nx + 3][None, :] * m["sine"][4:ny + 4][:, None]
+        omega = w_0 * s_core ** 2 / (s_core ** 2 + s_eq ** 2)
+        st["vz"][I] = np.where(above[3:nx + 3][None, :], omega * s_eq, 0.0)[None]
+    xF = g.x.faces[2:nx + 4]                                 # xF(-1..nx)
+    sinf = m["sinf"][2:ny + 4]                               # sinf(-1..ny)
+    ac = xF[None, :] * sinf[:, None] * 0.5                   # A_phi(j,k) = xF(j) sinf(k)/2
+    bx, by = _sph_face_field(cfg, g, m, ac)
+    b_0 = b_gauss * gauss2code
+    F = (slice(2, nz + 3), slice(2, ny + 3), slice(2, nx + 3))
+    st["bx"][F] = bx * b_0
+    st["by"][F] = by * b_0
+    u = interior_uniform(cfg, seed)
+    r3 = st["rho"]
+    inside = np.zeros(cfg.shape, bool)
+    inside[I] = True
+    st["rho"] = np.where(inside & (r3 > rho_nx), r3 * (1.0 + noise * (u - 0.5) / 0.5), r3)
+    prim_a = np.zeros(8)
+    prim_a[0], prim_a[4] = rho_nx, p1[nx + 2]
+    eps_a = prim_a[4] / prim_a[0] / (gam - 1.0)
+    st["gravity"] = np.concatenate([prim_a, [eps_a, units["m_bh"], vel2code]])
+    return st
+
+
 GENERATORS = {
     "briowu": brio_wu,
     "orszag_tang": orszag_tang,
@@ -448,4 +564,5 @@ GENERATORS = {
     "torus": torus,
     "torus_sph": torus_sph,
     "sph_blast": sph_blast,
+    "collapsar": collapsar,
 }

@@ -26,7 +26,8 @@ PD = C.POINTER(C.c_double)
 
 # symbols declared in include/cumc3d_hip.h
 EXPORTS = ("cumc3d_init", "cumc3d_upload", "cumc3d_initial_update", "cumc3d_finddt",
-           "cumc3d_rk3_step", "cumc3d_run", "cumc3d_download", "cumc3d_profile", "cumc3d_divb",
+           "cumc3d_rk3_step", "cumc3d_run", "cumc3d_download", "cumc3d_profile", "cumc3d_gravity",
+           "cumc3d_divb",
            "cumc3d_finalize", "cumc3d_error_string", "cumc3d_last_error",
            "cumc3d_set_timing", "cumc3d_get_timing",
            "cumc3d_selector_init", "cumc3d_selector_set_params", "cumc3d_selector_get_params",
@@ -44,7 +45,28 @@ class Desc(C.Structure):
                 ("riemann", C.c_int), ("recon", C.c_int), ("model", C.c_int),
                 ("rt_gravity", C.c_int), ("hlld_bn_mode", C.c_int), ("precision", C.c_int),
                 ("cfl", C.c_double), ("ggas", C.c_double), ("r_sh", C.c_double),
-                ("xF", PD), ("yF", PD), ("zF", PD), ("fixpole", C.c_int), ("metrics", C.c_void_p)]
+                ("xF", PD), ("yF", PD), ("zF", PD), ("fixpole", C.c_int), ("metrics", C.c_void_p),
+                ("gravity", C.c_void_p)]
+
+
+class GravityParams(C.Structure):
+    """cumc3d_gravity_params: the collapsar's CUSTOM_DEF values after GET_MODEL."""
+    _fields_ = [("prim_a", C.c_double * 4), ("eps_a", C.c_double), ("m_bh", C.c_double),
+                ("c_code", C.c_double), ("omega", C.c_double), ("tolerance", C.c_double),
+                ("relax_max", C.c_int), ("n_pot", C.c_int)]
+
+
+def gravity_params(cfg: Config, gravity) -> GravityParams:
+    """From a state's "gravity" record (prim_a(1:8), eps_a, m_bh, vel2code)."""
+    from .ic import CLIGHT
+    g = GravityParams()
+    for i in range(4):
+        g.prim_a[i] = float(gravity[i])
+    g.eps_a, g.m_bh = float(gravity[8]), float(gravity[9])
+    g.c_code = CLIGHT * float(gravity[10])          # clight*vel2code (model/collapsar/src/custom.f90:481)
+    g.omega, g.tolerance = cfg.sor_omega, cfg.sor_tolerance
+    g.relax_max, g.n_pot = cfg.relax_max, cfg.n_pot
+    return g
 
 
 class CumcError(RuntimeError):
@@ -71,6 +93,7 @@ def load_library(path: str = LIB_PATH):
     L.cumc3d_run.argtypes = [vp, C.c_int, C.c_double, C.c_int, PD, PD, C.POINTER(C.c_int)]
     L.cumc3d_download.argtypes = [vp, PD, PD, PD, PD]
     L.cumc3d_profile.argtypes = [vp, PD, PD, PD]
+    L.cumc3d_gravity.argtypes = [vp, PD, PD, PD, C.POINTER(C.c_int)]
     L.cumc3d_divb.argtypes = [vp, PD]
     L.cumc3d_finalize.argtypes = [vp]
     L.cumc3d_error_string.argtypes = [C.c_int]
@@ -106,11 +129,24 @@ def _check(rc: int):
 
 
 class Solver:
-    """One device context for a configuration."""
+    """One device context for a configuration.  The collapsar model needs the
+    gravity record of its initial model (``state["gravity"]``) at init: pass
+    it as ``gravity`` or the context is created by ``initial_model``."""
 
-    def __init__(self, cfg: Config, hlld_bn_mode: str = "carry", grid: Grid | None = None):
+    def __init__(self, cfg: Config, hlld_bn_mode: str = "carry", grid: Grid | None = None, gravity=None):
         self.cfg = cfg
         self.grid = grid or Grid(cfg)
+        self._bn_mode = hlld_bn_mode
+        self._h = None
+        self.t = 0.0
+        self.nstep = 0
+        if cfg.model == "collapsar" and gravity is None:
+            return
+        self._create(gravity)
+
+    def _create(self, gravity):
+        cfg = self.cfg
+        hlld_bn_mode = self._bn_mode
         L = load_library()
         self._faces = [np.ascontiguousarray(a.faces) for a in (self.grid.x, self.grid.y, self.grid.z)]
         d = Desc()
@@ -128,11 +164,13 @@ class Solver:
         d.xF, d.yF, d.zF = (_p(f) for f in self._faces)
         d.fixpole = int(cfg.fixpole)
         d.metrics = None
+        self._gp = None
+        if cfg.model == "collapsar":
+            self._gp = gravity_params(cfg, gravity)
+            d.gravity = C.cast(C.pointer(self._gp), C.c_void_p)
         h = C.c_void_p()
         _check(L.cumc3d_init(C.byref(d), C.byref(h)))
         self._h = h
-        self.t = 0.0
-        self.nstep = 0
 
     def close(self):
         h = getattr(self, "_h", None)
@@ -150,6 +188,8 @@ class Solver:
     def initial_model(self, state: dict):
         """Upload GET_MODEL output (prim, epsilon, floors) to the device."""
         cfg = self.cfg
+        if self._h is None:
+            self._create(state["gravity"])
         prim = np.ascontiguousarray(np.stack([state[n] for n in PRIM_ORDER], axis=-1), dtype=np.float64)
         if prim.shape != cfg.shape + (8,):
             raise ValueError(f"state arrays must have shape {cfg.shape}")
@@ -186,6 +226,14 @@ class Solver:
         self.t = t.value
         self.nstep += n.value
         return dts[:n.value]
+
+    def gravity(self) -> dict:
+        """Collapsar: potential phi over the ghosted box, m_bh, r_sh and the
+        iteration count of the last Poisson solve."""
+        phi = np.empty(self.cfg.shape)
+        m, r, it = C.c_double(), C.c_double(), C.c_int()
+        _check(load_library().cumc3d_gravity(self._h, _p(phi), C.byref(m), C.byref(r), C.byref(it)))
+        return {"phi": phi, "m_bh": m.value, "r_sh": r.value, "iterations": it.value}
 
     def find_divb(self) -> float:
         m = C.c_double()

@@ -30,15 +30,18 @@ enum {
     CUMC3D_EHIP = 2,        /* HIP runtime error */
     CUMC3D_ENOMEM = 3,      /* device allocation failed */
     CUMC3D_ESTATE = 4,      /* call out of order (e.g. step before upload) */
-    CUMC3D_ENONFINITE = 5   /* dt not finite or not positive: the state holds NaN/negative energy */
+    CUMC3D_ENONFINITE = 5,  /* dt not finite or not positive: the state holds NaN/negative energy */
+    CUMC3D_ESTOP = 6        /* the model stopped the run where the reference STOPs (collapsar:
+                               horizon beyond the inner face, model/collapsar/src/custom.f90:482-484;
+                               Poisson solve converged on its last sweep, :644-647) */
 };
 
 /* Riemann solvers: parameter.h:91-101 (LF_flag, HLL_flag, HLLC_flag, HLLD_flag) */
 enum { CUMC3D_LF = 0, CUMC3D_HLL = 1, CUMC3D_HLLC = 2, CUMC3D_HLLD = 3 };
 /* Reconstructions: parameter.h:103-115 (tvdmm, tvdvl, tvdmc, ppmc, weno flags) */
 enum { CUMC3D_TVDMM = 0, CUMC3D_TVDVL = 1, CUMC3D_TVDMC = 2, CUMC3D_PPM = 3, CUMC3D_WENO = 4 };
-/* In-step model hooks: model/riemann, model/torus-cyn, model/torus-sph */
-enum { CUMC3D_MODEL_RIEMANN = 0, CUMC3D_MODEL_TORUS = 1, CUMC3D_MODEL_TORUS_SPH = 2 };
+/* In-step model hooks: model/riemann, model/torus-cyn, model/torus-sph, model/collapsar */
+enum { CUMC3D_MODEL_RIEMANN = 0, CUMC3D_MODEL_TORUS = 1, CUMC3D_MODEL_TORUS_SPH = 2, CUMC3D_MODEL_COLLAPSAR = 3 };
 /* Arithmetic of the spatial operator */
 enum { CUMC3D_FP64 = 0, CUMC3D_FP32 = 1 };
 /* Value of HLLD's unassigned usstar(ibn) (rsolver_module.f90:704-741):
@@ -63,6 +66,21 @@ typedef struct cumc3d_grid_metrics {
     const double *sinf;                  /* sinf(-3:ny+3) */
 } cumc3d_grid_metrics;
 
+/* The collapsar's self-gravity and black hole (module CUSTOM_DEF,
+ * model/collapsar/src/def.f90:6-46, and param.h:37-44), as GET_MODEL leaves
+ * them (model/collapsar/src/model.f90:44-52,239-242).  Read during
+ * cumc3d_init only. */
+typedef struct cumc3d_gravity_params {
+    double prim_a[4];           /* atmosphere prim_a(irho:ivz): the floor CUSTOM_CHECKRHO resets to */
+    double eps_a;               /* atmosphere epsilon eps_a */
+    double m_bh;                /* black-hole mass m_bh at the start */
+    double c_code;              /* clight*vel2code: r_sh = 2 m_bh/c_code^2 (custom.f90:481) */
+    double omega;               /* SOR weight omega_weight (def.f90:31, 1.90) */
+    double tolerance;           /* SOR relative tolerance (param.h:44, 1e-10) */
+    int relax_max;              /* SOR sweep cap (param.h:41, 100000) */
+    int n_pot;                  /* potential solved every n_pot steps (param.h:38, 10) */
+} cumc3d_gravity_params;
+
 typedef struct cumc3d_desc {
     int nx, ny, nz;             /* parameter.h:70-72 */
     int n_dim;                  /* parameter.h:37 */
@@ -82,6 +100,7 @@ typedef struct cumc3d_desc {
                                    0 and pi with sinf = 0 (getgrid.f90:91-98) and the radial edge
                                    EMFs on them averaged over phi (mhd_module.f90:437-461) */
     const cumc3d_grid_metrics *metrics;   /* optional, see cumc3d_grid_metrics */
+    const cumc3d_gravity_params *gravity; /* CUMC3D_MODEL_COLLAPSAR only (FP64, spherical) */
 } cumc3d_desc;
 
 typedef struct cumc3d_ctx cumc3d_ctx;
@@ -96,7 +115,9 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out);
  * models, NULL otherwise) to the device. */
 int cumc3d_upload(cumc3d_ctx *ctx, const double *prim, const double *eps, const double *floors3);
 
-/* INITIAL_UPDATE (initial.f90:86-116): floors, cons, BOUNDARY, UPDATE(0). */
+/* INITIAL_UPDATE (initial.f90:86-116): floors, cons, BOUNDARY, UPDATE(0).
+ * For the collapsar UPDATE(0) includes CUSTOM_UPDATE(0): r_sh from m_bh and
+ * the potential solved from a zero guess (model/collapsar/src/custom.f90:481-649). */
 int cumc3d_initial_update(cumc3d_ctx *ctx);
 
 /* FINDDT (rungekutta.f90:221-332): CFL time step, returned to the host. */
@@ -127,6 +148,14 @@ int cumc3d_download(cumc3d_ctx *ctx, double *prim, double *eps, double *bcell, d
  * sections of dataio.f90:26,220,196; libcumc3d_io writes the file
  * (include/cumc3d_io.h). */
 int cumc3d_profile(cumc3d_ctx *ctx, double *primitive, double *epsilon, double *bfield);
+
+/* The collapsar's gravity state (module CUSTOM_DEF, model/collapsar/src/def.f90:16-36):
+ * phi(-2:nx+3, -2:ny+3, -2:nz+3) in Fortran order (cells 0..n+1 are the
+ * solved potential and its boundary values, the outer ghost layers are 0),
+ * m_bh and r_sh after the last UPDATE, and the DO variable n of the last
+ * Poisson solve (relax_max + 1 when it ran out without converging).  Any
+ * pointer may be NULL; CUMC3D_EINVAL on other models. */
+int cumc3d_gravity(cumc3d_ctx *ctx, double *phi, double *m_bh, double *r_sh, int *iterations);
 
 /* FIND_DIVB (mhd_module.f90:110-201): max |div B| over the interior. */
 int cumc3d_divb(cumc3d_ctx *ctx, double *maxdivb);

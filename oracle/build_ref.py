@@ -12,7 +12,9 @@ Recipe (test infrastructure only):
     -DNOT=.NOT. maps it to the standard operator without editing the file;
   * main.f90 (HDF5 output) and dataio.f90 are replaced by the harness driver
     oracle/ref_harness/driver.f90, and the model's GET_MODEL by a file-fed
-    initial model (oracle/ref_harness/ic_plain.f90 / ic_torus.f90);
+    initial model (oracle/ref_harness/ic_plain.f90 / ic_torus.f90 /
+    ic_collapsar.f90); the collapsar's CUSTOM_GRID reads the case's faces
+    from ./profile/*_grid.dat, which oracle/refio.py writes next to the run;
   * flang -O2 -ffp-contract=off -fopenmp, as planned for the CPU baseline.
 Outputs: oracle/_ref/<case>/cumc3d_ref.
 
@@ -47,7 +49,7 @@ HIPLIB_DIR = os.path.join(REPO, "cumc3d_amd")
 DROPIN_SRC = [os.path.join(REPO, "cumc3d_amd", "fortran", f)
               for f in ("cumc3d_hip.f90", "rungekutta_hip.f90", "cumc3d_io.f90", "dataio_hip.f90")]
 DROPIN_CASES = ("c1_briowu", "ot_hlld_weno", "ot_bc_mixed", "torus_hlld_ppm", "ot2d_hlld_ppm",
-                "sph_torus_ppm", "sph_blast_weno")
+                "sph_torus_ppm", "sph_blast_weno", "collapsar2d_ppm", "collapsar_ppm")
 FFLAGS = ["-cpp", "-O2", "-ffp-contract=off", "-fopenmp", "-DDIVB", "-DFIXPOLE"]
 
 # solver/mhd/source.list minus main.f90, dataio.f90 (HDF5) and openacc.f90
@@ -62,6 +64,8 @@ MODEL_SRC = {
     "torus": ("model/torus-cyn/src", ["def.f90", "custom.f90", "findpressure.f90",
                                       "getepsilon.f90", "soundspeed.f90"]),
     "torus_sph": ("model/torus-sph/src", ["def.f90", "custom.f90", "findpressure.f90",
+                                          "getepsilon.f90", "soundspeed.f90"]),
+    "collapsar": ("model/collapsar/src", ["def.f90", "custom.f90", "poisson.f90", "findpressure.f90",
                                           "getepsilon.f90", "soundspeed.f90"]),
 }
 TORUS_MODELS = ("torus", "torus_sph")
@@ -126,6 +130,29 @@ def torus_param_h(cfg) -> str:
         ""])
 
 
+def collapsar_param_h(cfg) -> str:
+    # Names and values of model/collapsar/src/param.h; the Poisson settings
+    # (n_pot, relax_max, tolerance) come from the configuration.  The unit
+    # and rotation settings only feed GET_MODEL, which the file-fed initial
+    # model replaces; the SOR weight omega_weight is a PARAMETER of
+    # model/collapsar/src/def.f90:31 (1.90), which the configuration must match.
+    if cfg.sor_omega != 1.90:
+        raise ValueError("the reference's SOR weight is fixed at 1.90 (def.f90:31)")
+    return "\n".join([
+        "! collapsar model parameters for the oracle build",
+        "REAL*8, PARAMETER :: clight = 2.99792458D10",
+        "REAL*8 :: max_alv = 5.0d0*clight",
+        "INTEGER, PARAMETER :: rotation_rule = 2",
+        "REAL*8 :: s_0 = (5.0D3)*(1.0D5)",
+        "REAL*8 :: w_0 = 10.0D0",
+        "REAL*8 :: b_0 = 1.0D10",
+        "REAL*8 :: v_crit = 0.01D0*clight",
+        f"INTEGER, PARAMETER :: n_pot = {cfg.n_pot}",
+        f"INTEGER, PARAMETER :: relax_max = {cfg.relax_max}",
+        f"REAL*8, PARAMETER :: tolerance = {fort_real(cfg.sor_tolerance)}",
+        ""])
+
+
 def run(cmd, cwd):
     r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -145,6 +172,9 @@ def build_case(name: str) -> str:
     if cfg.model in TORUS_MODELS:
         with open(os.path.join(work, "param.h"), "w") as f:
             f.write(torus_param_h(cfg))
+    elif cfg.model == "collapsar":
+        with open(os.path.join(work, "param.h"), "w") as f:
+            f.write(collapsar_param_h(cfg))
     else:
         # model/riemann/src/custom.f90 includes param.h for test_model; 0 selects
         # no extra gravity source (custom.f90:143).
@@ -152,8 +182,9 @@ def build_case(name: str) -> str:
             f.write("INTEGER, PARAMETER :: test_model = 0\n")
     srcs = [(os.path.join(REF, "solver/mhd", s), s) for s in SOLVER]
     srcs += [(os.path.join(REF, mdir, s), "model_" + s) for s in msrc]
-    harness = ["harness_io.f90", "ic_torus.f90" if cfg.model in TORUS_MODELS else "ic_plain.f90",
-               "driver.f90"]
+    ic_src = {"torus": "ic_torus.f90", "torus_sph": "ic_torus.f90", "collapsar": "ic_collapsar.f90"}
+    harness = ["harness_io.f90", ic_src.get(cfg.model, "ic_plain.f90"), "driver.f90"]
+    dflags = ["-DCUMC3D_COLLAPSAR"] if cfg.model == "collapsar" else []
     objs = []
     for path, link in srcs:
         if not os.path.isfile(path):
@@ -165,7 +196,8 @@ def build_case(name: str) -> str:
         objs.append(obj)
     for h in harness:
         obj = h.replace(".f90", ".o")
-        run([FLANG, *FFLAGS, "-c", os.path.join(HERE, "ref_harness", h), "-o", obj], work)
+        run([FLANG, *FFLAGS, *(dflags if h == "driver.f90" else []), "-c", os.path.join(HERE, "ref_harness", h),
+             "-o", obj], work)
         objs.append(obj)
     outdir = os.path.join(OUT, name)
     os.makedirs(outdir, exist_ok=True)
@@ -174,19 +206,20 @@ def build_case(name: str) -> str:
          f"-Wl,-rpath,{libdir}"], work)
     if name in DROPIN_CASES and all(os.path.exists(os.path.join(HIPLIB_DIR, f))
                                     for f in ("libcumc3d_hip.so", "libcumc3d_io.so")):
-        build_dropin(name, cfg, work, objs, libdir)
+        build_dropin(name, cfg, work, objs, libdir, dflags)
     return outdir
 
 
-def build_dropin(name, cfg, work, objs, libdir):
+def build_dropin(name, cfg, work, objs, libdir, dflags=()):
     """Link the reference program with the Fortran drop-in instead of rungekutta.f90."""
-    extra = {"torus": ["-DCUMC3D_TORUS"], "torus_sph": ["-DCUMC3D_TORUS", "-DCUMC3D_TORUS_SPH"]}.get(cfg.model, [])
+    extra = {"torus": ["-DCUMC3D_TORUS"], "torus_sph": ["-DCUMC3D_TORUS", "-DCUMC3D_TORUS_SPH"],
+             "collapsar": ["-DCUMC3D_COLLAPSAR"]}.get(cfg.model, [])
     dobjs = [o for o in objs if o not in ("rungekutta.o", "driver.o")]
     for src in DROPIN_SRC:
         obj = "dropin_" + os.path.basename(src).replace(".f90", ".o")
         run([FLANG, *FFLAGS, *extra, "-c", src, "-o", obj], work)
         dobjs.append(obj)
-    run([FLANG, *FFLAGS, "-DHIPDROP", "-c", os.path.join(HERE, "ref_harness", "driver.f90"),
+    run([FLANG, *FFLAGS, "-DHIPDROP", *dflags, "-c", os.path.join(HERE, "ref_harness", "driver.f90"),
          "-o", "driver_hipdrop.o"], work)
     dobjs.append("driver_hipdrop.o")
     outdir = os.path.join(OUT, "dropin", name)

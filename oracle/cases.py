@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from cumc3d_amd.config import Config, briowu_1d, torus_sph  # noqa: E402
+from cumc3d_amd.config import Config, briowu_1d, collapsar, torus_sph  # noqa: E402
 
 
 @dataclasses.dataclass
@@ -99,6 +99,17 @@ CASES = {c.name: c for c in [
                                  z_start=0.0, z_end=2.0 * math.pi, cfl=0.4, ggas=5.0 / 3.0,
                                  riemann="hllc", recon="ppm", boundary_flag=(1, 1, 3, 3, 0, 0)),
          "sph_blast", 100),
+    # collapsar: self-gravity (the red-black SOR potential at the initial update
+    # and every n_pot steps), black-hole accretion through the inner face and
+    # the atmosphere floor, on a logarithmic radial grid the reference reads
+    # from ./profile/*_grid.dat.  The reference's SOR, whose periodic phi ghost
+    # values lag one sweep behind, diverges with fewer than about 64 phi
+    # cells, so the 3D case has 96; axisymmetric 2D converges quickly.
+    Case("collapsar2d_ppm", collapsar(24, 16, n_dim=2, recon="ppm", riemann="hlld"), "collapsar", 20),
+    Case("collapsar2d_weno_hll", collapsar(24, 16, n_dim=2, recon="weno", riemann="hll"), "collapsar", 20),
+    Case("collapsar2d_npot2", collapsar(16, 12, n_dim=2, recon="tvdmc", riemann="hllc").with_(n_pot=2),
+         "collapsar", 9),
+    Case("collapsar_ppm", collapsar(12, 8, 96, recon="ppm", riemann="hlld"), "collapsar", 12),
 ]}
 
 
@@ -109,3 +120,5 @@ BENCH_CASES = {c.name: c for c in [
                                   total_time=0.02, riemann="hlld", recon="weno",
                                   boundary_flag=(0,) * 6), "blast", 3),
 ]}
+
+ALL_CASES = {**CASES, **BENCH_CASES}

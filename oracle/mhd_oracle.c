@@ -43,6 +43,11 @@ struct oracle {
     double *rho_floor, *p_floor, *eps_floor;
     double maxdivb;
     double uss_keep[2][9];   /* usstarL/usstarR carried across interfaces (hlld_bn_mode 2) */
+    /* model 3 (model/collapsar/src/def.f90:6-46) */
+    double *phi, *phi_old, *ajp1, *ajm1, *bkp1, *bkm1, *clp1, *clm1, *epsc, *mflux;
+    double m_bh, mbh_old, r_sh;
+    int nstep, sor_iter, status;
+    double sor_err;
 };
 
 #define CI(o, j, k, l) ((long)((j) + 2) + (long)(o)->NX * ((long)((k) + 2) + (long)(o)->NY * ((l) + 2)))
@@ -182,6 +187,36 @@ oracle *oracle_create(const oracle_desc *d)
     for (int i = 1; i <= 6; ++i) o->eface[i] = alloc_cells(o);
     o->efx = alloc_cells(o); o->efy = alloc_cells(o); o->efz = alloc_cells(o);
     o->rho_floor = alloc_cells(o); o->p_floor = alloc_cells(o); o->eps_floor = alloc_cells(o);
+    o->phi = alloc_cells(o); o->phi_old = alloc_cells(o);
+    o->mflux = (double *)calloc((size_t)o->NY * o->NZ, sizeof(double));
+    if (d->model == 3) {
+        /* get_poisson / poisson_coef, model/collapsar/src/poisson.f90:6-64 */
+        int nx = o->nx, ny = o->ny, nz = o->nz;
+        o->ajp1 = (double *)calloc(nx + 1, sizeof(double));
+        o->ajm1 = (double *)calloc(nx + 1, sizeof(double));
+        o->bkp1 = (double *)calloc((size_t)(nx + 1) * (ny + 1), sizeof(double));
+        o->bkm1 = (double *)calloc((size_t)(nx + 1) * (ny + 1), sizeof(double));
+        o->clp1 = alloc_cells(o); o->clm1 = alloc_cells(o); o->epsc = alloc_cells(o);
+        for (int i = 1; i <= nx; ++i)
+            for (int j = 1; j <= ny; ++j)
+                for (int k = 1; k <= nz; ++k) {
+                    double xm1 = A1(o->x, i - 1), xc = A1(o->x, i), xp1 = A1(o->x, i + 1);
+                    double ym1 = A1(o->y, j - 1), yc = A1(o->y, j), yp1 = A1(o->y, j + 1);
+                    double zm1 = A1(o->z, k - 1), zc = A1(o->z, k), zp1 = A1(o->z, k + 1);
+                    double t = tan(yc), sn = sin(yc);
+                    long c = CI(o, i, j, k);
+                    o->epsc[c] = 2.0 * (xp1 + xm1 - 3.0 * xc) / xc / (xp1 - xc) / (xc - xm1)
+                               + (yp1 + ym1 - 2.0 * yc - 2.0 * t) / (xc * xc) / t / (yp1 - yc) / (yc - ym1)
+                               - 2.0 / (xc * xc) / (sn * sn) / (zp1 - zc) / (zc - zm1);
+                    o->ajp1[i] = 2.0 * (2.0 * xc - xm1) / (xp1 - xm1) / (xp1 - xc) / xc;
+                    o->ajm1[i] = 2.0 * (2.0 * xc - xp1) / (xp1 - xm1) / (xc - xm1) / xc;
+                    o->bkp1[(size_t)j * (nx + 1) + i] = (2.0 * t + yc - ym1) / (xc * xc) / t / (yp1 - yc) / (yp1 - ym1);
+                    o->bkm1[(size_t)j * (nx + 1) + i] = (2.0 * t + yc - yp1) / (xc * xc) / t / (yp1 - ym1) / (yc - ym1);
+                    o->clp1[c] = 2.0 / (xc * xc) / (sn * sn) / (zp1 - zm1) / (zp1 - zc);
+                    o->clm1[c] = 2.0 / (xc * xc) / (sn * sn) / (zp1 - zm1) / (zc - zm1);
+                }
+        o->m_bh = d->m_bh;
+    }
     return o;
 }
 
@@ -201,10 +236,37 @@ void oracle_destroy(oracle *o)
     for (int i = IBX; i <= IBZ; ++i) free(o->bcell[i]);
     for (int i = IEX; i <= IEZ; ++i) free(o->ecell[i]);
     for (int i = 1; i <= 6; ++i) free(o->eface[i]);
+    double *g[] = {o->phi, o->phi_old, o->ajp1, o->ajm1, o->bkp1, o->bkm1, o->clp1, o->clm1, o->epsc, o->mflux};
+    for (size_t i = 0; i < sizeof(g) / sizeof(g[0]); ++i) free(g[i]);
     free(o);
 }
 
 long oracle_cells(const oracle *o) { return o->G; }
+
+long oracle_poisson(const oracle *o, const char *name, double *out)
+{
+    if (o->d.model != 3) return -1;
+    long n1 = o->nx + 1, n2 = (long)(o->nx + 1) * (o->ny + 1);
+    const char *nm[7] = {"ajp1", "ajm1", "bkp1", "bkm1", "clp1", "clm1", "epsc"};
+    const double *a[7] = {o->ajp1, o->ajm1, o->bkp1, o->bkm1, o->clp1, o->clm1, o->epsc};
+    long len[7] = {n1, n1, n2, n2, o->G, o->G, o->G};
+    for (int i = 0; i < 7; ++i)
+        if (!strcmp(name, nm[i])) {
+            memcpy(out, a[i], (size_t)len[i] * sizeof(double));
+            return len[i];
+        }
+    return -1;
+}
+
+int oracle_gravity(const oracle *o, double *phi, double *m_bh, double *r_sh, int *iterations)
+{
+    if (o->d.model != 3) return -1;
+    if (phi) memcpy(phi, o->phi, (size_t)o->G * sizeof(double));
+    if (m_bh) *m_bh = o->m_bh;
+    if (r_sh) *r_sh = o->r_sh;
+    if (iterations) *iterations = o->sor_iter;
+    return o->status;
+}
 
 int oracle_load(oracle *o, const double *prim_aos, const double *eps, const double *floors)
 {
@@ -762,6 +824,13 @@ static void get_source(oracle *o)
     }
 }
 
+/* first_derivative, model/collapsar/src/custom.f90:402-409 */
+static inline double first_derivative(double xm1, double xc, double xp1, double fm1, double fc, double fp1)
+{
+    double h2 = xp1 - xc, h1 = xc - xm1;
+    return ((fp1 - fc) * h1 * h1 + (fc - fm1) * h2 * h2) / (h1 * h2 * (h1 + h2));
+}
+
 /* CUSTOM_SOURCE: model/riemann/src/custom.f90:143-155 (RT gravity),
  * model/torus-cyn/src/custom.f90:297-313 and model/torus-sph/src/custom.f90:283-296
  * (Paczynski-Wiita gravity with floor mask) */
@@ -778,6 +847,33 @@ static void custom_source(oracle *o)
         return;
     }
     double rsh = o->d.r_sh;
+    if (o->d.model == 3) {
+        /* model/collapsar/src/custom.f90:363-396: the matter potential's
+         * gradient (three-point, non-uniform) plus the black hole */
+        const double *phi = o->phi;
+        long sy = o->NX, sz = (long)o->NX * o->NY;
+        for (int l = 1; l <= o->nz; ++l)
+            for (int k = 1; k <= o->ny; ++k)
+                for (int j = 1; j <= o->nx; ++j) {
+                    long c = CI(o, j, k, l);
+                    double rho = o->prim[IRHO][c];
+                    double diff = rho - o->d.prim_a[0];
+                    double factor = fmax2(fsign(1.0, diff), 0.0);
+                    double xj = A1(o->x, j), sk = A1(o->sine, k);
+                    double dphidx = first_derivative(A1(o->x, j - 1), xj, A1(o->x, j + 1), phi[c - 1], phi[c], phi[c + 1]);
+                    double dphidy = first_derivative(A1(o->y, k - 1), A1(o->y, k), A1(o->y, k + 1), phi[c - sy], phi[c],
+                                                     phi[c + sy]);
+                    double dphidz = first_derivative(A1(o->z, l - 1), A1(o->z, l), A1(o->z, l + 1), phi[c - sz], phi[c],
+                                                     phi[c + sz]);
+                    dphidx = dphidx + o->m_bh / ((xj - o->r_sh) * (xj - o->r_sh));
+                    double vx = o->prim[IVX][c], vy = o->prim[IVY][c], vz = o->prim[IVZ][c];
+                    o->sc[IVX][c] = o->sc[IVX][c] - factor * rho * dphidx;
+                    o->sc[IVY][c] = o->sc[IVY][c] - factor * rho * dphidy / xj;
+                    o->sc[IVZ][c] = o->sc[IVZ][c] - factor * rho * dphidz / xj / sk;
+                    o->sc[ITAU][c] = o->sc[ITAU][c] - factor * rho * (vx * dphidx + vy * dphidy / xj + vz * dphidz / xj / sk);
+                }
+        return;
+    }
     if (o->d.model == 2) {
         for (int l = 1; l <= o->nz; ++l)
             for (int k = 1; k <= o->ny; ++k)
@@ -1057,6 +1153,9 @@ static void spatial(oracle *o)
         riemann(o, dir);
         flux_diff(o, dir);
         mhd_flux(o, dir);
+        if (dir == XDIR)    /* GETFLUX_X of the collapsar, custom.f90:220-240: mflux_x = flux(irho,0,k,l) */
+            for (int l = 1; l <= o->nz; ++l)
+                for (int k = 1; k <= o->ny; ++k) o->mflux[(size_t)(l + 2) * o->NY + (k + 2)] = o->flux[IRHO][CI(o, 0, k, l)];
     }
     flux_ct(o);
     l_operator(o);
@@ -1140,6 +1239,14 @@ static void checkrho(oracle *o)
                 long c = CI(o, j, k, l);
                 if (o->d.model == 0) {
                     o->eps[c] = fmax2(o->eps[c], 0.0);
+                } else if (o->d.model == 3) {
+                    /* model/collapsar/src/custom.f90:266-282 */
+                    double diff = o->prim[IRHO][c] - o->d.prim_a[0];
+                    double f = fmax2(fsign(1.0, diff), 0.0);
+                    for (int i = IRHO; i <= IVZ; ++i)
+                        o->prim[i][c] = f * o->prim[i][c] + (1.0 - f) * o->d.prim_a[i - IRHO];
+                    o->eps[c] = f * o->eps[c] + (1.0 - f) * o->d.eps_a;
+                    if (o->eps[c] < 0.0) o->eps[c] = o->d.eps_a;
                 } else {
                     double diff = o->prim[IRHO][c] - o->rho_floor[c];
                     double f = fmax2(fsign(1.0, diff), 0.0);
@@ -1277,7 +1384,7 @@ static void update(oracle *o)
     double g = o->d.ggas;
     for (long c = 0; c < o->G; ++c) {
         double rho = o->prim[IRHO][c];
-        if (o->d.model == 0 || rho > o->rho_floor[c]) {
+        if (o->d.model == 0 || o->d.model == 3 || rho > o->rho_floor[c]) {
             o->prim[ITAU][c] = rho * o->eps[c] * (g - 1.0);
             o->dpdrho[c] = o->eps[c] * (g - 1.0);
             o->dpdeps[c] = rho * (g - 1.0);
@@ -1290,12 +1397,133 @@ static void update(oracle *o)
     }
 }
 
+/* The red-black SOR relaxation of the potential, CUSTOM_UPDATE's loop at
+ * model/collapsar/src/custom.f90:511-647.  Red cells ((-1)**(j+k+l) > 0) are
+ * swept first, then black; each reads only cells of the other colour, so the
+ * order inside a sweep does not matter.
+ * The error is the OpenMP reduction of :588-598 as the flang -fopenmp build
+ * computes it on one thread: the private copy folds
+ * MAX(abserror, |dphi/phi_old|) in l, k, j order with flang's MAX
+ * (a > b ? a : b), so a NaN term replaces the running value and the next
+ * term replaces the NaN; the combine with the shared 1e-50 drops a NaN
+ * result.  A NaN last term (0/0 where phi stays 0, as on the first sweep
+ * from the zero guess) therefore gives abserror = 1e-50 and ends the loop;
+ * a serial build would keep the NaN and go on.  Returns the iteration at
+ * which the loop ended (relax_max + 1 when it ran out without converging). */
+static int sor_solve(oracle *o)
+{
+    int nx = o->nx, ny = o->ny, nz = o->nz;
+    long sy = o->NX, sz = (long)o->NX * o->NY;
+    double *phi = o->phi, *old = o->phi_old;
+    double pa = o->d.prim_a[0], w = o->d.omega;
+    const double pi = 4.0 * atan(1.0);   /* parameter.h:21 */
+    int n;
+    for (n = 1; n <= o->d.relax_max; ++n) {
+        for (int l = 1; l <= nz; ++l)
+            for (int k = 1; k <= ny; ++k)
+                for (int j = 1; j <= nx; ++j) old[CI(o, j, k, l)] = phi[CI(o, j, k, l)];
+        for (int colour = 0; colour < 2; ++colour)
+            for (int l = 1; l <= nz; ++l)
+                for (int k = 1; k <= ny; ++k)
+                    for (int j = 1; j <= nx; ++j) {
+                        if (((j + k + l) & 1) != colour) continue;
+                        long c = CI(o, j, k, l);
+                        double rho = o->prim[IRHO][c];
+                        double diff = rho - pa;
+                        double factor = diff > 0.0 ? 1.0 : 0.0;
+                        double rho_in = factor * rho;
+                        size_t jk = (size_t)k * (nx + 1) + j;
+                        double s = o->ajp1[j] * phi[c + 1] + o->ajm1[j] * phi[c - 1] + o->bkp1[jk] * phi[c + sy] +
+                                   o->bkm1[jk] * phi[c - sy] + o->clp1[c] * phi[c + sz] + o->clm1[c] * phi[c - sz];
+                        double rhs = (4.0 * pi * rho_in - s) / o->epsc[c];
+                        phi[c] = (1.0 - w) * phi[c] + w * rhs;
+                    }
+        double priv = -DBL_MAX;
+        for (int l = 1; l <= nz; ++l)
+            for (int k = 1; k <= ny; ++k)
+                for (int j = 1; j <= nx; ++j) {
+                    long c = CI(o, j, k, l);
+                    priv = fmax2(priv, fabs((phi[c] - old[c]) / old[c]));
+                }
+        double err = priv != priv ? 1.0e-50 : fmax2(1.0e-50, priv);
+        if (o->d.sor_fold == 1) {   /* planted: the serial fold from 1e-50 keeps a NaN last term */
+            err = 1.0e-50;
+            for (int l = 1; l <= nz; ++l)
+                for (int k = 1; k <= ny; ++k)
+                    for (int j = 1; j <= nx; ++j) {
+                        long c = CI(o, j, k, l);
+                        err = fmax2(err, fabs((phi[c] - old[c]) / old[c]));
+                    }
+        }
+        o->sor_err = err;
+        for (int l = 1; l <= nz; ++l)
+            for (int k = 1; k <= ny; ++k) {
+                phi[CI(o, 0, k, l)] = phi[CI(o, 1, k, l)];
+                phi[CI(o, nx + 1, k, l)] = -1.0;
+            }
+        for (int l = 1; l <= nz; ++l)
+            for (int j = 1; j <= nx; ++j) {
+                phi[CI(o, j, 0, l)] = phi[CI(o, j, 1, l)];
+                phi[CI(o, j, ny + 1, l)] = phi[CI(o, j, ny, l)];
+            }
+        for (int k = 1; k <= ny; ++k)
+            for (int j = 1; j <= nx; ++j) {
+                phi[CI(o, j, k, 0)] = phi[CI(o, j, k, nz)];
+                phi[CI(o, j, k, nz + 1)] = phi[CI(o, j, k, 1)];
+            }
+        if (err <= o->d.tolerance) break;
+    }
+    return n;
+}
+
+/* CUSTOM_UPDATE of the collapsar, model/collapsar/src/custom.f90:416-658:
+ * black-hole accretion through the inner face (stages 1-3, from the x mass
+ * flux GETFLUX_X saved), the horizon check, and the potential solve at the
+ * initial update and on stage 3 of every n_pot-th step.  A STOP of the
+ * reference sets o->status (1 horizon, 2 convergence) and ends oracle_run. */
+static void custom_update(oracle *o, int p, double dt)
+{
+    if (o->d.model != 3 || o->status) return;
+    if (p > 0) {
+        const double rk20 = 3.0 / 4.0, rk21 = 1.0 / 4.0, rk22 = 1.0 / 4.0;
+        const double rk30 = 1.0 / 3.0, rk31 = 2.0 / 3.0, rk32 = 2.0 / 3.0;
+        double xf0 = A1(o->xF, 0), mdot = 0.0;
+        for (int l = 1; l <= o->nz; ++l)
+            for (int k = 1; k <= o->ny; ++k)
+                mdot = mdot + (-o->mflux[(size_t)(l + 2) * o->NY + (k + 2)]) * xf0 * xf0 * A1(o->dcose, k) * A1(o->dz, l);
+        mdot = fmax2(mdot, 0.0);
+        if (p == 1) {
+            o->mbh_old = o->m_bh;
+            o->m_bh = o->mbh_old + dt * mdot;
+        } else if (p == 2) {
+            o->m_bh = rk20 * o->mbh_old + rk21 * o->m_bh + rk22 * dt * mdot;
+        } else {
+            o->m_bh = rk30 * o->mbh_old + rk31 * o->m_bh + rk32 * dt * mdot;
+        }
+    }
+    o->r_sh = 2.0 * o->m_bh / (o->d.c_code * o->d.c_code);
+    if (o->r_sh >= A1(o->xF, 0)) {
+        o->status = 1;
+        return;
+    }
+    if (p == 0 || (p == 3 && o->nstep % o->d.n_pot == 0)) {
+        if (p == 0)
+            for (int l = 0; l <= o->nz + 1; ++l)
+                for (int k = 0; k <= o->ny + 1; ++k)
+                    for (int j = 0; j <= o->nx + 1; ++j) o->phi[CI(o, j, k, l)] = 0.0;
+        int n = sor_solve(o);
+        o->sor_iter = n;
+        if (n == o->d.relax_max) o->status = 2;
+    }
+}
+
 void oracle_initial_update(oracle *o)
 {
     checkrho(o);
     fromrvetou(o);
     boundary(o);
     update(o);
+    custom_update(o, 0, 0.0);
 }
 
 /* FINDDT, rungekutta.f90:256-323 */
@@ -1366,13 +1594,15 @@ void oracle_rungekutta(oracle *o, double dt)
         fromrvetou(o);
         boundary(o);
         update(o);
+        custom_update(o, stage, dt);
     }
+    o->nstep = o->nstep + 1;   /* main.f90:112, after the step */
 }
 
 int oracle_run(oracle *o, int max_steps, double total_time, double *t, double *dts)
 {
     int n = 0;
-    while (*t < total_time && n < max_steps) {
+    while (*t < total_time && n < max_steps && !o->status) {
         double dt = oracle_finddt(o);
         if (*t + dt > total_time) dt = total_time - *t;
         oracle_rungekutta(o, dt);

@@ -24,7 +24,7 @@ typedef struct oracle_desc {
     int boundary_flag[6];
     int riemann;   /* 0 LF, 1 HLL, 2 HLLC, 3 HLLD            (parameter.h:91-101) */
     int recon;     /* 0 minmod, 1 van Leer, 2 MC, 3 PPM, 4 WENO (parameter.h:103-115) */
-    int model;     /* 0 riemann hooks, 1 torus-cyn hooks, 2 torus-sph hooks */
+    int model;     /* 0 riemann hooks, 1 torus-cyn hooks, 2 torus-sph hooks, 3 collapsar hooks */
     int rt_gravity;
     int hlld_bn_mode; /* value of the unassigned usstar(ibn) in HLLDNM: 0 -> 0.0, 1 -> Bn,
                       2 -> carried from earlier double-star interfaces (serial reference) */
@@ -32,6 +32,16 @@ typedef struct oracle_desc {
     const double *xF, *yF, *zF;  /* face coordinates -3..n+3 (n+7 values), before the FIXPOLE override */
     int fixpole;                 /* -DFIXPOLE: polar sinf = 0 and phi-averaged polar EMFs (spherical only);
                                     2 = the same with a double accumulator, a planted mistake for tests */
+    /* model 3 (model/collapsar): self-gravity and black hole */
+    double prim_a[4];            /* atmosphere rho, vx, vy, vz (prim_a(irho:ivz)) */
+    double eps_a;                /* atmosphere epsilon */
+    double m_bh;                 /* initial black-hole mass */
+    double c_code;               /* clight*vel2code, the speed of light in code units */
+    double omega;                /* SOR weight omega_weight */
+    double tolerance;            /* SOR tolerance */
+    int relax_max, n_pot;        /* SOR iteration cap; potential solved every n_pot steps */
+    int sor_fold;                /* SOR error fold: 0 the one-thread OpenMP reduction of the
+                                    reference build; 1 a plain serial fold, a planted mistake for tests */
 } oracle_desc;
 
 typedef struct oracle oracle;
@@ -54,6 +64,15 @@ int oracle_grid(const oracle *o, const char *name, double *out);
  * "bz","eps","cs","bcx","bcy","bcz","u1".."u8","lrk1".."lrk8","flux1".."flux8". */
 int oracle_get(oracle *o, const char *name, double *out);
 long oracle_cells(const oracle *o);
+/* model 3: the potential phi over the ghosted box, black-hole mass and
+ * Schwarzschild radius, SOR iterations of the last solve, and the status
+ * (0 ok, 1 horizon beyond the inner face, 2 converged at the relax cap). */
+int oracle_gravity(const oracle *o, double *phi, double *m_bh, double *r_sh, int *iterations);
+/* model 3: poisson_coef's coefficients (model/collapsar/src/poisson.f90:35-64):
+ * "ajp1","ajm1" at [j] (nx+1 values), "bkp1","bkm1" at [k*(nx+1)+j]
+ * ((nx+1)*(ny+1)), "clp1","clm1","epsc" over the ghosted box (cell index as
+ * for oracle_get; interior cells only set).  Returns the length or -1. */
+long oracle_poisson(const oracle *o, const char *name, double *out);
 
 /* Point kernels for unit tests. */
 void oracle_tvd(int limiter, const double w[2], double vm, double vc, double vp,

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 
 from cumc3d_amd.config import MODELS, RECON, RIEMANN, Config  # noqa: E402
 from cumc3d_amd.grid import Grid  # noqa: E402
-from cumc3d_amd.ic import PRIM_ORDER  # noqa: E402
+from cumc3d_amd.ic import CLIGHT, PRIM_ORDER  # noqa: E402
 
 PD = C.POINTER(C.c_double)
 
@@ -27,7 +27,10 @@ class _Desc(C.Structure):
                 ("riemann", C.c_int), ("recon", C.c_int), ("model", C.c_int),
                 ("rt_gravity", C.c_int), ("hlld_bn_mode", C.c_int),
                 ("cfl", C.c_double), ("ggas", C.c_double), ("r_sh", C.c_double),
-                ("xF", PD), ("yF", PD), ("zF", PD), ("fixpole", C.c_int)]
+                ("xF", PD), ("yF", PD), ("zF", PD), ("fixpole", C.c_int),
+                ("prim_a", C.c_double * 4), ("eps_a", C.c_double), ("m_bh", C.c_double),
+                ("c_code", C.c_double), ("omega", C.c_double), ("tolerance", C.c_double),
+                ("relax_max", C.c_int), ("n_pot", C.c_int), ("sor_fold", C.c_int)]
 
 
 def build() -> None:
@@ -57,6 +60,10 @@ def lib():
         L.oracle_find_divb.restype = C.c_double
         L.oracle_get.argtypes = [C.c_void_p, C.c_char_p, PD]
         L.oracle_grid.argtypes = [C.c_void_p, C.c_char_p, PD]
+        L.oracle_gravity.argtypes = [C.c_void_p, PD, PD, PD, C.POINTER(C.c_int)]
+        L.oracle_gravity.restype = C.c_int
+        L.oracle_poisson.argtypes = [C.c_void_p, C.c_char_p, PD]
+        L.oracle_poisson.restype = C.c_long
         L.oracle_cells.argtypes = [C.c_void_p]
         L.oracle_cells.restype = C.c_long
         L.oracle_tvd.argtypes = [C.c_int, PD, C.c_double, C.c_double, C.c_double, PD, PD]
@@ -77,7 +84,7 @@ class Oracle:
     """One CPU instance of the reference algorithm on a configuration."""
 
     def __init__(self, cfg: Config, state: dict, hlld_bn_mode: int = 2, grid: Grid | None = None,
-                 fixpole: int | None = None):
+                 fixpole: int | None = None, sor_fold: int = 0):
         self.cfg = cfg
         self.grid = grid or Grid(cfg)
         L = lib()
@@ -95,6 +102,15 @@ class Oracle:
         d.cfl, d.ggas, d.r_sh = cfg.cfl, cfg.ggas, cfg.r_sh
         d.xF, d.yF, d.zF = (_p(f) for f in self._faces)
         d.fixpole = int(cfg.fixpole) if fixpole is None else fixpole
+        if cfg.model == "collapsar":
+            gv = state["gravity"]
+            for i in range(4):
+                d.prim_a[i] = gv[i]
+            d.eps_a, d.m_bh = gv[8], gv[9]
+            d.c_code = CLIGHT * gv[10]                       # clight*vel2code, custom.f90:481
+            d.omega, d.tolerance = cfg.sor_omega, cfg.sor_tolerance
+            d.relax_max, d.n_pot = cfg.relax_max, cfg.n_pot
+            d.sor_fold = sor_fold
         self._h = L.oracle_create(C.byref(d))
         if not self._h:
             raise ValueError("invalid boundary flag combination")
@@ -129,6 +145,32 @@ class Oracle:
         self.t = t.value
         self.nstep += n
         return dts[:n]
+
+    def gravity(self) -> dict:
+        """Collapsar: potential phi (ghosted box), m_bh, r_sh, the SOR
+        iterations of the last solve and the status (0 running, 1 the horizon
+        reached xF(0), 2 the solve converged on its last allowed sweep)."""
+        phi = np.empty(self.cfg.shape)
+        m, r, it = C.c_double(), C.c_double(), C.c_int()
+        status = lib().oracle_gravity(self._h, _p(phi), C.byref(m), C.byref(r), C.byref(it))
+        if status < 0:
+            raise ValueError("not a collapsar run")
+        return {"phi": phi, "m_bh": m.value, "r_sh": r.value, "iterations": it.value, "status": status}
+
+    def poisson(self, name: str) -> np.ndarray:
+        """poisson_coef coefficient `name`: ajp1/ajm1 as [j], bkp1/bkm1 as
+        [k, j], clp1/clm1/epsc on the ghosted box."""
+        cfg = self.cfg
+        size = {"ajp1": cfg.nx + 1, "ajm1": cfg.nx + 1, "bkp1": (cfg.nx + 1) * (cfg.ny + 1),
+                "bkm1": (cfg.nx + 1) * (cfg.ny + 1)}.get(name, int(np.prod(cfg.shape)))
+        out = np.zeros(size)
+        if lib().oracle_poisson(self._h, name.encode(), _p(out)) < 0:
+            raise KeyError(name)
+        if name in ("bkp1", "bkm1"):
+            return out.reshape(cfg.ny + 1, cfg.nx + 1)
+        if name in ("clp1", "clm1", "epsc"):
+            return out.reshape(cfg.shape)
+        return out
 
     def divb(self) -> float:
         return lib().oracle_find_divb(self._h)

@@ -11,13 +11,18 @@
 ! device) through the print_hydroprofile of cumc3d_amd/fortran/dataio_hip.f90.
 !
 ! It also writes OUT_FILE.grid, the GETGRID arrays the step reads
-! (getgrid.f90:26-98, after CUSTOM_GRID and the FIXPOLE override).
+! (getgrid.f90:26-98, after CUSTOM_GRID and the FIXPOLE override), and with
+! -DCUMC3D_COLLAPSAR OUT_FILE.grav: m_bh, r_sh and the potential phi of
+! module CUSTOM_DEF (model/collapsar/src/def.f90:16-36) after the last step.
 !
 ! usage: cumc3d_ref IC_FILE OUT_FILE MAX_STEPS [THREADS]
 PROGRAM REF_HARNESS
 USE DEFINITION
 USE MHD_MODULE
 USE HARNESS_IO
+#ifdef CUMC3D_COLLAPSAR
+USE CUSTOM_DEF, ONLY : phi, m_bh, r_sh
+#endif
 #ifdef HIPDROP
 USE CUMC3D_HIP_STATE, ONLY : CUMC3D_HIP_SYNC_HOST, CUMC3D_HIP_DETACH
 #endif
@@ -93,6 +98,13 @@ WRITE(u) cs
 WRITE(u) cons
 WRITE(u) dts(1:cnt)
 CLOSE(u)
+#ifdef CUMC3D_COLLAPSAR
+OPEN(NEWUNIT=u, FILE=TRIM(out_path)//'.grav', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE', &
+     ACTION='WRITE')
+WRITE(u) m_bh, r_sh
+WRITE(u) phi
+CLOSE(u)
+#endif
 #ifdef HIPDROP
 CALL CUMC3D_HIP_DETACH
 #endif

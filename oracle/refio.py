@@ -71,12 +71,39 @@ def dropin_binary(case: str) -> str:
     return os.path.join(HERE, "_ref", "dropin", case, "cumc3d_dropin")
 
 
+def write_grid_files(d: str, cfg) -> None:
+    """The collapsar's CUSTOM_GRID input (model/collapsar/src/custom.f90:68-155):
+    ./profile/r_grid.dat, th_grid.dat, phi_grid.dat with the faces -3..n+3 one
+    per line, written as shortest round-trip decimals so the list-directed READ
+    gets the same doubles."""
+    from cumc3d_amd.grid import Grid
+    g = Grid(cfg)
+    os.makedirs(os.path.join(d, "profile"), exist_ok=True)
+    for name, ax in (("r_grid.dat", g.x), ("th_grid.dat", g.y), ("phi_grid.dat", g.z)):
+        with open(os.path.join(d, "profile", name), "w") as f:
+            f.write("".join(repr(float(v)) + "\n" for v in ax.faces))
+
+
+def read_grav(path: str, shape) -> dict:
+    with open(path, "rb") as f:
+        m_bh, r_sh = np.fromfile(f, dtype=np.float64, count=2)
+        phi = np.fromfile(f, dtype=np.float64, count=int(np.prod(shape))).reshape(shape)
+    return {"m_bh": float(m_bh), "r_sh": float(r_sh), "phi": phi}
+
+
 def run_ref(case: str, ic_path: str, steps: int, threads: int = 1, timeout: float = 600,
             binary: str | None = None) -> dict:
     ic_path = os.path.abspath(ic_path)
+    try:
+        from oracle.cases import ALL_CASES
+    except ImportError:     # run from oracle/ itself (build_ref.py)
+        from cases import ALL_CASES
+    cfg = ALL_CASES[case].cfg
     with tempfile.TemporaryDirectory() as d:
         # the program runs in a scratch directory; profiles go to ./outfile
         os.makedirs(os.path.join(d, "outfile"))
+        if cfg.model == "collapsar":
+            write_grid_files(d, cfg)
         out = os.path.join(d, "dump.bin")
         env = dict(os.environ)
         # Fresh heap memory filled with a fixed byte pattern, so that nothing
@@ -89,6 +116,8 @@ def run_ref(case: str, ic_path: str, steps: int, threads: int = 1, timeout: floa
         res = read_dump(out)
         if os.path.exists(out + ".grid"):
             res["grid"] = read_grid(out + ".grid")
+        if os.path.exists(out + ".grav"):
+            res["grav"] = read_grav(out + ".grav", cfg.shape)
         if os.path.exists(out + ".divb"):
             # FIND_DIVB's log record, WRITE(999,*) global_time, maxDivB
             with open(out + ".divb") as f:

@@ -6,7 +6,8 @@ sources by oracle/build_ref.py) single-threaded with zero-filled allocations
 (oracle/refio.run_ref) on each case of oracle/cases.py and stores, per case,
 the dt sequence and the final state on cells 0..n+1 of every axis:
 rho, vx, vy, vz, p, bx, by, bz (face), eps, bcx, bcy, bcz; for spherical
-cases also the GETGRID metric arrays the step reads (grid_*).
+cases also the GETGRID metric arrays the step reads (grid_*); for the
+collapsar also m_bh, r_sh and the potential phi on cells 0..n+1.
 
 usage: python tests/golden/make_golden.py [case ...]
 """
@@ -55,6 +56,12 @@ def main(names):
             # the reference's GETGRID arrays (after FIXPOLE), -3..n+3 at j+3
             for k, v in ref["grid"].items():
                 out["grid_" + k] = v
+        if "grav" in ref:
+            cfg = case.cfg
+            out["m_bh"] = np.array(ref["grav"]["m_bh"])
+            out["r_sh"] = np.array(ref["grav"]["r_sh"])
+            # phi on cells 0..n+1 of every axis (the solved box, custom.f90:497-631)
+            out["phi"] = ref["grav"]["phi"][2:cfg.nz + 4, 2:cfg.ny + 4, 2:cfg.nx + 4]
         path = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(path, **out)
         print(f"{name}: {ref['nstep']} steps -> {path} ({os.path.getsize(path) // 1024} KB)")

@@ -47,6 +47,11 @@ def test_dropin_matches_reference(name):
         a, b = got[f][w], gold[f]
         assert np.array_equal(a, b, equal_nan=True), f"{name}: {f} differs, max {np.nanmax(np.abs(a - b)):.3e}"
     assert got["maxdivb"] == float(gold["maxdivb"])
+    if "phi" in gold:   # collapsar: CUSTOM_DEF's phi, m_bh, r_sh synced back from the device
+        cfg = case.cfg
+        g = got["grav"]
+        assert g["m_bh"] == float(gold["m_bh"]) and g["r_sh"] == float(gold["r_sh"])
+        assert np.array_equal(g["phi"][2:cfg.nz + 4, 2:cfg.ny + 4, 2:cfg.nx + 4], gold["phi"])
     check_profile(got["profiles"]["rkiter-1-nm.hdf5"], case.cfg, got, got["t"])
     o = O.Oracle(case.cfg, st)         # the state after INITIAL_UPDATE
     check_profile(got["profiles"]["rkiter-0-nm.hdf5"], case.cfg, {f: o.get(f) for f in FIELDS[:9]}, 0.0)

@@ -7,7 +7,7 @@ import pytest
 from conftest import check_profile, interior_window, load_golden
 from cumc3d_amd import hydroprofile
 from cumc3d_amd import ic
-from cumc3d_amd.config import Config, orszag_tang_3d, torus_cyl, torus_sph
+from cumc3d_amd.config import Config, collapsar, orszag_tang_3d, torus_cyl, torus_sph
 from cumc3d_amd.solver import make_solver
 from oracle import oracle as O
 from oracle.cases import CASES
@@ -39,12 +39,18 @@ def test_gpu_matches_reference_fixture(name):
     st = s.state()
     assert_same({f: st[f][w] for f in FIELDS}, gold, label=name + ": ")
     assert s.find_divb() == float(gold["maxdivb"])
+    if "phi" in gold:   # collapsar: black hole and potential after the last UPDATE
+        g = s.gravity()
+        assert g["m_bh"] == float(gold["m_bh"]) and g["r_sh"] == float(gold["r_sh"])
+        cfg = case.cfg
+        box = (slice(2, cfg.nz + 4), slice(2, cfg.ny + 4), slice(2, cfg.nx + 4))
+        assert np.array_equal(g["phi"][box], gold["phi"]), "phi differs"
 
 
 @pytest.mark.parametrize("threads", [1, 37])
 @pytest.mark.parametrize("name", ["ot_hlld_weno", "ot_hlld_tvdmc", "ot_hllc_ppm", "ot_bc_mixed",
                                   "blast_weno", "ot2d_hlld_ppm", "torus_hlld_ppm", "torus_sym_tvdmc",
-                                  "sph_torus_ppm", "sph_blast_weno"])
+                                  "sph_torus_ppm", "sph_blast_weno", "collapsar_ppm"])
 def test_gpu_long_march_segments(name, threads, monkeypatch):
     """The y/z sweeps with long segments: CUMC3D_MARCH_THREADS=1 gives one
     segment per column (every thread marches the whole column), 37 an uneven
@@ -144,3 +150,90 @@ def test_gpu_profile_file(name, precision, tmp_path):
     path = hydroprofile.write_profile(s, str(tmp_path / "outfile"), 4)
     assert path.endswith("outfile/rkiter-4-nm.hdf5")
     check_profile(hydroprofile.read_profile(path), case.cfg, s.state(), s.t)
+
+
+# ---------------------------------------------------------------------------
+# collapsar: self-gravity and black hole
+# ---------------------------------------------------------------------------
+def assert_gravity(s, o, cfg):
+    g, h = s.gravity(), o.gravity()
+    assert g["m_bh"] == h["m_bh"] and g["r_sh"] == h["r_sh"]
+    assert g["iterations"] == h["iterations"]
+    box = (slice(2, cfg.nz + 4), slice(2, cfg.ny + 4), slice(2, cfg.nx + 4))
+    assert np.array_equal(g["phi"][box], h["phi"][box]), "phi differs"
+
+
+@pytest.mark.parametrize("cfg,steps", [
+    (collapsar(48, 32, n_dim=2, recon="weno", riemann="hll"), 21),
+    (collapsar(20, 12, 96, recon="ppm", riemann="hlld"), 11),
+    (collapsar(24, 16, n_dim=2, recon="tvdmc", riemann="hlld").with_(n_pot=3), 10),
+], ids=["2d_48x32_weno_hll", "3d_20x12x96_ppm", "2d_npot3_tvdmc"])
+def test_gpu_collapsar_matches_oracle(cfg, steps):
+    """Several Poisson solves (the initial update, steps 0, n_pot, ...) and the
+    accretion of every stage against the C oracle: state, dt, m_bh, r_sh,
+    potential and the sweep count of the last solve."""
+    print(f"case collapsar {cfg.nx}x{cfg.ny}x{cfg.nz} {cfg.recon}/{cfg.riemann} n_pot={cfg.n_pot}", flush=True)
+    st = ic.collapsar(cfg)
+    s = make_solver(cfg, st)
+    o = O.Oracle(cfg, st)
+    assert_gravity(s, o, cfg)
+    assert np.array_equal(s.run(steps), o.run(steps))
+    assert_same(s.state(), o.state(), w=(slice(2, -2),) * 3 if cfg.n_dim == 3 else (slice(3, 4), slice(2, -2), slice(2, -2)))
+    assert_gravity(s, o, cfg)
+
+
+@pytest.mark.parametrize("name", ["ot2d_hlld_ppm", "collapsar2d_ppm", "sph2d_torus_tvdmc"])
+def test_gpu_ghost_fill_deterministic_on_inactive_axes(name):
+    """On the single-cell z axis of a 2D run the periodic ghost chain of the
+    reference resolves to cell 1; the one-pass ghost fill reads that cell
+    directly, so the whole downloaded box, outer ghost layers included, is the
+    same on every run (reading ghosts being written made them vary)."""
+    case = CASES[name]
+    st = ic.GENERATORS[case.ic](case.cfg)
+    a = make_solver(case.cfg, st)
+    b = make_solver(case.cfg, st)
+    a.run(case.steps)
+    b.run(case.steps)
+    assert_same(a.state(), b.state())
+    za = a.state()
+    for f in ("rho", "vy", "by"):     # copied fields (p is re-derived against each cell's own floor)
+        for l in (0, 1, 2, 4, 5, 6):             # l = -2..0 and 2..4 copy cell 1 (l index 3)
+            assert np.array_equal(za[f][l, 3:-3, 3:-3], za[f][3, 3:-3, 3:-3]), (f, l)
+
+
+def test_gpu_collapsar_host_step_path():
+    """finddt + rungekutta from the host (the drop-in's path) count steps for
+    the n_pot schedule as the device loop does."""
+    case = CASES["collapsar2d_npot2"]
+    st = ic.GENERATORS[case.ic](case.cfg)
+    a = make_solver(case.cfg, st)
+    b = make_solver(case.cfg, st)
+    for _ in range(5):
+        a.rungekutta(a.finddt())
+    b.run(2)
+    b.run(3)
+    assert_same(a.state(), b.state())
+    ga, gb = a.gravity(), b.gravity()
+    assert ga["m_bh"] == gb["m_bh"] and np.array_equal(ga["phi"], gb["phi"])
+
+
+def test_gpu_collapsar_stops_where_the_reference_stops():
+    """The two STOPs of CUSTOM_UPDATE come back as CUMC3D_ESTOP: the horizon
+    at the inner face (custom.f90:482-484) and a solve converging on its
+    last allowed sweep (:644-647); a solve that runs out goes on."""
+    from cumc3d_amd.solver import CumcError, Solver
+    cfg = collapsar(16, 12, n_dim=2)
+    st = ic.collapsar(cfg)
+    big = dict(st, gravity=st["gravity"].copy())
+    big["gravity"][9] = 0.5 * cfg.faces[0][3] * (ic.CLIGHT * st["gravity"][10]) ** 2 * 1.0001
+    s = Solver(cfg)
+    s.initial_model(big)
+    with pytest.raises(CumcError, match="event horizon"):
+        s.initial_update()
+    s = Solver(cfg.with_(relax_max=1))
+    s.initial_model(st)
+    with pytest.raises(CumcError, match="poisson"):
+        s.initial_update()
+    s = make_solver(cfg.with_(relax_max=3), st)
+    s.run(1)
+    assert s.gravity()["iterations"] == 4

@@ -36,6 +36,14 @@ def test_oracle_matches_reference_fixture(name):
         got = o.get(f)[w]
         assert np.array_equal(got, gold[f]), f"{f}: max diff {np.nanmax(np.abs(got - gold[f]))}"
     assert o.divb() == float(gold["maxdivb"])
+    if "phi" in gold:   # collapsar: black hole and potential after the last UPDATE
+        check_gravity(o.gravity(), gold, CASES[name].cfg)
+
+
+def check_gravity(g, gold, cfg):
+    assert g["m_bh"] == float(gold["m_bh"]) and g["r_sh"] == float(gold["r_sh"])
+    box = (slice(2, cfg.nz + 4), slice(2, cfg.ny + 4), slice(2, cfg.nx + 4))
+    assert np.array_equal(g["phi"][box], gold["phi"]), "phi differs"
 
 
 @pytest.mark.parametrize("name", ["ot_hlld_ppm", "torus_hlld_ppm", "ot2d_hlld_ppm"])
@@ -348,3 +356,136 @@ def test_spherical_divb_zero_away_from_frozen_faces():
            + (c(bz) - cm(bz, 0, 0, 1)) * (xdx * G["dy"][K + 3]) / (dcb * dc * G["dz"][L + 3] / 3.0))
     assert np.abs(div).max() < 1e-10 * np.abs(bx).max()
     assert o.divb() > 1e-3    # the frozen lower faces break it at cells 1
+
+
+# ---------------------------------------------------------------------------
+# collapsar: self-gravity and black hole (model/collapsar)
+# ---------------------------------------------------------------------------
+COLLAPSAR = [n for n in CASES if CASES[n].cfg.model == "collapsar"]
+
+
+def test_sor_error_fold_is_pinned_by_the_fixture():
+    """The SOR loop's error (custom.f90:588-598) is an OpenMP MAX reduction; in
+    the reference's one-thread build a NaN last term (0/0 where phi stays 0 on
+    the first sweep from the zero guess) is dropped by the combine, so the
+    initial solve stops after one sweep.  Planting a plain serial fold, which
+    keeps the NaN and relaxes on, breaks bit parity with the fixture."""
+    name = "collapsar2d_ppm"
+    case = CASES[name]
+    gold = load_golden(name)
+    st = ic.GENERATORS[case.ic](case.cfg)
+    good = O.Oracle(case.cfg, st)
+    assert good.gravity()["iterations"] == 1
+    bad = O.Oracle(case.cfg, st, sor_fold=1)
+    assert bad.gravity()["iterations"] > 1
+    bad.run(case.steps)
+    box = (slice(2, case.cfg.nz + 4), slice(2, case.cfg.ny + 4), slice(2, case.cfg.nx + 4))
+    assert not np.array_equal(bad.gravity()["phi"][box], gold["phi"])
+
+
+def _collapsar_oracle(nx=16, ny=12, nz=96, **kw):
+    from cumc3d_amd.config import collapsar
+    cfg = collapsar(nx, ny, nz, n_dim=3 if nz > 1 else 2, **kw)
+    return cfg, O.Oracle(cfg, ic.collapsar(cfg))
+
+
+def _stencil(o, cfg, f):
+    """poisson_coef's seven-point operator applied to f (a ghosted-box array) on the interior."""
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    I = (slice(3, nz + 3), slice(3, ny + 3), slice(3, nx + 3))
+    sh = lambda dl, dk, dj: f[3 + dl:nz + 3 + dl, 3 + dk:ny + 3 + dk, 3 + dj:nx + 3 + dj]
+    aj, am = o.poisson("ajp1")[1:][None, None, :], o.poisson("ajm1")[1:][None, None, :]
+    bp, bm = o.poisson("bkp1")[1:, 1:][None], o.poisson("bkm1")[1:, 1:][None]
+    cp, cm, e = o.poisson("clp1")[I], o.poisson("clm1")[I], o.poisson("epsc")[I]
+    return (aj * sh(0, 0, 1) + am * sh(0, 0, -1) + bp * sh(0, 1, 0) + bm * sh(0, -1, 0)
+            + cp * sh(1, 0, 0) + cm * sh(-1, 0, 0) + e * sh(0, 0, 0))
+
+
+def test_poisson_operator_against_the_laplacian():
+    """The coefficients of poisson_coef (poisson.f90:35-64), as the oracle
+    forms them, discretise the spherical Laplacian: they annihilate constants,
+    are exact for r^2 (Laplacian 6) on the non-uniform radial grid, give the
+    closed form (2 cos(dphi) - 2)/dphi^2 cos(phi)/(r sin(theta))^2 for cos(phi)
+    on the uniform azimuthal grid, and converge at second order for cos(theta)
+    (Laplacian -2 cos(theta)/r^2).  A dropped or mis-signed term fails one."""
+    cfg, o = _collapsar_oracle()
+    Z, Y, X = np.meshgrid(o.grid.z.c, o.grid.y.c, o.grid.x.c, indexing="ij")
+    I = (slice(3, cfg.nz + 3), slice(3, cfg.ny + 3), slice(3, cfg.nx + 3))
+    scale = np.abs(o.poisson("epsc")[I])
+    assert np.abs(_stencil(o, cfg, np.ones(cfg.shape))).max() < 1e-12 * scale.max()
+    lap = _stencil(o, cfg, X ** 2)
+    assert np.allclose(lap, 6.0, rtol=0, atol=1e-8 * (scale * X[I] ** 2).max())
+    dphi = o.grid.z.d[3]
+    want = (2 * np.cos(dphi) - 2) / dphi ** 2 * np.cos(Z[I]) / (X[I] * np.sin(Y[I])) ** 2
+    assert np.allclose(_stencil(o, cfg, np.cos(Z)), want, rtol=1e-9, atol=1e-12 * scale.max())
+    errs = []
+    for ny in (16, 32):
+        c2, o2 = _collapsar_oracle(nx=8, ny=ny, nz=1)
+        Z2, Y2, X2 = np.meshgrid(o2.grid.z.c, o2.grid.y.c, o2.grid.x.c, indexing="ij")
+        I2 = (slice(3, 4), slice(4, ny + 2), slice(3, 11))          # away from the polar ghost rows
+        got = _stencil(o2, c2, np.cos(Y2))[:, 1:-1, :]
+        exact = -2 * np.cos(Y2[I2]) / X2[I2] ** 2
+        errs.append(np.abs(got - exact).max() / np.abs(exact).max())
+    assert 3.0 < errs[0] / errs[1] < 5.0, errs
+
+
+def test_potential_solves_the_discrete_equation():
+    """After a converged solve the potential satisfies the discrete Poisson
+    equation operator(phi) = 4 pi rho_in (rho above the atmosphere) up to the
+    SOR tolerance, with the boundary values of custom.f90:601-631."""
+    from cumc3d_amd.config import collapsar
+    cfg = collapsar(24, 16, n_dim=2)
+    o = O.Oracle(cfg, ic.collapsar(cfg))
+    o.run(1)                                     # step 0: the stage-3 solve
+    g = o.gravity()
+    assert g["status"] == 0 and 1 < g["iterations"] < cfg.relax_max
+    phi = g["phi"].copy()
+    nx, ny = cfg.nx, cfg.ny
+    assert np.all(phi[3, 3:ny + 3, nx + 3] == -1.0)
+    assert np.array_equal(phi[3, 3:ny + 3, 2], phi[3, 3:ny + 3, 3])
+    assert np.array_equal(phi[3, 2, 3:nx + 3], phi[3, 3, 3:nx + 3])
+    rho = o.get("rho")
+    pa = ic.collapsar(cfg)["gravity"][0]
+    I = (slice(3, 4), slice(3, ny + 3), slice(3, nx + 3))
+    rho_in = np.where(rho[I] - pa > 0, rho[I], 0.0)
+    phi[2], phi[4] = phi[3], phi[3]              # periodic phi ghosts of one cell, as the last sweep set them
+    res = _stencil(o, cfg, phi) - 4 * np.pi * rho_in
+    assert np.abs(res).max() < 1e-6 * np.abs(o.poisson("epsc")[I]).max()
+
+
+def test_collapsar_stops_where_the_reference_stops():
+    """custom.f90:481-484: r_sh = 2 m_bh/c^2 reaching xF(0) stops the run;
+    :644-647: a solve that converges on sweep relax_max stops it (one that runs
+    out does not)."""
+    from cumc3d_amd.config import collapsar
+    cfg = collapsar(16, 12, n_dim=2)
+    st = ic.collapsar(cfg)
+    c2 = (ic.CLIGHT * st["gravity"][10]) ** 2
+    big = dict(st, gravity=st["gravity"].copy())
+    big["gravity"][9] = 0.5 * cfg.faces[0][3] * c2 * 1.0001
+    o = O.Oracle(cfg, big)
+    assert o.gravity()["status"] == 1
+    assert len(o.run(3)) == 0
+    o = O.Oracle(cfg.with_(relax_max=1), st)     # the first solve ends at sweep 1 (see above)
+    assert o.gravity()["status"] == 2
+    o = O.Oracle(cfg.with_(relax_max=3), st)
+    o.run(1)                                     # the stage-3 solve runs out: n = relax_max + 1, no stop
+    g = o.gravity()
+    assert g["status"] == 0 and g["iterations"] == 4
+
+
+def test_collapsar_profile_reader():
+    """ic.read_collapsar_profile reads the profile directory of
+    model/collapsar (faces, hydro.dat, par.dat) into a state that the
+    oracle runs; skipped where the reference tree is absent."""
+    path = "/root/reference/model/collapsar/profile"
+    if not os.path.isdir(path):
+        pytest.skip("reference profile directory not present")
+    from cumc3d_amd.config import collapsar
+    prof = ic.read_collapsar_profile(path)
+    nx, ny, nz = len(prof["xF"]) - 7, len(prof["yF"]) - 7, len(prof["zF"]) - 7
+    assert (nx, ny, nz) == (156, 44, 128) and prof["ggas"] == 4.0 / 3.0
+    cfg = collapsar(nx, ny, nz, faces=(tuple(prof["xF"]), tuple(prof["yF"]), tuple(prof["zF"])), ggas=prof["ggas"])
+    st = ic.collapsar(cfg, profile=prof)
+    assert st["gravity"][0] == prof["rho"][nx + 2] and st["gravity"][9] == prof["m_bh"]
+    assert np.array_equal(st["p"][5, 5, :], prof["p"])
