@@ -126,7 +126,72 @@ def make_workload(name: str, n: int):
     if name == "torus_cyl":
         cfg = torus_cyl(n, n // 2, n, recon="ppm")
         return cfg, ic.torus(cfg), f"cylindrical torus (r, phi, z) {cfg.nx}x{cfg.ny}x{cfg.nz}, HLLD + PPM"
+    if name == "collapsar":
+        from cumc3d_amd.config import collapsar
+        # n = radial cells; n = 156 is the reference's shipped progenitor grid
+        # (model/collapsar/profile: 156 x 44 x 128)
+        cfg = collapsar(n, max(8, (44 * n) // 156), max(96, (128 * n) // 156), recon="ppm", riemann="hlld")
+        return (cfg, ic.collapsar(cfg),
+                f"collapsar (r, theta, phi) {cfg.nx}x{cfg.ny}x{cfg.nz} with self-gravity (SOR every "
+                f"{cfg.n_pot} steps) and black-hole accretion, HLLD + PPM, FP64")
     raise SystemExit(f"unknown workload {name}")
+
+
+def sor_bytes(cfg) -> float:
+    """Algorithmic bytes of one SOR iteration (red and black sweeps,
+    custom.f90:537-584): rho, the three 3D coefficients clp1/clm1/epsc and
+    phi read once per cell, phi written once per cell; the 1D/2D
+    coefficients and the 7-point neighbour re-reads are cache traffic."""
+    return 48.0 * cfg.zones
+
+
+def bench_collapsar(args):
+    """The collapsar workload (FP64; self-gravity has no FP32 split): wall
+    time of `steps` steps after `warmup` (the warmup holds step 0, whose
+    potential solve starts from the initial update's single sweep), the
+    per-kernel times and the Poisson solves of the same number of steps with
+    device timing on."""
+    from cumc3d_amd.solver import make_solver
+    cfg, st, label = make_workload("collapsar", args.n)
+    print(f"[bench] {label}: initial update + warmup {args.warmup}", file=sys.stderr, flush=True)
+    s = make_solver(cfg, st)
+    s.run(args.warmup, total_time=1e300, sync_each_step=False)
+    s.set_timing(True)
+    s.run(args.steps, total_time=1e300, sync_each_step=False)
+    kt = s.timing()
+    s.set_timing(False)
+    print(f"[bench] timed {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    s.run(args.steps, total_time=1e300, sync_each_step=False)
+    wall = time.perf_counter() - t0
+    g = s.gravity()
+    del s
+    zones = cfg.zones
+    value = zones * args.steps / wall
+    solves = args.steps // cfg.n_pot
+    sweep_ms = kt["poisson"] / kt["sor_sweeps"] if kt["sor_sweeps"] else None
+    achieved = sor_bytes(cfg) / (sweep_ms * 1e-3) / 1e9 if sweep_ms else None
+    dom = max(KERNELS, key=lambda k: kt[k])
+    step_kernels = sum(kt[k] for k in KERNELS)
+    line = {
+        "metric": "zone-updates/s (collapsar with self-gravity, FP64)", "value": value, "unit": "zone-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (cumc3d_amd.ic.collapsar: power-law envelope around a cavity, rotating, uniform B)",
+        "config": {"workload": label, "zones": zones, "n_pot": cfg.n_pot, "solves_per_window": solves},
+        "gravity": {"poisson_ms_per_solve": kt["poisson"], "sor_sweeps_per_solve": kt["sor_sweeps"],
+                    "sor_ms_per_sweep": sweep_ms, "m_bh": g["m_bh"], "last_solve_sweeps": g["iterations"],
+                    "poisson_ms_per_step": kt["poisson"] * solves / args.steps if solves else 0.0},
+        "roofline": {"bound": "hbm", "kernel": "sor sweep pair (k_sor_sweep<0>, k_sor_sweep<1>) + k_sor_post",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "algorithmic_bytes": sor_bytes(cfg),
+                     "step_kernels_ms": {k: kt[k] for k in KERNELS}, "step_kernel_sum_ms": 3 * step_kernels,
+                     "dominant_step_kernel": dom},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    return 0
 
 
 def profile_io(cfg, st, variant: str, sel_w=None, steps: int = 3):
@@ -346,7 +411,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--workload", default="ot", choices=["ot", "blast", "turbulence", "torus_sph", "torus_cyl"])
+    ap.add_argument("--workload", default="ot",
+                    choices=["ot", "blast", "turbulence", "torus_sph", "torus_cyl", "collapsar"])
     ap.add_argument("--profile-io", action="store_true",
                     help="also time the profile output path (device pack, copy, HDF5 write)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -365,6 +431,8 @@ def main(argv=None):
               "(the halo-exchange design is described in DESIGN.md); run with --gpus 1.")
         return 2
 
+    if args.workload == "collapsar":
+        return bench_collapsar(args)
     cfg, st, label = make_workload(args.workload, args.n)
     zones = cfg.zones
     print(f"[bench] training the selector on a {args.train_n}^3 run", file=sys.stderr, flush=True)

@@ -156,14 +156,15 @@ static SorArgs sor_args(cumc3d_ctx *x) {
     memset(&a, 0, sizeof(a));
     a.nx = x->nx; a.ny = x->ny; a.nz = x->nz;
     a.NX = x->NX; a.NY = x->NY;
-    a.phi = x->phi;
-    a.err = x->sor_err;
+    a.pin = x->phi;
+    a.pout = x->phi_alt;
     a.rho = (const double *)x->P[0];
     a.ajp1 = x->ajp1; a.ajm1 = x->ajm1; a.bkp1 = x->bkp1; a.bkm1 = x->bkm1;
     a.clp1 = x->clp1; a.clm1 = x->clm1; a.epsc = x->epsc;
     a.part_nan = x->part_nan;
     a.part_max = x->part_max;
-    a.nerr = x->sor_nerr;
+    a.nblk = x->sor_nblk;
+    a.nbx = (int)nblk((long)x->nx * x->ny, SOR_SPAN);
     a.relax_max = x->grav_p.relax_max;
     a.pa = x->grav_p.prim_a[0];
     a.omega = x->grav_p.omega;
@@ -173,26 +174,40 @@ static SorArgs sor_args(cumc3d_ctx *x) {
     return a;
 }
 
-// The RBSOR loop (custom.f90:510-647): sweeps queued in growing batches, the
-// end flag read between batches.  A solve that converged on sweep relax_max
-// sets grav_error (the reference STOPs there); one that ran out continues.
+// The RBSOR loop (custom.f90:510-647): sweep pairs queued in growing batches,
+// the end flag read between batches; the potential alternates between its
+// two buffers, and the one the last pair wrote becomes x->phi with its
+// boundary values set.  A solve that converged on sweep relax_max sets
+// grav_error (the reference STOPs there); one that ran out continues.
 static int sor_solve(cumc3d_ctx *x, bool zero_guess) {
     hipStream_t st = x->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (x->timing) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, st));
+    }
     if (zero_guess) hipLaunchKernelGGL(k_fill0, dim3(nblk(x->G)), dim3(256), 0, st, x->phi, x->G);
     hipLaunchKernelGGL(k_sor_begin, dim3(1), dim3(1), 0, st, x->S);
-    const SorArgs a = sor_args(x);
+    SorArgs a = sor_args(x);
     const long hx = (x->nx + 1) / 2;
-    const dim3 gs(nblk(hx * x->ny * x->nz));
-    const dim3 gp((unsigned)a.nerr + nblk((long)x->ny * x->nz + (long)x->nx * x->nz + (long)x->nx * x->ny));
+    // red: one more block column, whose block (0, 0) decides the previous pair
+    const dim3 gr(nblk(hx * x->ny) + 1, x->nz), gb(nblk((long)x->nx * x->ny, SOR_SPAN), x->nz);
+    double *buf[2] = {x->phi, x->phi_alt};
     long queued = 0;
     int batch = 4, flags[3] = {0, 0, 0};
     for (;;) {
         for (int i = 0; i < batch; ++i) {
-            hipLaunchKernelGGL(k_sor_sweep<0>, gs, dim3(256), 0, st, a);
-            hipLaunchKernelGGL(k_sor_sweep<1>, gs, dim3(256), 0, st, a);
-            hipLaunchKernelGGL(k_sor_post, gp, dim3(256), 0, st, a);
+            a.pin = buf[queued & 1];
+            a.pout = buf[(queued + 1) & 1];
+            a.zero_ghosts = zero_guess && queued == 0;
+            a.combine = i > 0;
+            hipLaunchKernelGGL(k_sor_red, gr, dim3(256), 0, st, a);
+            hipLaunchKernelGGL(k_sor_black, gb, dim3(256), 0, st, a);
+            ++queued;
         }
-        queued += batch;
+        hipLaunchKernelGGL(k_sor_decide, dim3(1), dim3(256), 0, st, a);
+        if (e1) HIPCHK(hipEventRecord(e1, st));
         HIPCHK(hipMemcpyAsync(flags, &x->S->sor_state, sizeof(flags), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (flags[0]) break;
@@ -200,7 +215,22 @@ static int sor_solve(cumc3d_ctx *x, bool zero_guess) {
             return fail(CUMC3D_EHIP, "Poisson solve: sweep count not reported by the device");
         if (batch < 256) batch *= 2;
     }
+    if (flags[1] & 1) std::swap(x->phi, x->phi_alt);   // the buffer the last pair wrote
+    if (flags[1] > 0)
+        hipLaunchKernelGGL(k_sor_bc, dim3(nblk((long)x->ny * x->nz + (long)x->nx * x->nz + (long)x->nx * x->ny)),
+                           dim3(256), 0, st, x->phi, x->nx, x->ny, x->nz, x->NX, x->NY);
     if (flags[2] == x->grav_p.relax_max) x->grav_error = 1;
+    if (e0) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess && flags[1] > 0) {
+            x->tsum[8] += ms;
+            x->tcnt[8] += 1;
+            x->tsum[9] += flags[1];
+            x->tcnt[9] += 1;
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+    }
     return 0;
 }
 
@@ -209,9 +239,15 @@ static int grav_stage(cumc3d_ctx *x, int stage) {
     if (x->d.model != CUMC3D_MODEL_COLLAPSAR) return 0;
     const ArgsT<double> a = make_args<double>(x);
     const double c = x->grav_p.c_code;
-    if (stage == 1) hipLaunchKernelGGL(k_accrete<1>, dim3(1), dim3(64), 0, x->stream, a, c);
-    else if (stage == 2) hipLaunchKernelGGL(k_accrete<2>, dim3(1), dim3(64), 0, x->stream, a, c);
-    else hipLaunchKernelGGL(k_accrete<3>, dim3(1), dim3(64), 0, x->stream, a, c);
+    // mdot on the side stream from the end of this stage's x sweep, beside
+    // the y/z sweeps, EMFs, update and ghost fill; m_bh after both
+    HIPCHK(hipStreamWaitEvent(x->side, x->ev_x, 0));
+    hipLaunchKernelGGL(k_mdot, dim3(1), dim3(256), 0, x->side, a);
+    HIPCHK(hipEventRecord(x->ev_m, x->side));
+    HIPCHK(hipStreamWaitEvent(x->stream, x->ev_m, 0));
+    if (stage == 1) hipLaunchKernelGGL(k_mbh<1>, dim3(1), dim3(1), 0, x->stream, x->S, c, x->xF0);
+    else if (stage == 2) hipLaunchKernelGGL(k_mbh<2>, dim3(1), dim3(1), 0, x->stream, x->S, c, x->xF0);
+    else hipLaunchKernelGGL(k_mbh<3>, dim3(1), dim3(1), 0, x->stream, x->S, c, x->xF0);
     if (stage == 3 && x->cur_step % x->grav_p.n_pot == 0) return sor_solve(x, false);
     return 0;
 }
@@ -509,16 +545,21 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         if (hipMemset(x->stale, 0, (size_t)3 * x->G) != hipSuccess) { cleanup(); return fail(CUMC3D_EHIP, "memset"); }
     }
     if (d.model == CUMC3D_MODEL_COLLAPSAR) {
-        // phi (G), error terms (N), the coefficients, the error blocks' partials
-        const long N = (long)d.nx * d.ny * d.nz;
-        x->sor_nerr = (int)nblk(N);
-        const size_t bytes = ((size_t)x->G + N + pc.size() + 2 * (size_t)x->sor_nerr) * sizeof(double);
+        if (hipStreamCreateWithFlags(&x->side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&x->ev_x, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&x->ev_m, hipEventDisableTiming) != hipSuccess) {
+            cleanup();
+            return fail(CUMC3D_EHIP, "accretion stream");
+        }
+        // phi and its second buffer (G each), the coefficients, the error blocks' partials
+        x->sor_nblk = (int)nblk((long)d.nx * d.ny, SOR_SPAN) * d.nz;
+        const size_t bytes = (2 * (size_t)x->G + pc.size() + 2 * (size_t)x->sor_nblk) * sizeof(double);
         if (hipMalloc(&x->grav, bytes) != hipSuccess) { cleanup(); return fail(CUMC3D_ENOMEM, "gravity arrays"); }
         if (hipMemset(x->grav, 0, bytes) != hipSuccess) { cleanup(); return fail(CUMC3D_EHIP, "memset"); }
         double *g = (double *)x->grav;
         x->phi = g;
-        x->sor_err = g + x->G;
-        double *c0 = x->sor_err + N;
+        x->phi_alt = g + x->G;
+        double *c0 = x->phi_alt + x->G;
         if (hipMemcpy(c0, pc.data(), pc.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
             cleanup();
             return fail(CUMC3D_EHIP, "poisson coefficient upload");
@@ -526,7 +567,7 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         x->ajp1 = c0 + pco[0]; x->ajm1 = c0 + pco[1]; x->bkp1 = c0 + pco[2]; x->bkm1 = c0 + pco[3];
         x->clp1 = c0 + pco[4]; x->clm1 = c0 + pco[5]; x->epsc = c0 + pco[6];
         x->part_max = c0 + pc.size();
-        x->part_nan = (long long *)(x->part_max + x->sor_nerr);
+        x->part_nan = (long long *)(x->part_max + x->sor_nblk);
     }
     *out = x;
     return rc;
@@ -638,7 +679,7 @@ int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_st
             x->sel_timed.assign(TSTAGES, 0);
             for (auto &e : x->ev) HIPCHK(hipEventCreate(&e));
         }
-        for (int i = 0; i < 8; ++i) { x->tsum[i] = 0; x->tcnt[i] = 0; }
+        for (int i = 0; i < 10; ++i) { x->tsum[i] = 0; x->tcnt[i] = 0; }
         x->sel_timed.assign(TSTAGES, 0);
         x->tstage = 0;
     }
@@ -787,12 +828,16 @@ int cumc3d_divb(cumc3d_ctx *x, double *maxdivb) {
 int cumc3d_finalize(cumc3d_ctx *x) {
     if (!x) return 0;
     if (x->stream) hipStreamSynchronize(x->stream);
+    if (x->side) hipStreamSynchronize(x->side);
     for (auto &e : x->ev) hipEventDestroy(e);
     if (x->pool) hipFree(x->pool);
     if (x->met) hipFree(x->met);
     if (x->S) hipFree(x->S);
     if (x->stale) hipFree(x->stale);
     if (x->grav) hipFree(x->grav);
+    if (x->ev_x) hipEventDestroy(x->ev_x);
+    if (x->ev_m) hipEventDestroy(x->ev_m);
+    if (x->side) hipStreamDestroy(x->side);
     void *sel[] = {x->sel_w, x->sel_m, x->sel_v, x->sel_frag, x->mask, x->samp_feat, x->samp_label,
                    x->samp_counts, x->grad_part, x->loss_part, x->loss_dev};
     for (void *q : sel)
@@ -978,7 +1023,7 @@ int cumc3d_set_timing(cumc3d_ctx *x, int enable) {
 
 int cumc3d_get_timing(cumc3d_ctx *x, double *ms, int n) {
     if (!x || !ms) return fail(CUMC3D_EINVAL, "null argument");
-    for (int i = 0; i < n && i < 8; ++i) ms[i] = x->tcnt[i] ? x->tsum[i] / x->tcnt[i] : 0.0;
+    for (int i = 0; i < n && i < 10; ++i) ms[i] = x->tcnt[i] ? x->tsum[i] / x->tcnt[i] : 0.0;
     return 0;
 }
 

@@ -49,21 +49,24 @@ struct cumc3d_ctx {
     std::vector<hipEvent_t> ev;   // TMARKS marks per stage, TSTAGES stages
     std::vector<char> sel_timed;  // stage refreshed the selector masks
     int tstage = 0;               // stages recorded in the current run
-    double tsum[8] = {0};
-    long tcnt[8] = {0};
+    double tsum[10] = {0};        // [8] Poisson solve ms, [9] SOR sweeps per solve
+    long tcnt[10] = {0};
     // collapsar self-gravity (model/collapsar/src/def.f90:16-36): one
-    // allocation holding the potential, the per-cell SOR error terms, the
-    // poisson_coef coefficients and the error blocks' partials
+    // allocation holding the potential's two buffers, the poisson_coef
+    // coefficients and the error blocks' partials
     cumc3d_gravity_params grav_p{};
     void *grav = nullptr;
-    double *phi = nullptr, *sor_err = nullptr, *ajp1 = nullptr, *ajm1 = nullptr, *bkp1 = nullptr, *bkm1 = nullptr;
+    double *phi = nullptr, *phi_alt = nullptr;   // the potential and the SOR's second buffer
+    double *ajp1 = nullptr, *ajm1 = nullptr, *bkp1 = nullptr, *bkm1 = nullptr;
     double *clp1 = nullptr, *clm1 = nullptr, *epsc = nullptr, *part_max = nullptr;
     long long *part_nan = nullptr;
-    int sor_nerr = 0;
+    int sor_nblk = 0;
     double xF0 = 0.0;             // xF(0), the inner radial face (horizon check)
     long nstep_total = 0;         // steps since the initial update (main.f90's n_step)
     long cur_step = 0;            // n_step of the step being queued
     int grav_error = 0;           // 1: a solve converged on its last allowed sweep (the reference STOPs)
+    hipStream_t side = nullptr;   // the accretion sum runs here beside the stage's later kernels
+    hipEvent_t ev_x = nullptr, ev_m = nullptr;   // x sweep done (main stream); mdot done (side)
 };
 
 namespace cumc3d {
@@ -353,6 +356,7 @@ template <class W> static void run_stage_t(cumc3d_ctx *x, int stage) {
     const long nn = (long)x->nx * x->ny * x->nz;
     tmark(x, 0);
     launch_sweep(x, a, 0, st);
+    if (x->ev_x) hipEventRecord(x->ev_x, st);   // the face-0 density flux the accretion sum reads
     tmark(x, 1);
     if (x->d.n_dim > 1) launch_sweep(x, a, 1, st);
     tmark(x, 2);
@@ -360,7 +364,7 @@ template <class W> static void run_stage_t(cumc3d_ctx *x, int stage) {
     tmark(x, 3);
     hipLaunchKernelGGL(k_emf_prep<W>, dim3(nblk(emf_prep_count(a))), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_edge<W>, dim3(nblk((long)(x->nx + 1) * (x->ny + 1) * (x->nz + 1))), dim3(256), 0, st, a);
-    if (a.fixpole) hipLaunchKernelGGL(k_fixpole<W>, dim3(nblk(2L * x->nx)), dim3(256), 0, st, a);
+    if (a.fixpole) hipLaunchKernelGGL(k_fixpole<W>, dim3(2 * x->nx), dim3(64), 0, st, a);
     tmark(x, 4);
     launch_update<W>(x->d.coordinate_flag, stage, dim3(nblk(nn)), st, a);
     tmark(x, 5);

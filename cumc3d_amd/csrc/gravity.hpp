@@ -2,16 +2,26 @@
 // CUSTOM_UPDATE (model/collapsar/src/custom.f90:416-658) that run after
 // FINDPRESSURE in UPDATE(p).
 //
-//   k_accrete<STAGE>  mass accretion rate through the inner face from the x
-//                     sweep's density flux at face 0 (GETFLUX_X, :222-245),
-//                     the RK update of m_bh, r_sh and the horizon check
-//   k_sor_sweep<C>    one colour of a red-black SOR sweep of the potential
-//   k_sor_post        the error fold over all cells and the boundary values
+//   k_mdot            mass accretion rate through the inner face from the x
+//                     sweep's density flux at face 0 (GETFLUX_X, :222-245)
+//   k_mbh<STAGE>      the RK update of m_bh, r_sh and the horizon check
+//   k_sor_red         the red half of one SOR sweep pair (and the decision
+//                     of the pair before it)
+//   k_sor_black       the black half and the blocks' error partials
+//   k_sor_decide      the decision of a batch's last pair
+//   k_sor_bc          the boundary values of the final potential
 //
-// FP64 only.  The SOR sweeps are bit-identical to the serial loop: a red cell
+// FP64 only.  The sweeps are bit-identical to the serial loop: a red cell
 // ((-1)**(j+k+l) > 0) reads only black neighbours and vice versa, so the order
-// within a sweep does not matter, and each sweep computes the relative change
-// of its own cells (the value read before the update is phi_old).
+// within a sweep does not matter.  The potential ping-pongs between two
+// buffers: a sweep pair reads `pin` (phi at the start of the pair, with its
+// boundary values) and writes every interior cell of `pout`; red cells read
+// pin, black cells read the new red values from pout.  The boundary values a
+// pair reads are those the reference's previous pair set (custom.f90:601-631),
+// formed on the fly from pin: phi(0) = phi(1), phi(nx+1) = -1, zero theta
+// gradient, periodic in phi -- and 0 on the first pair after the zero guess
+// of UPDATE(0) (:497-505), which had set no boundary values yet.  k_sor_bc
+// writes them into the final buffer for the gravity source.
 // The error is the OpenMP MAX reduction of :588-598 as the reference's
 // one-thread CPU build evaluates it: a private fold of
 // MAX(abserror, |dphi/phi_old|) in l, k, j order with flang's MAX
@@ -19,29 +29,34 @@
 // replaces the NaN -- then a combine with the shared 1e-50 that drops a NaN.
 // So the result is 1e-50 when the last term is NaN (0/0 where phi stays 0,
 // as on the first sweep from the zero guess), else the larger of 1e-50 and
-// the maximum of the terms after the last NaN.  k_sor_post forms it exactly
-// from per-block partials (the last NaN index of the block and the maximum of
-// the terms after it), combined by the last block to finish.  The host runs
-// sweeps in batches and reads the end flag between batches; sweeps queued
-// after the solve ended return at once.
+// the maximum of the terms after the last NaN.  k_sor_black leaves per-block
+// partials (the last NaN index of the block and the maximum of the terms
+// after it) and one block of the next kernel combines them exactly.  The
+// host runs sweep pairs in batches and reads the end flag between batches;
+// pairs queued after the solve ended return at once.
 #pragma once
 #include "kernels.hpp"
 
 namespace cumc3d {
 
+constexpr int SOR_SPAN = 1024;   // interior cells per black block (4 per thread)
+
 struct SorArgs {
     int nx, ny, nz;
     long NX, NY;
-    double *phi;                        // ghosted box, cidx layout
-    double *err;                        // |dphi/phi_old| per interior cell, q = (j-1) + nx((k-1) + ny(l-1))
+    const double *pin;                  // phi at the start of the sweep pair, ghosted box (cidx layout)
+    double *pout;                       // phi after it (interior cells)
     const double *rho;                  // prim(irho), ghosted box
     const double *ajp1, *ajm1;          // [j], j = 1..nx
     const double *bkp1, *bkm1;          // [k(nx+1) + j]
-    const double *clp1, *clm1, *epsc;   // [q]
-    long long *part_nan;                // per error block: index of its last NaN term, -1 if none
-    double *part_max;                   // per error block: max of the terms after it, -1 if none
-    int nerr;                           // error blocks (256 terms each)
+    const double *clp1, *clm1, *epsc;   // [q], q = (j-1) + nx((k-1) + ny(l-1))
+    long long *part_nan;                // per black block: q of its last NaN term, -1 if none
+    double *part_max;                   // per black block: max of the terms after it, -1 if none
+    int nblk;                           // black blocks: ceil(nx ny / SOR_SPAN) per plane, times nz
+    int nbx;                            // black blocks per plane
     int relax_max;
+    int zero_ghosts;                    // the first pair after the zero guess: boundary values 0
+    int combine;                        // red kernel: block 0 first decides the previous pair's exit
     double pa, omega, tol, pi4;         // prim_a(irho), omega_weight, tolerance, 4 pi
     DevScalars *S;
 };
@@ -56,34 +71,49 @@ __device__ __forceinline__ void grav_horizon(DevScalars *S, double c_code, doubl
     }
 }
 
-// custom.f90:450-484 for p = STAGE: mdot = sum over l, k (serial order) of
-// -mflux_x(k,l) xF(0)^2 dcose(k) dz(l), clipped at 0, then the RK combine of
-// m_bh.  The 64 lanes form the terms; lane 0 adds them in the serial order.
-template <int STAGE> __global__ __launch_bounds__(64) void k_accrete(ArgsT<double> a, double c_code) {
+// custom.f90:450-465: mdot = sum over l, k (serial order) of
+// -mflux_x(k,l) xF(0)^2 dcose(k) dz(l), clipped at 0, from the x sweep's
+// density flux at face 0.  256 threads form up to 4096 terms at a time into
+// LDS (all loads in flight together); one wave then adds them in the serial
+// order, reading ahead of the dependent add chain, which is the cost.  Runs
+// on a second stream beside the stage's later kernels.
+constexpr int ACC_CHUNK = 4096;
+__global__ __launch_bounds__(256) void k_mdot(ArgsT<double> a) {
     if (!a.S->active) return;
-    __shared__ double term[64];
-    const int lane = threadIdx.x;
+    __shared__ double term[ACC_CHUNK];
     const int nk = a.ny * a.nz;
     const double xf0 = a.ax[0].xF[3];
     double mdot = 0.0;
-    for (int base = 0; base < nk; base += 64) {
-        const int t = base + lane;
-        if (t < nk) {
+    for (int base = 0; base < nk; base += ACC_CHUNK) {
+        const int m = nk - base < ACC_CHUNK ? nk - base : ACC_CHUNK;
+        for (int i = threadIdx.x; i < m; i += blockDim.x) {
+            const int t = base + i;
             const int k = 1 + t % a.ny, l = 1 + t / a.ny;
             const double mf = a.F[0][0][cidx(a, 0, k, l)];
-            term[lane] = (-mf) * xf0 * xf0 * a.dcose[k + 3] * a.ax[2].d[l + 3];
+            term[i] = (-mf) * xf0 * xf0 * a.dcose[k + 3] * a.ax[2].d[l + 3];
         }
         __syncthreads();
-        if (lane == 0) {
-            const int m = nk - base < 64 ? nk - base : 64;
-            for (int i = 0; i < m; ++i) mdot = mdot + term[i];
+        if (threadIdx.x < 64) {
+            int i = 0;
+            for (; i + 8 <= m; i += 8) {
+                double r[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) r[u] = term[i + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) mdot = mdot + r[u];
+            }
+            for (; i < m; ++i) mdot = mdot + term[i];
         }
         __syncthreads();
     }
-    if (lane != 0) return;
-    mdot = fmax2(mdot, 0.0);
-    DevScalars *S = a.S;
-    const double dt = S->dt;
+    if (threadIdx.x == 0) a.S->mdot = fmax2(mdot, 0.0);
+}
+
+// custom.f90:467-484 for p = STAGE: the RK combine of m_bh with the stage's
+// mdot, then r_sh and the horizon check
+template <int STAGE> __global__ void k_mbh(DevScalars *S, double c_code, double xF0) {
+    if (!S->active) return;
+    const double dt = S->dt, mdot = S->mdot;
     if constexpr (STAGE == 1) {
         S->mbh_old = S->m_bh;
         S->m_bh = S->mbh_old + dt * mdot;
@@ -92,7 +122,7 @@ template <int STAGE> __global__ __launch_bounds__(64) void k_accrete(ArgsT<doubl
     } else {
         S->m_bh = (1.0 / 3.0) * S->mbh_old + (2.0 / 3.0) * S->m_bh + (2.0 / 3.0) * dt * mdot;
     }
-    grav_horizon(S, c_code, xf0);
+    grav_horizon(S, c_code, xF0);
 }
 
 // UPDATE(0): m_bh as the initial model left it, then r_sh and the check
@@ -103,18 +133,17 @@ __global__ void k_grav_start(DevScalars *S, double m_bh, double c_code, double x
     grav_horizon(S, c_code, xF0);
 }
 
-// Start of a solve: the zero guess phi = 0 on cells 0..n+1 for p = 0
-// (custom.f90:491-507; the whole box here, the outer layers are never read)
-// and the iteration state.  An inactive run (finished or stopped) skips it.
+// Start of a solve: the iteration state (k_fill0 writes the zero guess phi = 0
+// of p = 0, custom.f90:491-507, over the whole box; the outer layers are
+// never read).  An inactive run (finished or stopped) skips the solve.
 __global__ void k_sor_begin(DevScalars *S) {
+    S->sor_n = 0;
     if (!S->active) {
         S->sor_state = 1;
         S->sor_iter = 0;
         return;
     }
     S->sor_state = 0;
-    S->sor_n = 0;
-    S->sor_ctr = 0;
 }
 
 __global__ __launch_bounds__(256) void k_fill0(double *p, long n) {
@@ -122,34 +151,39 @@ __global__ __launch_bounds__(256) void k_fill0(double *p, long n) {
     if (i < n) p[i] = 0.0;
 }
 
-// One colour of the sweep (custom.f90:537-584).  Threads walk the cells of
-// the colour along x (every second cell), rows in (k, l) order.
-template <int COLOUR> __global__ __launch_bounds__(256) void k_sor_sweep(SorArgs a) {
-    if (a.S->sor_state || !a.S->active) return;
-    const int hx = (a.nx + 1) >> 1;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)hx * a.ny * a.nz) return;
-    const int i = (int)(t % hx);
-    const long r = t / hx;
-    const int k = 1 + (int)(r % a.ny), l = 1 + (int)(r / a.ny);
-    const int j = (((1 + k + l) & 1) == COLOUR ? 1 : 2) + 2 * i;   // (j + k + l) & 1 == COLOUR
-    if (j > a.nx) return;
+__device__ __forceinline__ long sor_c(const SorArgs &a, int j, int k, int l) {
+    return (long)(j + 2) + a.NX * ((long)(k + 2) + a.NY * (long)(l + 2));
+}
+
+// The neighbour values one cell's update reads (custom.f90:548-550): cells
+// of the other colour from `src` (pin for a red cell, pout for a black one),
+// the boundary values from pin.
+__device__ __forceinline__ void sor_neigh(const SorArgs &a, const double *src, int j, int k, int l, long c,
+                                          double nb[6]) {
     const long sy = a.NX, sz = a.NX * a.NY;
-    const long c = (long)(j + 2) + a.NX * ((long)(k + 2) + a.NY * (long)(l + 2));
+    const double *pin = a.pin;
+    const bool z0 = a.zero_ghosts;
+    nb[0] = j < a.nx ? src[c + 1] : (z0 ? 0.0 : -1.0);
+    nb[1] = j > 1 ? src[c - 1] : (z0 ? 0.0 : pin[c]);
+    nb[2] = k < a.ny ? src[c + sy] : (z0 ? 0.0 : pin[c]);
+    nb[3] = k > 1 ? src[c - sy] : (z0 ? 0.0 : pin[c]);
+    nb[4] = l < a.nz ? src[c + sz] : (z0 ? 0.0 : pin[sor_c(a, j, k, 1)]);
+    nb[5] = l > 1 ? src[c - sz] : (z0 ? 0.0 : pin[sor_c(a, j, k, a.nz)]);
+}
+
+// one cell's relaxation (custom.f90:544-551); returns the new value
+__device__ __forceinline__ double sor_cell(const SorArgs &a, int j, int k, int l, long c, const double nb[6],
+                                           double old) {
     const long q = (long)(j - 1) + (long)a.nx * ((long)(k - 1) + (long)a.ny * (l - 1));
-    double *phi = a.phi;
     const double rho = a.rho[c];
     const double diff = rho - a.pa;
     const double factor = diff > 0.0 ? 1.0 : 0.0;
     const double rho_in = factor * rho;
-    const long jk = (long)k * (a.nx + 1) + j;
-    const double s = a.ajp1[j] * phi[c + 1] + a.ajm1[j] * phi[c - 1] + a.bkp1[jk] * phi[c + sy] +
-                     a.bkm1[jk] * phi[c - sy] + a.clp1[q] * phi[c + sz] + a.clm1[q] * phi[c - sz];
+    const int jk = k * (a.nx + 1) + j;
+    const double s = a.ajp1[j] * nb[0] + a.ajm1[j] * nb[1] + a.bkp1[jk] * nb[2] + a.bkm1[jk] * nb[3] +
+                     a.clp1[q] * nb[4] + a.clm1[q] * nb[5];
     const double rhs = (a.pi4 * rho_in - s) / a.epsc[q];
-    const double old = phi[c];
-    const double nw = (1.0 - a.omega) * old + a.omega * rhs;
-    phi[c] = nw;
-    a.err[q] = fabs((nw - old) / old);
+    return (1.0 - a.omega) * old + a.omega * rhs;
 }
 
 __device__ __forceinline__ long long block_max_ll(long long v) {
@@ -183,79 +217,36 @@ __device__ __forceinline__ double block_max_d(double v) {
     return m;
 }
 
-// Blocks [0, nerr): the error fold (custom.f90:587-598); the last of them to
-// finish combines the partials and decides the loop exit (:638, :644).
-// Blocks [nerr, ...): the boundary values (:601-631): phi(0) = phi(1),
-// phi(nx+1) = -1; zero gradient at both theta ends; periodic in phi.
-__global__ __launch_bounds__(256) void k_sor_post(SorArgs a) {
-    if (a.S->sor_state || !a.S->active) return;
-    const long N = (long)a.nx * a.ny * a.nz;
-    if ((int)blockIdx.x >= a.nerr) {
-        long t = (long)(blockIdx.x - a.nerr) * blockDim.x + threadIdx.x;
-        const long sy = a.NX, sz = a.NX * a.NY;
-        double *phi = a.phi;
-        auto ci = [&](int j, int k, int l) { return (long)(j + 2) + a.NX * ((long)(k + 2) + a.NY * (long)(l + 2)); };
-        if (t < (long)a.ny * a.nz) {
-            const int k = 1 + (int)(t % a.ny), l = 1 + (int)(t / a.ny);
-            const long c = ci(1, k, l);
-            phi[c - 1] = phi[c];
-            phi[ci(a.nx + 1, k, l)] = -1.0;
-            return;
-        }
-        t -= (long)a.ny * a.nz;
-        if (t < (long)a.nx * a.nz) {
-            const int j = 1 + (int)(t % a.nx), l = 1 + (int)(t / a.nx);
-            const long c1 = ci(j, 1, l), cn = ci(j, a.ny, l);
-            phi[c1 - sy] = phi[c1];
-            phi[cn + sy] = phi[cn];
-            return;
-        }
-        t -= (long)a.nx * a.nz;
-        if (t < (long)a.nx * a.ny) {
-            const int j = 1 + (int)(t % a.nx), k = 1 + (int)(t / a.nx);
-            const long c1 = ci(j, k, 1), cn = ci(j, k, a.nz);
-            phi[c1 - sz] = phi[cn];
-            phi[cn + sz] = phi[c1];
-        }
-        return;
-    }
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const double e = q < N ? a.err[q] : 0.0;
-    const bool nan = q < N && e != e;
-    const long long last = block_max_ll(nan ? (long long)q : -1LL);
-    const double m = block_max_d(q < N && !nan && q > last ? e : -1.0);
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
-        a.part_nan[blockIdx.x] = last;
-        a.part_max[blockIdx.x] = m;
-        __threadfence();
-        const unsigned done = atomicAdd(&a.S->sor_ctr, 1u);
-        is_last = done == (unsigned)(a.nerr - 1);
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
+// The previous pair's abserror from the black blocks' partials and the exit
+// decision (custom.f90:638, :644), by one block of 256 threads.
+__device__ void sor_decide(const SorArgs &a) {
+    const int nxy = a.nx * a.ny;
+    DevScalars *S = a.S;
     long long L = -1;
-    for (int b = threadIdx.x; b < a.nerr; b += blockDim.x) {
-        const long long v = a.part_nan[b];
+    for (int i = threadIdx.x; i < a.nblk; i += blockDim.x) {
+        const long long v = a.part_nan[i];
         L = v > L ? v : L;
     }
     L = block_max_ll(L);
-    const long long bL = L >= 0 ? L / 256 : 0;
+    // block i covers q = plane + (i % nbx) SOR_SPAN ... up to the plane's end
     double M = -1.0;
-    for (int b = threadIdx.x; b < a.nerr; b += blockDim.x) {
-        if (b < bL) continue;
-        const double v = a.part_max[b];
+    for (int i = threadIdx.x; i < a.nblk; i += blockDim.x) {
+        if (L >= 0) {
+            const long long plane = (long long)(i / a.nbx) * nxy;
+            long long last_q = plane + (long long)(i % a.nbx) * SOR_SPAN + SOR_SPAN - 1;
+            if (last_q > plane + nxy - 1) last_q = plane + nxy - 1;   // the plane's last block is partial
+            if (last_q < L) continue;   // wholly before the last NaN
+        }
+        const double v = a.part_max[i];
         M = v > M ? v : M;
     }
     M = block_max_d(M);
     if (threadIdx.x != 0) return;
-    DevScalars *S = a.S;
+    const long long N = (long long)nxy * a.nz;
     double err;
     if (L == N - 1) err = 1.0e-50;   // the private result is NaN: the combine keeps 1e-50
     else err = M > 1.0e-50 ? M : 1.0e-50;
     S->sor_err = err;
-    S->sor_ctr = 0;
     const int n = S->sor_n + 1;
     S->sor_n = n;
     if (err <= a.tol) {
@@ -264,6 +255,115 @@ __global__ __launch_bounds__(256) void k_sor_post(SorArgs a) {
     } else if (n == a.relax_max) {
         S->sor_state = 1;
         S->sor_iter = n + 1;   // the DO loop ran out: n = relax_max + 1 after it
+    }
+}
+
+// Red cells ((j + k + l) even), every second cell along x: grid (x: the
+// plane's red cells / 256 + 1 for the decision block, y: l - 1).  With
+// `combine`, block (0, 0) first decides the previous pair's exit; the other
+// blocks do not wait for it: a pair begun after the solve ended only writes
+// the buffer that is not the result, and the black kernel and every later
+// kernel see the decision and return.
+__global__ __launch_bounds__(256) void k_sor_red(SorArgs a) {
+    if (a.combine && blockIdx.x == 0 && blockIdx.y == 0) {
+        if (!a.S->sor_state && a.S->active) sor_decide(a);
+        return;
+    }
+    if (a.S->sor_state || !a.S->active) return;
+    const int hx = (a.nx + 1) >> 1;
+    const unsigned t = (blockIdx.x - 1) * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)(hx * a.ny)) return;
+    const int k = 1 + (int)(t / (unsigned)hx), i = (int)(t - (unsigned)(k - 1) * hx), l = 1 + blockIdx.y;
+    const int j = (((1 + k + l) & 1) == 0 ? 1 : 2) + 2 * i;
+    if (j > a.nx) return;
+    const long c = sor_c(a, j, k, l);
+    double nb[6];
+    sor_neigh(a, a.pin, j, k, l, c, nb);
+    a.pout[c] = sor_cell(a, j, k, l, c, nb, a.pin[c]);
+}
+
+// Every interior cell, x fastest: black cells relax from the new red values;
+// every cell forms its error term |dphi/phi_old| (custom.f90:593) from pin
+// and pout and the block folds them into its partial.  Grid (x: the plane's
+// cells / SOR_SPAN, y: l - 1): a block covers SOR_SPAN consecutive q of one
+// plane, SOR_SPAN / 256 cells per thread.
+__global__ __launch_bounds__(256) void k_sor_black(SorArgs a) {
+    if (a.S->sor_state || !a.S->active) return;
+    constexpr int PER = SOR_SPAN / 256;
+    const int nxy = a.nx * a.ny;
+    const int l = 1 + blockIdx.y;
+    double e[PER];
+    long long qq[PER];
+    long long last = -1;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const unsigned t = blockIdx.x * SOR_SPAN + u * 256 + threadIdx.x;
+        qq[u] = -1;
+        e[u] = -1.0;
+        if (t >= (unsigned)nxy) continue;
+        qq[u] = (long long)(l - 1) * nxy + t;
+        const int k = 1 + (int)(t / (unsigned)a.nx), j = 1 + (int)(t - (unsigned)(k - 1) * a.nx);
+        const long c = sor_c(a, j, k, l);
+        const double old = a.pin[c];
+        double nw;
+        if ((j + k + l) & 1) {
+            double nb[6];
+            sor_neigh(a, a.pout, j, k, l, c, nb);
+            nw = sor_cell(a, j, k, l, c, nb, old);
+            a.pout[c] = nw;
+        } else {
+            nw = a.pout[c];
+        }
+        e[u] = fabs((nw - old) / old);
+        if (e[u] != e[u]) last = qq[u];   // q grows with u: the thread's last NaN
+    }
+    last = block_max_ll(last);
+    double m = -1.0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+        if (qq[u] > last && e[u] == e[u]) m = e[u] > m ? e[u] : m;
+    m = block_max_d(m);
+    if (threadIdx.x == 0) {
+        const int b = blockIdx.y * gridDim.x + blockIdx.x;
+        a.part_nan[b] = last;
+        a.part_max[b] = m;
+    }
+}
+
+// The decision of a solve's last pair when no red kernel follows it.
+__global__ __launch_bounds__(256) void k_sor_decide(SorArgs a) {
+    if (!a.S->sor_state && a.S->active) sor_decide(a);
+}
+
+// The boundary values of the final potential (custom.f90:601-631), written
+// once a solve has ended: phi(0) = phi(1), phi(nx+1) = -1 over the interior
+// (k, l); theta ends copy their neighbours over (j, l); phi is periodic over
+// (j, k).
+__global__ __launch_bounds__(256) void k_sor_bc(double *phi, int nx, int ny, int nz, long NX, long NY) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long sy = NX, sz = NX * NY;
+    auto ci = [&](int j, int k, int l) { return (long)(j + 2) + NX * ((long)(k + 2) + NY * (long)(l + 2)); };
+    if (t < (long)ny * nz) {
+        const int k = 1 + (int)(t % ny), l = 1 + (int)(t / ny);
+        const long c = ci(1, k, l);
+        phi[c - 1] = phi[c];
+        phi[ci(nx + 1, k, l)] = -1.0;
+        return;
+    }
+    t -= (long)ny * nz;
+    if (t < (long)nx * nz) {
+        const int j = 1 + (int)(t % nx), l = 1 + (int)(t / nx);
+        const long c1 = ci(j, 1, l), cn = ci(j, ny, l);
+        phi[c1 - sy] = phi[c1];
+        phi[cn + sy] = phi[cn];
+        return;
+    }
+    t -= (long)nx * nz;
+    if (t < (long)nx * ny) {
+        const int j = 1 + (int)(t % nx), k = 1 + (int)(t / nx);
+        const long c1 = ci(j, k, 1), cn = ci(j, k, nz);
+        phi[c1 - sz] = phi[cn];
+        phi[cn + sz] = phi[c1];
     }
 }
 

@@ -31,8 +31,8 @@ struct DevScalars {
     int pad;
     // collapsar (model/collapsar/src/def.f90:34-39, custom.f90:416-649)
     double m_bh, mbh_old, r_sh;       // black-hole mass, its value at step start, Schwarzschild radius
+    double mdot;                      // the stage's accretion rate
     double sor_err;                   // abserror of the last SOR sweep pair
-    unsigned sor_ctr;                 // error blocks of the running post kernel that finished
     int sor_state;                    // 0 while the Poisson solve iterates, 1 once it ended
     int sor_n;                        // sweeps done in the running solve
     int sor_iter;                     // the DO variable n when the solve ended
@@ -653,17 +653,32 @@ template <class W> __global__ __launch_bounds__(256) void k_edge(ArgsT<W> a) {
 // (theta faces 0 and ny) becomes its average over phi, written to every phi
 // edge l=-2..nz+3.  The reference's accumulators sum1/sum2 are REAL (:293):
 // each addition is done in double and rounded to single, in the order of the
-// l loop of a one-thread run.  One thread per (radius j, pole); nz loads each.
-template <class W> __global__ __launch_bounds__(256) void k_fixpole(ArgsT<W> a) {
+// l loop of a one-thread run.  One wave per (radius j, pole): the lanes load
+// the ring into LDS, every lane runs the serial sum from it.
+template <class W> __global__ __launch_bounds__(64) void k_fixpole(ArgsT<W> a) {
     if (!a.S->active) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * a.nx) return;
-    const int j = 1 + (t >> 1), k = (t & 1) ? a.ny : 0;
+    constexpr int CH = 1024;
+    __shared__ W buf[CH];
+    const int j = 1 + (blockIdx.x >> 1), k = (blockIdx.x & 1) ? a.ny : 0;
     W *ex = a.E[0];
     float sum = 0.0f;
-    for (int l = 1; l <= a.nz; ++l) sum = (float)((double)sum + (double)ex[cidx(a, j, k, l)]);
+    for (int l0 = 1; l0 <= a.nz; l0 += CH) {
+        const int m = a.nz - l0 + 1 < CH ? a.nz - l0 + 1 : CH;
+        for (int i = threadIdx.x; i < m; i += 64) buf[i] = ex[cidx(a, j, k, l0 + i)];   // all loads in flight
+        __syncthreads();
+        int i = 0;
+        for (; i + 8 <= m; i += 8) {
+            W r[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) r[u] = buf[i + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) sum = (float)((double)sum + (double)r[u]);
+        }
+        for (; i < m; ++i) sum = (float)((double)sum + (double)buf[i]);
+        __syncthreads();
+    }
     const W avg = (W)((double)sum / (double)a.nz);
-    for (int l = -2; l <= a.nz + 3; ++l) ex[cidx(a, j, k, l)] = avg;
+    for (int l = -2 + (int)threadIdx.x; l <= a.nz + 3; l += 64) ex[cidx(a, j, k, l)] = avg;
 }
 
 // ---------------------------------------------------------------------------

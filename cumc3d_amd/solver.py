@@ -272,9 +272,10 @@ class Solver:
         _check(load_library().cumc3d_set_timing(self._h, int(on)))
 
     def timing(self) -> dict:
-        ms = np.zeros(8)
-        _check(load_library().cumc3d_get_timing(self._h, _p(ms), 8))
-        keys = ("flux_x", "flux_y", "flux_z", "edge", "update", "ghost", "step", "selector")
+        ms = np.zeros(10)
+        _check(load_library().cumc3d_get_timing(self._h, _p(ms), 10))
+        keys = ("flux_x", "flux_y", "flux_z", "edge", "update", "ghost", "step", "selector",
+                "poisson", "sor_sweeps")
         return dict(zip(keys, ms.tolist()))
 
     # -- learned reconstruction selector ---------------------------------------

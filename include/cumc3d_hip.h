@@ -171,7 +171,8 @@ const char *cumc3d_last_error(void);
 /* Mean device time in ms per launch of each kernel class over the last
  * cumc3d_run call with timing enabled: [0] x-flux, [1] y-flux, [2] z-flux,
  * [3] edge EMF, [4] update, [5] ghost fill, [6] whole step, [7] selector
- * inference (per refresh; n up to 8). */
+ * inference (per refresh); for the collapsar [8] one Poisson solve and [9]
+ * its mean number of SOR sweeps (a count, not ms).  n up to 10. */
 int cumc3d_set_timing(cumc3d_ctx *ctx, int enable);
 int cumc3d_get_timing(cumc3d_ctx *ctx, double *ms_per_launch, int n);
 
