@@ -137,6 +137,15 @@ def make_workload(name: str, n: int):
     raise SystemExit(f"unknown workload {name}")
 
 
+def sor_kernel_bytes(cfg, colour: str) -> float:
+    """Algorithmic bytes of one SOR kernel launch.  Red: rho and the three 3D
+    coefficients of the red cells (16 B per cell), phi of every cell read
+    once (8), the new red values written (4).  Black: the same for the black
+    cells (16), the new red and old black values read (8), the black values
+    written (4) and the old red values for the error terms (4)."""
+    return (28.0 if colour == "red" else 32.0) * cfg.zones
+
+
 def sor_bytes(cfg) -> float:
     """Algorithmic bytes of one SOR iteration (red and black sweeps,
     custom.f90:537-584): rho, the three 3D coefficients clp1/clm1/epsc and
@@ -358,15 +367,16 @@ def cpu_baseline(max_threads: int = 16):
             "sample": sample + f", {ref['seconds']:.1f} s loop time"}
 
 
-def trace_summary(path: str, n: int) -> str:
+def trace_summary(path: str, n: int, workload: str = "ot") -> str:
     """Markdown table of a rocprofv3 --kernel-trace CSV: per kernel, the
     dispatches at its largest grid (the n^3 workload; the selector's training
     run is smaller), with calls, mean/min/max duration and VGPRs, plus the
-    algorithmic bytes and GB/s of the stage kernels."""
+    algorithmic bytes and GB/s of the stage kernels (and, for the collapsar,
+    of the SOR kernels: red 28 B and black 32 B per cell, see sor_kernel_bytes)."""
     import csv
     from collections import defaultdict
     from cumc3d_amd.config import orszag_tang_3d
-    cfg = orszag_tang_3d(n)
+    cfg = make_workload(workload, n)[0] if workload != "ot" else orszag_tang_3d(n)
     groups = defaultdict(list)
     for r in csv.DictReader(open(path)):
         grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
@@ -393,6 +403,8 @@ def trace_summary(path: str, n: int) -> str:
             kb = kernel_bytes(cfg, "flux_y" if name.rstrip(")").split(",")[3].strip().startswith("1") else "flux_z", w)
         elif "k_update<" in name:
             kb = kernel_bytes(cfg, "update", w)
+        elif "k_sor_red" in name or "k_sor_black" in name:
+            kb = sor_kernel_bytes(cfg, "red" if "k_sor_red" in name else "black")
         mb = f"{kb / 1e6:.0f}" if kb else ""
         gbs = f"{kb / (mean * 1e-6) / 1e9:.0f}" if kb else ""
         short = name.replace("cumc3d::", "").replace("void ", "")
@@ -404,7 +416,7 @@ def trace_summary(path: str, n: int) -> str:
 def main(argv=None):
     if argv is None and len(sys.argv) > 2 and sys.argv[1] == "--trace-summary":
         n = int(sys.argv[3]) if len(sys.argv) > 3 else 256
-        print(trace_summary(sys.argv[2], n))
+        print(trace_summary(sys.argv[2], n, sys.argv[4] if len(sys.argv) > 4 else "ot"))
         return 0
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)

@@ -158,9 +158,8 @@ static SorArgs sor_args(cumc3d_ctx *x) {
     a.NX = x->NX; a.NY = x->NY;
     a.pin = x->phi;
     a.pout = x->phi_alt;
-    a.rho = (const double *)x->P[0];
+    a.rec = (const SorRec *)x->sor_recs;
     a.ajp1 = x->ajp1; a.ajm1 = x->ajm1; a.bkp1 = x->bkp1; a.bkm1 = x->bkm1;
-    a.clp1 = x->clp1; a.clm1 = x->clm1; a.epsc = x->epsc;
     a.part_nan = x->part_nan;
     a.part_max = x->part_max;
     a.nblk = x->sor_nblk;
@@ -190,6 +189,8 @@ static int sor_solve(cumc3d_ctx *x, bool zero_guess) {
     if (zero_guess) hipLaunchKernelGGL(k_fill0, dim3(nblk(x->G)), dim3(256), 0, st, x->phi, x->G);
     hipLaunchKernelGGL(k_sor_begin, dim3(1), dim3(1), 0, st, x->S);
     SorArgs a = sor_args(x);
+    hipLaunchKernelGGL(k_sor_rho, dim3(nblk((long)x->nx * x->ny), x->nz), dim3(256), 0, st, a,
+                       (const double *)x->P[0], (SorRec *)x->sor_recs);
     const long hx = (x->nx + 1) / 2;
     // red: one more block column, whose block (0, 0) decides the previous pair
     const dim3 gr(nblk(hx * x->ny) + 1, x->nz), gb(nblk((long)x->nx * x->ny, SOR_SPAN), x->nz);
@@ -444,18 +445,23 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
     }
     // get_poisson / poisson_coef (model/collapsar/src/poisson.f90:6-64) from
     // the cell centres, in its operation order; libm tan and sin
+    // ajp1, ajm1, bkp1, bkm1 as arrays; clp1, clm1, epsc into the per-colour
+    // cell records (SorRec, gravity.hpp), whose 4th slot rho_in each solve fills
     std::vector<double> pc;
-    long pco[7] = {0, 0, 0, 0, 0, 0, 0};
+    long pco[5] = {0, 0, 0, 0, 0};
     if (d.model == CUMC3D_MODEL_COLLAPSAR) {
         const int nx = d.nx, ny = d.ny, nz = d.nz;
-        const long N = (long)nx * ny * nz, nb = (long)(nx + 1) * (ny + 1);
-        const long len[7] = {nx + 1, nx + 1, nb, nb, N, N, N};
-        for (int i = 0; i < 7; ++i) {
+        const long N = (long)nx * ny * nz, nb = (long)(nx + 1) * (ny + 1), hx = (nx + 1) / 2;
+        const long len[4] = {nx + 1, nx + 1, nb, nb};
+        for (int i = 0; i < 4; ++i) {
             pco[i] = (long)pc.size();
             pc.resize(pc.size() + len[i], 0.0);
         }
+        pc.resize((pc.size() + 3) / 4 * 4, 0.0);   // records 32-byte aligned
+        pco[4] = (long)pc.size();
+        pc.resize(pc.size() + 4 * 2 * hx * ny * nz, 0.0);
+        std::vector<double> clp1(N), clm1(N), epsc(N);
         double *ajp1 = &pc[pco[0]], *ajm1 = &pc[pco[1]], *bkp1 = &pc[pco[2]], *bkm1 = &pc[pco[3]];
-        double *clp1 = &pc[pco[4]], *clm1 = &pc[pco[5]], *epsc = &pc[pco[6]];
         const std::vector<double> &X = centres[0], &Y = centres[1], &Z = centres[2];
         for (int i = 1; i <= nx; ++i)
             for (int j = 1; j <= ny; ++j)
@@ -474,6 +480,17 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
                     bkm1[(long)j * (nx + 1) + i] = (2.0 * t + yc - yp1) / (xc * xc) / t / (yp1 - ym1) / (yc - ym1);
                     clp1[q] = 2.0 / (xc * xc) / (sn * sn) / (zp1 - zm1) / (zp1 - zc);
                     clm1[q] = 2.0 / (xc * xc) / (sn * sn) / (zp1 - zm1) / (zc - zm1);
+                }
+        double *rec = &pc[pco[4]];
+        for (int l = 1; l <= nz; ++l)
+            for (int k = 1; k <= ny; ++k)
+                for (int j = 1; j <= nx; ++j) {
+                    const int colour = (j + k + l) & 1, j0 = ((1 + k + l) & 1) == colour ? 1 : 2;
+                    const long r = (long)colour * hx * ny * nz + (j - j0) / 2 + hx * ((long)(k - 1) + (long)ny * (l - 1));
+                    const long q = (long)(j - 1) + (long)nx * ((long)(k - 1) + (long)ny * (l - 1));
+                    rec[4 * r] = clp1[q];
+                    rec[4 * r + 1] = clm1[q];
+                    rec[4 * r + 2] = epsc[q];
                 }
     }
     int rc = 0;
@@ -553,19 +570,20 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         }
         // phi and its second buffer (G each), the coefficients, the error blocks' partials
         x->sor_nblk = (int)nblk((long)d.nx * d.ny, SOR_SPAN) * d.nz;
-        const size_t bytes = (2 * (size_t)x->G + pc.size() + 2 * (size_t)x->sor_nblk) * sizeof(double);
+        const size_t Gp = ((size_t)x->G + 3) / 4 * 4;   // keeps the records 32-byte aligned
+        const size_t bytes = (2 * Gp + pc.size() + 2 * (size_t)x->sor_nblk) * sizeof(double);
         if (hipMalloc(&x->grav, bytes) != hipSuccess) { cleanup(); return fail(CUMC3D_ENOMEM, "gravity arrays"); }
         if (hipMemset(x->grav, 0, bytes) != hipSuccess) { cleanup(); return fail(CUMC3D_EHIP, "memset"); }
         double *g = (double *)x->grav;
         x->phi = g;
-        x->phi_alt = g + x->G;
-        double *c0 = x->phi_alt + x->G;
+        x->phi_alt = g + Gp;
+        double *c0 = x->phi_alt + Gp;
         if (hipMemcpy(c0, pc.data(), pc.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
             cleanup();
             return fail(CUMC3D_EHIP, "poisson coefficient upload");
         }
         x->ajp1 = c0 + pco[0]; x->ajm1 = c0 + pco[1]; x->bkp1 = c0 + pco[2]; x->bkm1 = c0 + pco[3];
-        x->clp1 = c0 + pco[4]; x->clm1 = c0 + pco[5]; x->epsc = c0 + pco[6];
+        x->sor_recs = c0 + pco[4];
         x->part_max = c0 + pc.size();
         x->part_nan = (long long *)(x->part_max + x->sor_nblk);
     }

@@ -58,7 +58,7 @@ struct cumc3d_ctx {
     void *grav = nullptr;
     double *phi = nullptr, *phi_alt = nullptr;   // the potential and the SOR's second buffer
     double *ajp1 = nullptr, *ajm1 = nullptr, *bkp1 = nullptr, *bkm1 = nullptr;
-    double *clp1 = nullptr, *clm1 = nullptr, *epsc = nullptr, *part_max = nullptr;
+    double *sor_recs = nullptr, *part_max = nullptr;   // per-colour cell records (clp1, clm1, epsc, rho_in)
     long long *part_nan = nullptr;
     int sor_nblk = 0;
     double xF0 = 0.0;             // xF(0), the inner radial face (horizon check)

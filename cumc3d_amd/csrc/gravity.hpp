@@ -41,15 +41,24 @@ namespace cumc3d {
 
 constexpr int SOR_SPAN = 1024;   // interior cells per black block (4 per thread)
 
+// Per-cell SOR data, split by colour so that each sweep reads its cells
+// densely: colour c (0 red, 1 black) holds, for row (k, l), the cells
+// j = j0 + 2i with j0 = 1 when (1 + k + l) % 2 == c, else 2, at record
+// c*hx*ny*nz + i + hx*((k-1) + ny*(l-1)), hx = (nx+1)/2 (a padded slot
+// when j > nx).  The coefficients are poisson_coef's; rho_in, the density
+// above the atmosphere (custom.f90:544-546), is refreshed per solve.
+struct alignas(32) SorRec {
+    double clp1, clm1, epsc, rho_in;
+};
+
 struct SorArgs {
     int nx, ny, nz;
     long NX, NY;
     const double *pin;                  // phi at the start of the sweep pair, ghosted box (cidx layout)
     double *pout;                       // phi after it (interior cells)
-    const double *rho;                  // prim(irho), ghosted box
+    const SorRec *rec;                  // per-cell data by colour (see SorRec)
     const double *ajp1, *ajm1;          // [j], j = 1..nx
     const double *bkp1, *bkm1;          // [k(nx+1) + j]
-    const double *clp1, *clm1, *epsc;   // [q], q = (j-1) + nx((k-1) + ny(l-1))
     long long *part_nan;                // per black block: q of its last NaN term, -1 if none
     double *part_max;                   // per black block: max of the terms after it, -1 if none
     int nblk;                           // black blocks: ceil(nx ny / SOR_SPAN) per plane, times nz
@@ -171,19 +180,34 @@ __device__ __forceinline__ void sor_neigh(const SorArgs &a, const double *src, i
     nb[5] = l > 1 ? src[c - sz] : (z0 ? 0.0 : pin[sor_c(a, j, k, a.nz)]);
 }
 
-// one cell's relaxation (custom.f90:544-551); returns the new value
-__device__ __forceinline__ double sor_cell(const SorArgs &a, int j, int k, int l, long c, const double nb[6],
+// one cell's relaxation (custom.f90:544-551) from its record; returns the new value
+__device__ __forceinline__ double sor_cell(const SorArgs &a, int j, int k, const SorRec &r, const double nb[6],
                                            double old) {
-    const long q = (long)(j - 1) + (long)a.nx * ((long)(k - 1) + (long)a.ny * (l - 1));
-    const double rho = a.rho[c];
-    const double diff = rho - a.pa;
-    const double factor = diff > 0.0 ? 1.0 : 0.0;
-    const double rho_in = factor * rho;
     const int jk = k * (a.nx + 1) + j;
     const double s = a.ajp1[j] * nb[0] + a.ajm1[j] * nb[1] + a.bkp1[jk] * nb[2] + a.bkm1[jk] * nb[3] +
-                     a.clp1[q] * nb[4] + a.clm1[q] * nb[5];
-    const double rhs = (a.pi4 * rho_in - s) / a.epsc[q];
+                     r.clp1 * nb[4] + r.clm1 * nb[5];
+    const double rhs = (a.pi4 * r.rho_in - s) / r.epsc;
     return (1.0 - a.omega) * old + a.omega * rhs;
+}
+
+__device__ __forceinline__ long sor_rec(const SorArgs &a, int colour, int i, int k, int l) {
+    const long hx = (a.nx + 1) >> 1;
+    return (long)colour * hx * a.ny * a.nz + i + hx * ((long)(k - 1) + (long)a.ny * (l - 1));
+}
+
+// Start of a solve: rho_in of every cell into its record
+// (custom.f90:544-546: MERGE(1, 0, rho - prim_a(irho) > 0) * rho).
+__global__ __launch_bounds__(256) void k_sor_rho(SorArgs a, const double *rho, SorRec *rec) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)(a.nx * a.ny)) return;
+    const int l = 1 + blockIdx.y;
+    const int k = 1 + (int)(t / (unsigned)a.nx), j = 1 + (int)(t - (unsigned)(k - 1) * a.nx);
+    const double r = rho[sor_c(a, j, k, l)];
+    const double diff = r - a.pa;
+    const double factor = diff > 0.0 ? 1.0 : 0.0;
+    const int colour = (j + k + l) & 1;
+    const int j0 = ((1 + k + l) & 1) == colour ? 1 : 2;
+    rec[sor_rec(a, colour, (j - j0) >> 1, k, l)].rho_in = factor * r;
 }
 
 __device__ __forceinline__ long long block_max_ll(long long v) {
@@ -277,9 +301,10 @@ __global__ __launch_bounds__(256) void k_sor_red(SorArgs a) {
     const int j = (((1 + k + l) & 1) == 0 ? 1 : 2) + 2 * i;
     if (j > a.nx) return;
     const long c = sor_c(a, j, k, l);
+    const SorRec r = a.rec[sor_rec(a, 0, i, k, l)];
     double nb[6];
     sor_neigh(a, a.pin, j, k, l, c, nb);
-    a.pout[c] = sor_cell(a, j, k, l, c, nb, a.pin[c]);
+    a.pout[c] = sor_cell(a, j, k, r, nb, a.pin[c]);
 }
 
 // Every interior cell, x fastest: black cells relax from the new red values;
@@ -307,9 +332,11 @@ __global__ __launch_bounds__(256) void k_sor_black(SorArgs a) {
         const double old = a.pin[c];
         double nw;
         if ((j + k + l) & 1) {
+            const int j0 = ((1 + k + l) & 1) == 1 ? 1 : 2;
+            const SorRec r = a.rec[sor_rec(a, 1, (j - j0) >> 1, k, l)];
             double nb[6];
             sor_neigh(a, a.pout, j, k, l, c, nb);
-            nw = sor_cell(a, j, k, l, c, nb, old);
+            nw = sor_cell(a, j, k, r, nb, old);
             a.pout[c] = nw;
         } else {
             nw = a.pout[c];

@@ -119,6 +119,9 @@ BENCH_CASES = {c.name: c for c in [
     Case("bench_blast128", Config(nx=128, ny=128, nz=128, n_dim=3, cfl=0.4, ggas=5.0 / 3.0,
                                   total_time=0.02, riemann="hlld", recon="weno",
                                   boundary_flag=(0,) * 6), "blast", 3),
+    # the collapsar workload of bench.py --workload collapsar (156 x 44 x 128,
+    # the reference's shipped progenitor grid size)
+    Case("bench_collapsar156", collapsar(156, 44, 128, recon="ppm", riemann="hlld"), "collapsar", 22),
 ]}
 
 ALL_CASES = {**CASES, **BENCH_CASES}
