@@ -872,9 +872,18 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
         rho = a.P[0][c]; vx = a.P[1][c]; vy = a.P[2][c]; vz = a.P[3][c];
         bx = a.Bc[0][c]; by = a.Bc[1][c]; bz = a.Bc[2][c];
     }
+    // FP32 split: U holds the conserved state at step start and Uo[0..4] the
+    // stage results in between -- stage 1 reads U and writes Uo, stage 2
+    // reads both and writes Uo, stage 3 reads both and writes U -- so u_old
+    // is never copied.  FP64: the state is carried as primitives, and stage 1
+    // saves u_old in Uo.
     double u[5], uo[5];
     if constexpr (split) {
-        for (int i = 0; i < 5; ++i) u[i] = a.U[i][c];
+        if (STAGE == 1) {
+            for (int i = 0; i < 5; ++i) uo[i] = u[i] = a.U[i][c];
+        } else {
+            for (int i = 0; i < 5; ++i) { uo[i] = a.U[i][c]; u[i] = a.Uo[i][c]; }
+        }
     } else {
         double eps = a.eps[c];
         double vsq = vx * vx + vy * vy + vz * vz, bsq = bx * bx + by * by + bz * bz;
@@ -883,11 +892,11 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
         u[2] = vy * rho;
         u[3] = vz * rho;
         u[4] = rho * (eps + 0.5 * vsq) + 0.5 * bsq;
-    }
-    if (STAGE == 1) {
-        for (int i = 0; i < 5; ++i) { uo[i] = u[i]; a.Uo[i][c] = u[i]; }
-    } else {
-        for (int i = 0; i < 5; ++i) uo[i] = a.Uo[i][c];
+        if (STAGE == 1) {
+            for (int i = 0; i < 5; ++i) { uo[i] = u[i]; a.Uo[i][c] = u[i]; }
+        } else {
+            for (int i = 0; i < 5; ++i) uo[i] = a.Uo[i][c];
+        }
     }
     // sources: GET_SOURCE (spatial.f90:195-226) + CUSTOM_SOURCE
     double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1046,14 +1055,16 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
     // from the values just stored, for the next stage's edge EMFs
     cell_emf(a, c, (W)v[0], (W)v[1], (W)v[2], (W)bcf[0], (W)bcf[1], (W)bcf[2]);
     if constexpr (split) {
-        // FROMRVETOU of the double-precision primitives: the next stage's U
+        // FROMRVETOU of the double-precision primitives: the next stage's u
+        // (Uo after stages 1 and 2, U after stage 3)
         double vsq2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
         double bsq2 = bcf[0] * bcf[0] + bcf[1] * bcf[1] + bcf[2] * bcf[2];
-        a.U[0][c] = rn;
-        a.U[1][c] = v[0] * rn;
-        a.U[2][c] = v[1] * rn;
-        a.U[3][c] = v[2] * rn;
-        a.U[4][c] = rn * (en + 0.5 * vsq2) + 0.5 * bsq2;
+        double *const *dst = STAGE == 3 ? a.U : a.Uo;
+        dst[0][c] = rn;
+        dst[1][c] = v[0] * rn;
+        dst[2][c] = v[1] * rn;
+        dst[3][c] = v[2] * rn;
+        dst[4][c] = rn * (en + 0.5 * vsq2) + 0.5 * bsq2;
     }
     return STAGE == 3 ? cfl_dt<G>(a, j, k, l, cs, rn, v, bcf) : 1.0e5;
 }
