@@ -58,6 +58,9 @@ __global__ void k_set_dt(DevScalars *S, double dt) {
     S->active = !S->grav_stop;
 }
 
+// a new initial model starts a run afresh: no model STOP pending
+__global__ void k_clear_stop(DevScalars *S) { S->grav_stop = 0; }
+
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
@@ -156,8 +159,10 @@ static SorArgs sor_args(cumc3d_ctx *x) {
     memset(&a, 0, sizeof(a));
     a.nx = x->nx; a.ny = x->ny; a.nz = x->nz;
     a.NX = x->NX; a.NY = x->NY;
-    a.pin = x->phi;
-    a.pout = x->phi_alt;
+    a.rin = x->sor_phi[0][0];
+    a.bin = x->sor_phi[0][1];
+    a.rout = x->sor_phi[1][0];
+    a.bout = x->sor_phi[1][1];
     a.rec = (const SorRec *)x->sor_recs;
     a.ajp1 = x->ajp1; a.ajm1 = x->ajm1; a.bkp1 = x->bkp1; a.bkm1 = x->bkm1;
     a.part_nan = x->part_nan;
@@ -173,10 +178,10 @@ static SorArgs sor_args(cumc3d_ctx *x) {
     return a;
 }
 
-// The RBSOR loop (custom.f90:510-647): sweep pairs queued in growing batches,
-// the end flag read between batches; the potential alternates between its
-// two buffers, and the one the last pair wrote becomes x->phi with its
-// boundary values set.  A solve that converged on sweep relax_max sets
+// The RBSOR loop (custom.f90:510-647): phi moved into the colour-split
+// arrays, sweep pairs queued in growing batches with the end flag read
+// between batches, and the arrays the last pair wrote moved back into x->phi
+// with its boundary values set.  A solve that converged on sweep relax_max sets
 // grav_error (the reference STOPs there); one that ran out continues.
 static int sor_solve(cumc3d_ctx *x, bool zero_guess) {
     hipStream_t st = x->stream;
@@ -194,13 +199,16 @@ static int sor_solve(cumc3d_ctx *x, bool zero_guess) {
     const long hx = (x->nx + 1) / 2;
     // red: one more block column, whose block (0, 0) decides the previous pair
     const dim3 gr(nblk(hx * x->ny) + 1, x->nz), gb(nblk((long)x->nx * x->ny, SOR_SPAN), x->nz);
-    double *buf[2] = {x->phi, x->phi_alt};
+    const dim3 gc(nblk((long)x->nx * x->ny), x->nz);
+    hipLaunchKernelGGL(k_sor_copy, gc, dim3(256), 0, st, a, x->phi, x->sor_phi[0][0], x->sor_phi[0][1], 1);
     long queued = 0;
     int batch = 4, flags[3] = {0, 0, 0};
     for (;;) {
         for (int i = 0; i < batch; ++i) {
-            a.pin = buf[queued & 1];
-            a.pout = buf[(queued + 1) & 1];
+            a.rin = x->sor_phi[queued & 1][0];
+            a.bin = x->sor_phi[queued & 1][1];
+            a.rout = x->sor_phi[(queued + 1) & 1][0];
+            a.bout = x->sor_phi[(queued + 1) & 1][1];
             a.zero_ghosts = zero_guess && queued == 0;
             a.combine = i > 0;
             hipLaunchKernelGGL(k_sor_red, gr, dim3(256), 0, st, a);
@@ -216,10 +224,12 @@ static int sor_solve(cumc3d_ctx *x, bool zero_guess) {
             return fail(CUMC3D_EHIP, "Poisson solve: sweep count not reported by the device");
         if (batch < 256) batch *= 2;
     }
-    if (flags[1] & 1) std::swap(x->phi, x->phi_alt);   // the buffer the last pair wrote
-    if (flags[1] > 0)
+    if (flags[1] > 0) {   // the arrays the last pair wrote
+        const int f = flags[1] & 1;
+        hipLaunchKernelGGL(k_sor_copy, gc, dim3(256), 0, st, a, x->phi, x->sor_phi[f][0], x->sor_phi[f][1], 0);
         hipLaunchKernelGGL(k_sor_bc, dim3(nblk((long)x->ny * x->nz + (long)x->nx * x->nz + (long)x->nx * x->ny)),
                            dim3(256), 0, st, x->phi, x->nx, x->ny, x->nz, x->NX, x->NY);
+    }
     if (flags[2] == x->grav_p.relax_max) x->grav_error = 1;
     if (e0) {
         float ms = 0;
@@ -571,13 +581,15 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         // phi and its second buffer (G each), the coefficients, the error blocks' partials
         x->sor_nblk = (int)nblk((long)d.nx * d.ny, SOR_SPAN) * d.nz;
         const size_t Gp = ((size_t)x->G + 3) / 4 * 4;   // keeps the records 32-byte aligned
-        const size_t bytes = (2 * Gp + pc.size() + 2 * (size_t)x->sor_nblk) * sizeof(double);
+        const size_t Hp = ((size_t)((d.nx + 1) / 2) * d.ny * d.nz + 3) / 4 * 4;   // one colour of phi
+        const size_t bytes = (Gp + 4 * Hp + pc.size() + 2 * (size_t)x->sor_nblk) * sizeof(double);
         if (hipMalloc(&x->grav, bytes) != hipSuccess) { cleanup(); return fail(CUMC3D_ENOMEM, "gravity arrays"); }
         if (hipMemset(x->grav, 0, bytes) != hipSuccess) { cleanup(); return fail(CUMC3D_EHIP, "memset"); }
         double *g = (double *)x->grav;
         x->phi = g;
-        x->phi_alt = g + Gp;
-        double *c0 = x->phi_alt + Gp;
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) x->sor_phi[b][c] = g + Gp + (2 * b + c) * Hp;
+        double *c0 = g + Gp + 4 * Hp;
         if (hipMemcpy(c0, pc.data(), pc.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
             cleanup();
             return fail(CUMC3D_EHIP, "poisson coefficient upload");
@@ -619,6 +631,7 @@ int cumc3d_upload(cumc3d_ctx *x, const double *prim, const double *eps, const do
 int cumc3d_initial_update(cumc3d_ctx *x) {
     if (!x) return fail(CUMC3D_EINVAL, "null context");
     if (!x->uploaded) return fail(CUMC3D_ESTATE, "upload an initial model first");
+    hipLaunchKernelGGL(k_clear_stop, dim3(1), dim3(1), 0, x->stream, x->S);
     hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, x->stream, x->S, 0.0);   // also re-activates the stage kernels
     const unsigned gi = nblk((long)x->nx * x->ny * x->nz), gg = nblk(x->G);
     if (x->fp32) {

@@ -52,11 +52,12 @@ struct cumc3d_ctx {
     double tsum[10] = {0};        // [8] Poisson solve ms, [9] SOR sweeps per solve
     long tcnt[10] = {0};
     // collapsar self-gravity (model/collapsar/src/def.f90:16-36): one
-    // allocation holding the potential's two buffers, the poisson_coef
-    // coefficients and the error blocks' partials
+    // allocation holding the potential, its colour-split copies for the SOR,
+    // the poisson_coef coefficients and the error blocks' partials
     cumc3d_gravity_params grav_p{};
     void *grav = nullptr;
-    double *phi = nullptr, *phi_alt = nullptr;   // the potential and the SOR's second buffer
+    double *phi = nullptr;                  // the potential (ghosted box)
+    double *sor_phi[2][2] = {};             // during a solve: [buffer][red, black] (colour-split)
     double *ajp1 = nullptr, *ajm1 = nullptr, *bkp1 = nullptr, *bkm1 = nullptr;
     double *sor_recs = nullptr, *part_max = nullptr;   // per-colour cell records (clp1, clm1, epsc, rho_in)
     long long *part_nan = nullptr;

@@ -13,15 +13,17 @@
 //
 // FP64 only.  The sweeps are bit-identical to the serial loop: a red cell
 // ((-1)**(j+k+l) > 0) reads only black neighbours and vice versa, so the order
-// within a sweep does not matter.  The potential ping-pongs between two
-// buffers: a sweep pair reads `pin` (phi at the start of the pair, with its
-// boundary values) and writes every interior cell of `pout`; red cells read
-// pin, black cells read the new red values from pout.  The boundary values a
-// pair reads are those the reference's previous pair set (custom.f90:601-631),
-// formed on the fly from pin: phi(0) = phi(1), phi(nx+1) = -1, zero theta
-// gradient, periodic in phi -- and 0 on the first pair after the zero guess
-// of UPDATE(0) (:497-505), which had set no boundary values yet.  k_sor_bc
-// writes them into the final buffer for the gravity source.
+// within a sweep does not matter.  During a solve the potential lives split
+// by colour (every cell of one colour in a dense array, like the per-cell
+// records) and ping-pongs: a sweep pair reads the red and black values at its
+// start and writes both colours anew; red cells read the start values, black
+// cells the new red ones.  The boundary values a pair reads are those the
+// reference's previous pair set (custom.f90:601-631), formed on the fly from
+// the start values: phi(0) = phi(1), phi(nx+1) = -1, zero theta gradient,
+// periodic in phi -- and 0 on the first pair after the zero guess of
+// UPDATE(0) (:497-505), which had set no boundary values yet.  k_sor_copy
+// moves phi into the split arrays at the start and back at the end, and
+// k_sor_bc writes the boundary values for the gravity source.
 // The error is the OpenMP MAX reduction of :588-598 as the reference's
 // one-thread CPU build evaluates it: a private fold of
 // MAX(abserror, |dphi/phi_old|) in l, k, j order with flang's MAX
@@ -54,8 +56,11 @@ struct alignas(32) SorRec {
 struct SorArgs {
     int nx, ny, nz;
     long NX, NY;
-    const double *pin;                  // phi at the start of the sweep pair, ghosted box (cidx layout)
-    double *pout;                       // phi after it (interior cells)
+    // phi during a solve, split by colour like the records (index
+    // i + hx((k-1) + ny(l-1)) within a colour): the red and black values at
+    // the start of the sweep pair and after it
+    const double *rin, *bin;
+    double *rout, *bout;
     const SorRec *rec;                  // per-cell data by colour (see SorRec)
     const double *ajp1, *ajm1;          // [j], j = 1..nx
     const double *bkp1, *bkm1;          // [k(nx+1) + j]
@@ -164,20 +169,30 @@ __device__ __forceinline__ long sor_c(const SorArgs &a, int j, int k, int l) {
     return (long)(j + 2) + a.NX * ((long)(k + 2) + a.NY * (long)(l + 2));
 }
 
+// index of cell (j, k, l) within its colour's arrays
+__device__ __forceinline__ long sor_i(const SorArgs &a, int j, int k, int l) {
+    const long hx = (a.nx + 1) >> 1;
+    return (long)((j - 1) >> 1) + hx * ((long)(k - 1) + (long)a.ny * (l - 1));
+}
+
 // The neighbour values one cell's update reads (custom.f90:548-550): cells
-// of the other colour from `src` (pin for a red cell, pout for a black one),
-// the boundary values from pin.
-__device__ __forceinline__ void sor_neigh(const SorArgs &a, const double *src, int j, int k, int l, long c,
+// of the other colour from `other` (the black start values for a red cell,
+// the new red values for a black one), the boundary values from the start
+// values of the pair: phi(0) = phi(1) and the theta ends copy the cell
+// itself (`own`, its start value), phi(nx+1) = -1, periodic in phi.
+__device__ __forceinline__ void sor_neigh(const SorArgs &a, const double *other, int j, int k, int l, double own,
                                           double nb[6]) {
-    const long sy = a.NX, sz = a.NX * a.NY;
-    const double *pin = a.pin;
     const bool z0 = a.zero_ghosts;
-    nb[0] = j < a.nx ? src[c + 1] : (z0 ? 0.0 : -1.0);
-    nb[1] = j > 1 ? src[c - 1] : (z0 ? 0.0 : pin[c]);
-    nb[2] = k < a.ny ? src[c + sy] : (z0 ? 0.0 : pin[c]);
-    nb[3] = k > 1 ? src[c - sy] : (z0 ? 0.0 : pin[c]);
-    nb[4] = l < a.nz ? src[c + sz] : (z0 ? 0.0 : pin[sor_c(a, j, k, 1)]);
-    nb[5] = l > 1 ? src[c - sz] : (z0 ? 0.0 : pin[sor_c(a, j, k, a.nz)]);
+    nb[0] = j < a.nx ? other[sor_i(a, j + 1, k, l)] : (z0 ? 0.0 : -1.0);
+    nb[1] = j > 1 ? other[sor_i(a, j - 1, k, l)] : (z0 ? 0.0 : own);
+    nb[2] = k < a.ny ? other[sor_i(a, j, k + 1, l)] : (z0 ? 0.0 : own);
+    nb[3] = k > 1 ? other[sor_i(a, j, k - 1, l)] : (z0 ? 0.0 : own);
+    if (l < a.nz) nb[4] = other[sor_i(a, j, k, l + 1)];
+    else if (z0) nb[4] = 0.0;
+    else nb[4] = (((j + k + 1) & 1) ? a.bin : a.rin)[sor_i(a, j, k, 1)];
+    if (l > 1) nb[5] = other[sor_i(a, j, k, l - 1)];
+    else if (z0) nb[5] = 0.0;
+    else nb[5] = (((j + k + a.nz) & 1) ? a.bin : a.rin)[sor_i(a, j, k, a.nz)];
 }
 
 // one cell's relaxation (custom.f90:544-551) from its record; returns the new value
@@ -300,11 +315,12 @@ __global__ __launch_bounds__(256) void k_sor_red(SorArgs a) {
     const int k = 1 + (int)(t / (unsigned)hx), i = (int)(t - (unsigned)(k - 1) * hx), l = 1 + blockIdx.y;
     const int j = (((1 + k + l) & 1) == 0 ? 1 : 2) + 2 * i;
     if (j > a.nx) return;
-    const long c = sor_c(a, j, k, l);
+    const long ic = sor_i(a, j, k, l);
     const SorRec r = a.rec[sor_rec(a, 0, i, k, l)];
+    const double old = a.rin[ic];
     double nb[6];
-    sor_neigh(a, a.pin, j, k, l, c, nb);
-    a.pout[c] = sor_cell(a, j, k, r, nb, a.pin[c]);
+    sor_neigh(a, a.bin, j, k, l, old, nb);
+    a.rout[ic] = sor_cell(a, j, k, r, nb, old);
 }
 
 // Every interior cell, x fastest: black cells relax from the new red values;
@@ -328,18 +344,18 @@ __global__ __launch_bounds__(256) void k_sor_black(SorArgs a) {
         if (t >= (unsigned)nxy) continue;
         qq[u] = (long long)(l - 1) * nxy + t;
         const int k = 1 + (int)(t / (unsigned)a.nx), j = 1 + (int)(t - (unsigned)(k - 1) * a.nx);
-        const long c = sor_c(a, j, k, l);
-        const double old = a.pin[c];
-        double nw;
+        const long ic = sor_i(a, j, k, l);
+        double old, nw;
         if ((j + k + l) & 1) {
-            const int j0 = ((1 + k + l) & 1) == 1 ? 1 : 2;
-            const SorRec r = a.rec[sor_rec(a, 1, (j - j0) >> 1, k, l)];
+            const SorRec r = a.rec[sor_rec(a, 1, (j - 1) >> 1, k, l)];
+            old = a.bin[ic];
             double nb[6];
-            sor_neigh(a, a.pout, j, k, l, c, nb);
+            sor_neigh(a, a.rout, j, k, l, old, nb);
             nw = sor_cell(a, j, k, r, nb, old);
-            a.pout[c] = nw;
+            a.bout[ic] = nw;
         } else {
-            nw = a.pout[c];
+            old = a.rin[ic];
+            nw = a.rout[ic];
         }
         e[u] = fabs((nw - old) / old);
         if (e[u] != e[u]) last = qq[u];   // q grows with u: the thread's last NaN
@@ -360,6 +376,19 @@ __global__ __launch_bounds__(256) void k_sor_black(SorArgs a) {
 // The decision of a solve's last pair when no red kernel follows it.
 __global__ __launch_bounds__(256) void k_sor_decide(SorArgs a) {
     if (!a.S->sor_state && a.S->active) sor_decide(a);
+}
+
+// phi over the interior between the ghosted box and the colour-split
+// arrays: to_split at the start of a solve, back at its end.
+__global__ __launch_bounds__(256) void k_sor_copy(SorArgs a, double *phi, double *red, double *black, int to_split) {
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)(a.nx * a.ny)) return;
+    const int l = 1 + blockIdx.y;
+    const int k = 1 + (int)(t / (unsigned)a.nx), j = 1 + (int)(t - (unsigned)(k - 1) * a.nx);
+    double *s = ((j + k + l) & 1) ? black : red;
+    const long c = sor_c(a, j, k, l), ic = sor_i(a, j, k, l);
+    if (to_split) s[ic] = phi[c];
+    else phi[c] = s[ic];
 }
 
 // The boundary values of the final potential (custom.f90:601-631), written

@@ -230,6 +230,8 @@ def test_gpu_collapsar_stops_where_the_reference_stops():
     s.initial_model(big)
     with pytest.raises(CumcError, match="event horizon"):
         s.initial_update()
+    with pytest.raises(CumcError):
+        s.rungekutta(1.0)        # no step after the stop
     s = Solver(cfg.with_(relax_max=1))
     s.initial_model(st)
     with pytest.raises(CumcError, match="poisson"):
