@@ -163,7 +163,8 @@ static SorArgs sor_args(cumc3d_ctx *x) {
     a.bin = x->sor_phi[0][1];
     a.rout = x->sor_phi[1][0];
     a.bout = x->sor_phi[1][1];
-    a.rec = (const SorRec *)x->sor_recs;
+    a.rho_in = x->sor_rho;
+    a.a2 = x->sor_a2; a.e2 = x->sor_e2; a.zpm = x->sor_z[0]; a.zpc = x->sor_z[1]; a.zcm = x->sor_z[2];
     a.ajp1 = x->ajp1; a.ajm1 = x->ajm1; a.bkp1 = x->bkp1; a.bkm1 = x->bkm1;
     a.part_nan = x->part_nan;
     a.part_max = x->part_max;
@@ -195,7 +196,7 @@ static int sor_solve(cumc3d_ctx *x, bool zero_guess) {
     hipLaunchKernelGGL(k_sor_begin, dim3(1), dim3(1), 0, st, x->S);
     SorArgs a = sor_args(x);
     hipLaunchKernelGGL(k_sor_rho, dim3(nblk((long)x->nx * x->ny), x->nz), dim3(256), 0, st, a,
-                       (const double *)x->P[0], (SorRec *)x->sor_recs);
+                       (const double *)x->P[0], x->sor_rho);
     const long hx = (x->nx + 1) / 2;
     // red: one more block column, whose block (0, 0) decides the previous pair
     const dim3 gr(nblk(hx * x->ny) + 1, x->nz), gb(nblk((long)x->nx * x->ny, SOR_SPAN), x->nz);
@@ -455,53 +456,45 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
     }
     // get_poisson / poisson_coef (model/collapsar/src/poisson.f90:6-64) from
     // the cell centres, in its operation order; libm tan and sin
-    // ajp1, ajm1, bkp1, bkm1 as arrays; clp1, clm1, epsc into the per-colour
-    // cell records (SorRec, gravity.hpp), whose 4th slot rho_in each solve fills
+    // ajp1, ajm1, bkp1, bkm1 as arrays; clp1, clm1 and epsc as the (j, k)
+    // factor 2/xc^2/sin^2, epsc's radial + polar terms and the phi spacings,
+    // from which the kernels form them with poisson_coef's divisions
+    // (gravity.hpp); then room for rho_in, split by colour
     std::vector<double> pc;
-    long pco[5] = {0, 0, 0, 0, 0};
+    long pco[10] = {};
     if (d.model == CUMC3D_MODEL_COLLAPSAR) {
         const int nx = d.nx, ny = d.ny, nz = d.nz;
-        const long N = (long)nx * ny * nz, nb = (long)(nx + 1) * (ny + 1), hx = (nx + 1) / 2;
-        const long len[4] = {nx + 1, nx + 1, nb, nb};
-        for (int i = 0; i < 4; ++i) {
+        const long nb = (long)(nx + 1) * (ny + 1), hx = (nx + 1) / 2;
+        const long len[9] = {nx + 1, nx + 1, nb, nb, nb, nb, nz + 1, nz + 1, nz + 1};
+        for (int i = 0; i < 9; ++i) {
             pco[i] = (long)pc.size();
             pc.resize(pc.size() + len[i], 0.0);
         }
-        pc.resize((pc.size() + 3) / 4 * 4, 0.0);   // records 32-byte aligned
-        pco[4] = (long)pc.size();
-        pc.resize(pc.size() + 4 * 2 * hx * ny * nz, 0.0);
-        std::vector<double> clp1(N), clm1(N), epsc(N);
+        pco[9] = (long)pc.size();
+        pc.resize(pc.size() + 2 * hx * ny * nz, 0.0);
         double *ajp1 = &pc[pco[0]], *ajm1 = &pc[pco[1]], *bkp1 = &pc[pco[2]], *bkm1 = &pc[pco[3]];
+        double *a2 = &pc[pco[4]], *e2 = &pc[pco[5]], *zpm = &pc[pco[6]], *zpc = &pc[pco[7]], *zcm = &pc[pco[8]];
         const std::vector<double> &X = centres[0], &Y = centres[1], &Z = centres[2];
         for (int i = 1; i <= nx; ++i)
-            for (int j = 1; j <= ny; ++j)
-                for (int k = 1; k <= nz; ++k) {
-                    const double xm1 = X[i + 2], xc = X[i + 3], xp1 = X[i + 4];
-                    const double ym1 = Y[j + 2], yc = Y[j + 3], yp1 = Y[j + 4];
-                    const double zm1 = Z[k + 2], zc = Z[k + 3], zp1 = Z[k + 4];
-                    const double t = std::tan(yc), sn = std::sin(yc);
-                    const long q = (long)(i - 1) + (long)nx * ((long)(j - 1) + (long)ny * (k - 1));
-                    epsc[q] = 2.0 * (xp1 + xm1 - 3.0 * xc) / xc / (xp1 - xc) / (xc - xm1) +
-                              (yp1 + ym1 - 2.0 * yc - 2.0 * t) / (xc * xc) / t / (yp1 - yc) / (yc - ym1) -
-                              2.0 / (xc * xc) / (sn * sn) / (zp1 - zc) / (zc - zm1);
-                    ajp1[i] = 2.0 * (2.0 * xc - xm1) / (xp1 - xm1) / (xp1 - xc) / xc;
-                    ajm1[i] = 2.0 * (2.0 * xc - xp1) / (xp1 - xm1) / (xc - xm1) / xc;
-                    bkp1[(long)j * (nx + 1) + i] = (2.0 * t + yc - ym1) / (xc * xc) / t / (yp1 - yc) / (yp1 - ym1);
-                    bkm1[(long)j * (nx + 1) + i] = (2.0 * t + yc - yp1) / (xc * xc) / t / (yp1 - ym1) / (yc - ym1);
-                    clp1[q] = 2.0 / (xc * xc) / (sn * sn) / (zp1 - zm1) / (zp1 - zc);
-                    clm1[q] = 2.0 / (xc * xc) / (sn * sn) / (zp1 - zm1) / (zc - zm1);
-                }
-        double *rec = &pc[pco[4]];
-        for (int l = 1; l <= nz; ++l)
-            for (int k = 1; k <= ny; ++k)
-                for (int j = 1; j <= nx; ++j) {
-                    const int colour = (j + k + l) & 1, j0 = ((1 + k + l) & 1) == colour ? 1 : 2;
-                    const long r = (long)colour * hx * ny * nz + (j - j0) / 2 + hx * ((long)(k - 1) + (long)ny * (l - 1));
-                    const long q = (long)(j - 1) + (long)nx * ((long)(k - 1) + (long)ny * (l - 1));
-                    rec[4 * r] = clp1[q];
-                    rec[4 * r + 1] = clm1[q];
-                    rec[4 * r + 2] = epsc[q];
-                }
+            for (int j = 1; j <= ny; ++j) {
+                const double xm1 = X[i + 2], xc = X[i + 3], xp1 = X[i + 4];
+                const double ym1 = Y[j + 2], yc = Y[j + 3], yp1 = Y[j + 4];
+                const double t = std::tan(yc), sn = std::sin(yc);
+                const long jk = (long)j * (nx + 1) + i;
+                e2[jk] = 2.0 * (xp1 + xm1 - 3.0 * xc) / xc / (xp1 - xc) / (xc - xm1) +
+                         (yp1 + ym1 - 2.0 * yc - 2.0 * t) / (xc * xc) / t / (yp1 - yc) / (yc - ym1);
+                a2[jk] = 2.0 / (xc * xc) / (sn * sn);
+                ajp1[i] = 2.0 * (2.0 * xc - xm1) / (xp1 - xm1) / (xp1 - xc) / xc;
+                ajm1[i] = 2.0 * (2.0 * xc - xp1) / (xp1 - xm1) / (xc - xm1) / xc;
+                bkp1[jk] = (2.0 * t + yc - ym1) / (xc * xc) / t / (yp1 - yc) / (yp1 - ym1);
+                bkm1[jk] = (2.0 * t + yc - yp1) / (xc * xc) / t / (yp1 - ym1) / (yc - ym1);
+            }
+        for (int k = 1; k <= nz; ++k) {
+            const double zm1 = Z[k + 2], zc = Z[k + 3], zp1 = Z[k + 4];
+            zpm[k] = zp1 - zm1;
+            zpc[k] = zp1 - zc;
+            zcm[k] = zc - zm1;
+        }
     }
     int rc = 0;
     auto cleanup = [&]() { cumc3d_finalize(x); };
@@ -580,7 +573,7 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         }
         // phi and its second buffer (G each), the coefficients, the error blocks' partials
         x->sor_nblk = (int)nblk((long)d.nx * d.ny, SOR_SPAN) * d.nz;
-        const size_t Gp = ((size_t)x->G + 3) / 4 * 4;   // keeps the records 32-byte aligned
+        const size_t Gp = ((size_t)x->G + 3) / 4 * 4;
         const size_t Hp = ((size_t)((d.nx + 1) / 2) * d.ny * d.nz + 3) / 4 * 4;   // one colour of phi
         const size_t bytes = (Gp + 4 * Hp + pc.size() + 2 * (size_t)x->sor_nblk) * sizeof(double);
         if (hipMalloc(&x->grav, bytes) != hipSuccess) { cleanup(); return fail(CUMC3D_ENOMEM, "gravity arrays"); }
@@ -595,7 +588,9 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
             return fail(CUMC3D_EHIP, "poisson coefficient upload");
         }
         x->ajp1 = c0 + pco[0]; x->ajm1 = c0 + pco[1]; x->bkp1 = c0 + pco[2]; x->bkm1 = c0 + pco[3];
-        x->sor_recs = c0 + pco[4];
+        x->sor_a2 = c0 + pco[4]; x->sor_e2 = c0 + pco[5];
+        for (int i = 0; i < 3; ++i) x->sor_z[i] = c0 + pco[6 + i];
+        x->sor_rho = c0 + pco[9];
         x->part_max = c0 + pc.size();
         x->part_nan = (long long *)(x->part_max + x->sor_nblk);
     }

@@ -59,7 +59,8 @@ struct cumc3d_ctx {
     double *phi = nullptr;                  // the potential (ghosted box)
     double *sor_phi[2][2] = {};             // during a solve: [buffer][red, black] (colour-split)
     double *ajp1 = nullptr, *ajm1 = nullptr, *bkp1 = nullptr, *bkm1 = nullptr;
-    double *sor_recs = nullptr, *part_max = nullptr;   // per-colour cell records (clp1, clm1, epsc, rho_in)
+    double *sor_rho = nullptr, *part_max = nullptr;   // rho_in split by colour; error blocks' maxima
+    double *sor_a2 = nullptr, *sor_e2 = nullptr, *sor_z[3] = {};   // poisson_coef factors (gravity.hpp)
     long long *part_nan = nullptr;
     int sor_nblk = 0;
     double xF0 = 0.0;             // xF(0), the inner radial face (horizon check)

@@ -43,15 +43,18 @@ namespace cumc3d {
 
 constexpr int SOR_SPAN = 1024;   // interior cells per black block (4 per thread)
 
-// Per-cell SOR data, split by colour so that each sweep reads its cells
+// Per-cell SOR data are split by colour so that each sweep reads its cells
 // densely: colour c (0 red, 1 black) holds, for row (k, l), the cells
-// j = j0 + 2i with j0 = 1 when (1 + k + l) % 2 == c, else 2, at record
-// c*hx*ny*nz + i + hx*((k-1) + ny*(l-1)), hx = (nx+1)/2 (a padded slot
-// when j > nx).  The coefficients are poisson_coef's; rho_in, the density
-// above the atmosphere (custom.f90:544-546), is refreshed per solve.
-struct alignas(32) SorRec {
-    double clp1, clm1, epsc, rho_in;
-};
+// j = j0 + 2i with j0 = 1 when (1 + k + l) % 2 == c, else 2, at
+// c*hx*ny*nz + i + hx*((k-1) + ny*(l-1)), hx = (nx+1)/2 (a padded slot when
+// j > nx).  Per cell only rho_in, the density above the atmosphere
+// (custom.f90:544-546), refreshed per solve.  poisson_coef's azimuthal
+// coefficients (poisson.f90:52-63) are formed in the kernel from a (j, k)
+// factor and the cell's phi spacings with the same divisions in the same
+// order, so they are the values the reference stores:
+//   clp1 = ((2/xc^2/sin^2) / (zp1-zm1)) / (zp1-zc)
+//   clm1 = ((2/xc^2/sin^2) / (zp1-zm1)) / (zc-zm1)
+//   epsc = (radial + polar terms) - ((2/xc^2/sin^2) / (zp1-zc)) / (zc-zm1)
 
 struct SorArgs {
     int nx, ny, nz;
@@ -61,9 +64,11 @@ struct SorArgs {
     // the start of the sweep pair and after it
     const double *rin, *bin;
     double *rout, *bout;
-    const SorRec *rec;                  // per-cell data by colour (see SorRec)
+    const double *rho_in;               // per cell, split by colour (see above)
     const double *ajp1, *ajm1;          // [j], j = 1..nx
     const double *bkp1, *bkm1;          // [k(nx+1) + j]
+    const double *a2, *e2;              // [k(nx+1) + j]: 2/xc^2/sin^2 and epsc's radial + polar terms
+    const double *zpm, *zpc, *zcm;      // [l]: zp1-zm1, zp1-zc, zc-zm1
     long long *part_nan;                // per black block: q of its last NaN term, -1 if none
     double *part_max;                   // per black block: max of the terms after it, -1 if none
     int nblk;                           // black blocks: ceil(nx ny / SOR_SPAN) per plane, times nz
@@ -195,13 +200,17 @@ __device__ __forceinline__ void sor_neigh(const SorArgs &a, const double *other,
     else nb[5] = (((j + k + a.nz) & 1) ? a.bin : a.rin)[sor_i(a, j, k, a.nz)];
 }
 
-// one cell's relaxation (custom.f90:544-551) from its record; returns the new value
-__device__ __forceinline__ double sor_cell(const SorArgs &a, int j, int k, const SorRec &r, const double nb[6],
+// one cell's relaxation (custom.f90:544-551); returns the new value
+__device__ __forceinline__ double sor_cell(const SorArgs &a, int j, int k, int l, double rho_in, const double nb[6],
                                            double old) {
     const int jk = k * (a.nx + 1) + j;
+    const double A = a.a2[jk], zpc = a.zpc[l], zcm = a.zcm[l];
+    const double t = A / a.zpm[l];
+    const double clp1 = t / zpc, clm1 = t / zcm;
+    const double epsc = a.e2[jk] - A / zpc / zcm;
     const double s = a.ajp1[j] * nb[0] + a.ajm1[j] * nb[1] + a.bkp1[jk] * nb[2] + a.bkm1[jk] * nb[3] +
-                     r.clp1 * nb[4] + r.clm1 * nb[5];
-    const double rhs = (a.pi4 * r.rho_in - s) / r.epsc;
+                     clp1 * nb[4] + clm1 * nb[5];
+    const double rhs = (a.pi4 * rho_in - s) / epsc;
     return (1.0 - a.omega) * old + a.omega * rhs;
 }
 
@@ -210,9 +219,9 @@ __device__ __forceinline__ long sor_rec(const SorArgs &a, int colour, int i, int
     return (long)colour * hx * a.ny * a.nz + i + hx * ((long)(k - 1) + (long)a.ny * (l - 1));
 }
 
-// Start of a solve: rho_in of every cell into its record
+// Start of a solve: rho_in of every cell, split by colour
 // (custom.f90:544-546: MERGE(1, 0, rho - prim_a(irho) > 0) * rho).
-__global__ __launch_bounds__(256) void k_sor_rho(SorArgs a, const double *rho, SorRec *rec) {
+__global__ __launch_bounds__(256) void k_sor_rho(SorArgs a, const double *rho, double *rho_in) {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (unsigned)(a.nx * a.ny)) return;
     const int l = 1 + blockIdx.y;
@@ -222,7 +231,7 @@ __global__ __launch_bounds__(256) void k_sor_rho(SorArgs a, const double *rho, S
     const double factor = diff > 0.0 ? 1.0 : 0.0;
     const int colour = (j + k + l) & 1;
     const int j0 = ((1 + k + l) & 1) == colour ? 1 : 2;
-    rec[sor_rec(a, colour, (j - j0) >> 1, k, l)].rho_in = factor * r;
+    rho_in[sor_rec(a, colour, (j - j0) >> 1, k, l)] = factor * r;
 }
 
 __device__ __forceinline__ long long block_max_ll(long long v) {
@@ -316,11 +325,11 @@ __global__ __launch_bounds__(256) void k_sor_red(SorArgs a) {
     const int j = (((1 + k + l) & 1) == 0 ? 1 : 2) + 2 * i;
     if (j > a.nx) return;
     const long ic = sor_i(a, j, k, l);
-    const SorRec r = a.rec[sor_rec(a, 0, i, k, l)];
+    const double rho_in = a.rho_in[sor_rec(a, 0, i, k, l)];
     const double old = a.rin[ic];
     double nb[6];
     sor_neigh(a, a.bin, j, k, l, old, nb);
-    a.rout[ic] = sor_cell(a, j, k, r, nb, old);
+    a.rout[ic] = sor_cell(a, j, k, l, rho_in, nb, old);
 }
 
 // Every interior cell, x fastest: black cells relax from the new red values;
@@ -347,11 +356,11 @@ __global__ __launch_bounds__(256) void k_sor_black(SorArgs a) {
         const long ic = sor_i(a, j, k, l);
         double old, nw;
         if ((j + k + l) & 1) {
-            const SorRec r = a.rec[sor_rec(a, 1, (j - 1) >> 1, k, l)];
+            const double rho_in = a.rho_in[sor_rec(a, 1, (j - 1) >> 1, k, l)];
             old = a.bin[ic];
             double nb[6];
             sor_neigh(a, a.rout, j, k, l, old, nb);
-            nw = sor_cell(a, j, k, r, nb, old);
+            nw = sor_cell(a, j, k, l, rho_in, nb, old);
             a.bout[ic] = nw;
         } else {
             old = a.rin[ic];
