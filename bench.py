@@ -191,7 +191,7 @@ def bench_collapsar(args):
         "gravity": {"poisson_ms_per_solve": kt["poisson"], "sor_sweeps_per_solve": kt["sor_sweeps"],
                     "sor_ms_per_sweep": sweep_ms, "m_bh": g["m_bh"], "last_solve_sweeps": g["iterations"],
                     "poisson_ms_per_step": kt["poisson"] * solves / args.steps if solves else 0.0},
-        "roofline": {"bound": "hbm", "kernel": "sor sweep pair (k_sor_sweep<0>, k_sor_sweep<1>) + k_sor_post",
+        "roofline": {"bound": "hbm", "kernel": "sor sweep pair (k_sor_red + k_sor_black)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
                      "algorithmic_bytes": sor_bytes(cfg),

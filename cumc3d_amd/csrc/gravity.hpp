@@ -14,8 +14,8 @@
 // FP64 only.  The sweeps are bit-identical to the serial loop: a red cell
 // ((-1)**(j+k+l) > 0) reads only black neighbours and vice versa, so the order
 // within a sweep does not matter.  During a solve the potential lives split
-// by colour (every cell of one colour in a dense array, like the per-cell
-// records) and ping-pongs: a sweep pair reads the red and black values at its
+// by colour (every cell of one colour in a dense array, like rho_in) and
+// ping-pongs: a sweep pair reads the red and black values at its
 // start and writes both colours anew; red cells read the start values, black
 // cells the new red ones.  The boundary values a pair reads are those the
 // reference's previous pair set (custom.f90:601-631), formed on the fly from
@@ -59,7 +59,7 @@ constexpr int SOR_SPAN = 1024;   // interior cells per black block (4 per thread
 struct SorArgs {
     int nx, ny, nz;
     long NX, NY;
-    // phi during a solve, split by colour like the records (index
+    // phi during a solve, split by colour like rho_in (index
     // i + hx((k-1) + ny(l-1)) within a colour): the red and black values at
     // the start of the sweep pair and after it
     const double *rin, *bin;
@@ -214,6 +214,7 @@ __device__ __forceinline__ double sor_cell(const SorArgs &a, int j, int k, int l
     return (1.0 - a.omega) * old + a.omega * rhs;
 }
 
+// position of cell i of row (k, l) of a colour in the colour-split rho_in
 __device__ __forceinline__ long sor_rec(const SorArgs &a, int colour, int i, int k, int l) {
     const long hx = (a.nx + 1) >> 1;
     return (long)colour * hx * a.ny * a.nz + i + hx * ((long)(k - 1) + (long)a.ny * (l - 1));
