@@ -434,7 +434,7 @@ def main(argv=None):
     ap.add_argument("--variants", default="fp64,fp32,fp32_sel")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="copy 1 GiB device to device first (the known-size kernel that scales the HBM counters)")
-    ap.add_argument("--sel-every", type=int, default=4)
+    ap.add_argument("--sel-every", type=int, default=8)
     ap.add_argument("--thr-ppm", type=float, default=1e-3)
     ap.add_argument("--thr-weno", type=float, default=1e-2)
     args = ap.parse_args(argv)
