@@ -61,6 +61,9 @@ __global__ void k_set_dt(DevScalars *S, double dt) {
 // a new initial model starts a run afresh: no model STOP pending
 __global__ void k_clear_stop(DevScalars *S) { S->grav_stop = 0; }
 
+// SPATIAL on its own runs the stage kernels whatever the run's end time
+__global__ void k_activate(DevScalars *S) { S->active = !S->grav_stop; }
+
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
@@ -648,6 +651,7 @@ int cumc3d_initial_update(cumc3d_ctx *x) {
     if (int rc = grav_check(x)) return rc;
     x->initialized = true;
     x->dt_pending = false;
+    x->next_stage = 1;
     return 0;
 }
 
@@ -672,6 +676,7 @@ int cumc3d_finddt(cumc3d_ctx *x, double *dt) {
 int cumc3d_rk3_step(cumc3d_ctx *x, double dt) {
     if (!x) return fail(CUMC3D_EINVAL, "null context");
     if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
+    if (x->next_stage != 1) return fail(CUMC3D_ESTATE, "a step run by cumc3d_stage is not finished");
     if (int rc = grav_check(x)) return rc;
     hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, x->stream, x->S, dt);
     x->cur_step = x->nstep_total;
@@ -686,10 +691,58 @@ int cumc3d_rk3_step(cumc3d_ctx *x, double dt) {
     return 0;
 }
 
+int cumc3d_stage(cumc3d_ctx *x, int stage, double dt) {
+    if (!x) return fail(CUMC3D_EINVAL, "null context");
+    if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
+    if (stage < 1 || stage > 3) return fail(CUMC3D_EINVAL, "stage must be 1, 2 or 3");
+    if (stage != x->next_stage) return fail(CUMC3D_ESTATE, "the stages of a step run in the order 1, 2, 3");
+    if (int rc = grav_check(x)) return rc;
+    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, x->stream, x->S, dt);
+    if (stage == 1) {
+        x->cur_step = x->nstep_total;
+        x->dt_pending = false;
+    }
+    if (int rc = run_stage(x, stage)) return rc;
+    x->next_stage = stage % 3 + 1;
+    if (stage == 3) {
+        x->nstep_total += 1;
+        x->dt_pending = true;   // stage 3 reduced the CFL minimum of the new state
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return grav_check(x);
+}
+
+int cumc3d_spatial(cumc3d_ctx *x, double *l_rk) {
+    if (!x || !l_rk) return fail(CUMC3D_EINVAL, "null argument");
+    if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
+    if (int rc = grav_check(x)) return rc;
+    const size_t bytes = (size_t)8 * x->G * sizeof(double);
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, bytes));
+    // the entries SPATIAL does not form keep the caller's values
+    int rc = 0;
+    if (hipMemcpyAsync(d, l_rk, bytes, hipMemcpyHostToDevice, x->stream) != hipSuccess) rc = CUMC3D_EHIP;
+    if (!rc) {
+        hipLaunchKernelGGL(k_activate, dim3(1), dim3(1), 0, x->stream, x->S);
+        if (x->fp32) f32::spatial(x, d);
+        else spatial_t<double>(x, d);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(l_rk, d, bytes, hipMemcpyDeviceToHost, x->stream) != hipSuccess ||
+            hipStreamSynchronize(x->stream) != hipSuccess)
+            rc = CUMC3D_EHIP;
+    }
+    hipStreamSynchronize(x->stream);
+    hipFree(d);
+    if (rc) return fail(rc, "SPATIAL failed on the device");
+    return 0;
+}
+
 int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_step, double *t, double *dts,
                int *steps_done) {
     if (!x || !t) return fail(CUMC3D_EINVAL, "null argument");
     if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
+    if (x->next_stage != 1) return fail(CUMC3D_ESTATE, "a step run by cumc3d_stage is not finished");
     double *dlog = nullptr;
     if (dts && max_steps > 0) HIPCHK(hipMalloc(&dlog, (size_t)max_steps * sizeof(double)));
     // set t and total_time

@@ -44,6 +44,7 @@ struct cumc3d_ctx {
     int bfac[6][12];
     cumc3d::Axis ax[3];
     bool uploaded = false, initialized = false, dt_pending = false;
+    int next_stage = 1;           // cumc3d_stage: the stage that runs next (1 between steps)
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;   // TMARKS marks per stage, TSTAGES stages
@@ -224,7 +225,8 @@ template <class W> __global__ __launch_bounds__(256) void k_divb(ArgsT<W> a, uns
 // k_update for the grid's geometry and the stage
 template <class W, int G>
 static void launch_update_g(int stage, dim3 grid, hipStream_t st, const ArgsT<W> &a) {
-    if (stage == 1) hipLaunchKernelGGL((k_update<1, W, G>), grid, dim3(256), 0, st, a);
+    if (stage == 0) hipLaunchKernelGGL((k_update<0, W, G>), grid, dim3(256), 0, st, a);
+    else if (stage == 1) hipLaunchKernelGGL((k_update<1, W, G>), grid, dim3(256), 0, st, a);
     else if (stage == 2) hipLaunchKernelGGL((k_update<2, W, G>), grid, dim3(256), 0, st, a);
     else hipLaunchKernelGGL((k_update<3, W, G>), grid, dim3(256), 0, st, a);
 }
@@ -352,22 +354,39 @@ static inline void tmark(cumc3d_ctx *x, int slot) {
     if (x->timing && x->tstage < TSTAGES) hipEventRecord(x->ev[x->tstage * TMARKS + slot], x->stream);
 }
 
+// SPATIAL up to l_operator: the three sweeps (fluxes, face and cell EMFs)
+// and the edge EMFs of the current state; `marks` records the stage's
+// timing events
+template <class W> static void sweeps_t(cumc3d_ctx *x, const ArgsT<W> &a, hipStream_t st, bool marks) {
+    auto mark = [&](int slot) { if (marks) tmark(x, slot); };
+    mark(0);
+    launch_sweep(x, a, 0, st);
+    if (x->ev_x) hipEventRecord(x->ev_x, st);   // the face-0 density flux the accretion sum reads
+    mark(1);
+    if (x->d.n_dim > 1) launch_sweep(x, a, 1, st);
+    mark(2);
+    if (x->d.n_dim > 2) launch_sweep(x, a, 2, st);
+    mark(3);
+    hipLaunchKernelGGL(k_emf_prep<W>, dim3(nblk(emf_prep_count(a))), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_edge<W>, dim3(nblk((long)(x->nx + 1) * (x->ny + 1) * (x->nz + 1))), dim3(256), 0, st, a);
+    if (a.fixpole) hipLaunchKernelGGL(k_fixpole<W>, dim3(2 * x->nx), dim3(64), 0, st, a);
+    mark(4);
+}
+
+// SPATIAL alone: l_rk of the current state into lrk (device, (8, G), var
+// fastest); the state is not changed
+template <class W> static void spatial_t(cumc3d_ctx *x, double *lrk) {
+    ArgsT<W> a = make_args<W>(x);
+    a.lrk = lrk;
+    sweeps_t<W>(x, a, x->stream, false);
+    launch_update<W>(x->d.coordinate_flag, 0, dim3(nblk((long)x->nx * x->ny * x->nz)), x->stream, a);
+}
+
 template <class W> static void run_stage_t(cumc3d_ctx *x, int stage) {
     hipStream_t st = x->stream;
     ArgsT<W> a = make_args<W>(x);
     const long nn = (long)x->nx * x->ny * x->nz;
-    tmark(x, 0);
-    launch_sweep(x, a, 0, st);
-    if (x->ev_x) hipEventRecord(x->ev_x, st);   // the face-0 density flux the accretion sum reads
-    tmark(x, 1);
-    if (x->d.n_dim > 1) launch_sweep(x, a, 1, st);
-    tmark(x, 2);
-    if (x->d.n_dim > 2) launch_sweep(x, a, 2, st);
-    tmark(x, 3);
-    hipLaunchKernelGGL(k_emf_prep<W>, dim3(nblk(emf_prep_count(a))), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_edge<W>, dim3(nblk((long)(x->nx + 1) * (x->ny + 1) * (x->nz + 1))), dim3(256), 0, st, a);
-    if (a.fixpole) hipLaunchKernelGGL(k_fixpole<W>, dim3(2 * x->nx), dim3(64), 0, st, a);
-    tmark(x, 4);
+    sweeps_t<W>(x, a, st, true);
     launch_update<W>(x->d.coordinate_flag, stage, dim3(nblk(nn)), st, a);
     tmark(x, 5);
     x->cur = 1 - x->cur;
@@ -409,6 +428,7 @@ template <class W> static void sel_collect_t(cumc3d_ctx *x, int stride, unsigned
 // Entry points of the FP32 translation unit (fp32.hip).
 namespace f32 {
 void run_stage(cumc3d_ctx *x, int stage);
+void spatial(cumc3d_ctx *x, double *lrk);
 void sel_infer(cumc3d_ctx *x);
 void sel_collect(cumc3d_ctx *x, int stride, unsigned seed, float t1, float t2);
 void finddt(cumc3d_ctx *x);

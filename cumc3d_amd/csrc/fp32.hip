@@ -10,6 +10,8 @@ namespace f32 {
 
 void run_stage(cumc3d_ctx *x, int stage) { run_stage_t<float>(x, stage); }
 
+void spatial(cumc3d_ctx *x, double *lrk) { spatial_t<float>(x, lrk); }
+
 void sel_infer(cumc3d_ctx *x) { sel_infer_t<float>(x); }
 
 void sel_collect(cumc3d_ctx *x, int stride, unsigned seed, float t1, float t2) {

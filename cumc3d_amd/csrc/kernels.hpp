@@ -85,6 +85,8 @@ template <class W> struct ArgsT {
     // collapsar: potential over the ghosted box, atmosphere prim_a(irho:ivz) and eps_a
     const double *phi;
     double pa[4], eps_a;
+    // SPATIAL alone (k_update<0>): l_rk(1:8) in the host layout, var fastest
+    double *lrk;
 };
 
 // Load field[idx] as scalar base + 32-bit byte offset (the global_load saddr
@@ -860,7 +862,8 @@ __device__ __forceinline__ double cfl_dt_rt(const A &a, int j, int k, int l, dou
 
 // One interior cell: FLUX_DIFF + sources + l_operator + RK combine +
 // FROMUTORVE + CUSTOM_CHECKRHO + FROMRVETOU + FINDPRESSURE; returns the
-// cell's CFL time step in stage 3.
+// cell's CFL time step in stage 3.  STAGE 0 is SPATIAL alone: it stores the
+// cell's l_rk (and that of its lowest faces) in a.lrk and changes no state.
 template <int STAGE, class W, int G>
 __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, int l, double dt) {
     constexpr bool split = sizeof(W) == 4;   // FP32 split: U holds the conserved state
@@ -1001,6 +1004,18 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
                                lB_z<G>(a, j, k, ey_c, ey_mx, ex_c, ex_my)};
         const double Llo[2] = {lB_x<G>(a, j - 1, k, l, ez_mx, ez_mxmy, ey_mx, ey_mxmz),
                                lB_y<G>(a, j, k - 1, l, ex_my, ex_mymz, ez_my, ez_mxmy)};
+        if constexpr (STAGE == 0) {
+            // l_rk(1:5) of the cell, l_rk(6:8) of its upper faces and of the
+            // faces at index 0 (spatial.f90:620-702); a face shared with a
+            // lower cell is that cell's upper face
+            double *o = a.lrk;
+            for (int i = 0; i < 5; ++i) o[8 * c + i] = L[i];
+            for (int d = 0; d < 3; ++d) o[8 * c + 5 + d] = Lup[d];
+            if (j == 1) o[8 * (c - 1) + 5] = Llo[0];
+            if (k == 1) o[8 * (c - sy) + 6] = Llo[1];
+            if (l == 1) o[8 * (c - sz) + 7] = lB_z<G>(a, j, k, ey_mz, ey_mxmz, ex_mz, ex_mymz);
+            return 1.0e5;
+        }
         for (int d = 0; d < 3; ++d) {
             double bup = a.Bin[d][c];
             double uoup;
@@ -1082,7 +1097,7 @@ __global__ __launch_bounds__(256) void k_update(ArgsT<W> a) {
     if (!a.S->active) {
         // a step past total_time: keep the state, but the host swaps the face
         // buffers after every stage, so carry the interior faces across
-        if (live) {
+        if (STAGE != 0 && live) {
             const long c = cidx(a, j, k, l);
             for (int d = 0; d < 3; ++d) a.Bout[d][c] = a.Bin[d][c];
         }

@@ -70,6 +70,17 @@ INTERFACE
     TYPE(C_PTR), VALUE :: ctx
     REAL(C_DOUBLE), VALUE :: dt
   END FUNCTION
+  INTEGER(C_INT) FUNCTION cumc3d_stage(ctx, stage, dt) BIND(C, NAME='cumc3d_stage')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR), VALUE :: ctx
+    INTEGER(C_INT), VALUE :: stage
+    REAL(C_DOUBLE), VALUE :: dt
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION cumc3d_spatial(ctx, l_rk) BIND(C, NAME='cumc3d_spatial')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR), VALUE :: ctx
+    REAL(C_DOUBLE), INTENT(INOUT) :: l_rk(*)
+  END FUNCTION
   INTEGER(C_INT) FUNCTION cumc3d_run(ctx, max_steps, total_time, sync_each_step, t, dts, steps_done) &
       BIND(C, NAME='cumc3d_run')
     IMPORT :: C_INT, C_PTR, C_DOUBLE

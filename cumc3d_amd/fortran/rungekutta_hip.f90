@@ -184,6 +184,17 @@ SUBROUTINE CUMC3D_HIP_SYNC_HOST
 #endif
 END SUBROUTINE
 
+! SPATIAL (spatial.f90:11-162) of the device state into the module's l_rk:
+! l_rk(1:5) of cells 1..n and l_rk(6:8) of the faces the update advances
+! (the other elements keep their values), for checking the operator against
+! the reference's own SPATIAL.  The device state is not changed.
+SUBROUTINE CUMC3D_HIP_SPATIAL
+  USE DEFINITION, ONLY : l_rk
+  IMPLICIT NONE
+  CALL CUMC3D_HIP_ATTACH
+  CALL cumc3d_check(cumc3d_spatial(hip_ctx, l_rk), 'SPATIAL')
+END SUBROUTINE
+
 SUBROUTINE CUMC3D_HIP_DETACH
   IMPLICIT NONE
   IF (.NOT. hip_attached) RETURN

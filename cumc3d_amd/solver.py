@@ -26,7 +26,7 @@ PD = C.POINTER(C.c_double)
 
 # symbols declared in include/cumc3d_hip.h
 EXPORTS = ("cumc3d_init", "cumc3d_upload", "cumc3d_initial_update", "cumc3d_finddt",
-           "cumc3d_rk3_step", "cumc3d_run", "cumc3d_download", "cumc3d_profile", "cumc3d_gravity",
+           "cumc3d_rk3_step", "cumc3d_stage", "cumc3d_spatial", "cumc3d_run", "cumc3d_download", "cumc3d_profile", "cumc3d_gravity",
            "cumc3d_divb",
            "cumc3d_finalize", "cumc3d_error_string", "cumc3d_last_error",
            "cumc3d_set_timing", "cumc3d_get_timing",
@@ -90,6 +90,8 @@ def load_library(path: str = LIB_PATH):
     L.cumc3d_initial_update.argtypes = [vp]
     L.cumc3d_finddt.argtypes = [vp, PD]
     L.cumc3d_rk3_step.argtypes = [vp, C.c_double]
+    L.cumc3d_stage.argtypes = [vp, C.c_int, C.c_double]
+    L.cumc3d_spatial.argtypes = [vp, PD]
     L.cumc3d_run.argtypes = [vp, C.c_int, C.c_double, C.c_int, PD, PD, C.POINTER(C.c_int)]
     L.cumc3d_download.argtypes = [vp, PD, PD, PD, PD]
     L.cumc3d_profile.argtypes = [vp, PD, PD, PD]
@@ -214,6 +216,26 @@ class Solver:
         _check(load_library().cumc3d_rk3_step(self._h, dt))
         self.t += dt
         self.nstep += 1
+
+    def stage(self, stage: int, dt: float):
+        """One stage of RUNGEKUTTA (1, 2, 3 in order, the step's dt each time);
+        stage 3 ends the step."""
+        _check(load_library().cumc3d_stage(self._h, stage, dt))
+        if stage == 3:
+            self.t += dt
+            self.nstep += 1
+
+    def spatial(self, l_rk: np.ndarray | None = None) -> dict:
+        """SPATIAL on the current state: l_rk as lrk1..lrk8 (ghosted box
+        arrays).  l_rk(1:5) is formed on cells 1..n and l_rk(6:8) on the faces
+        the update advances; elsewhere the values of `l_rk` (shape + (8,),
+        zeros if None) are kept."""
+        cfg = self.cfg
+        buf = np.zeros(cfg.shape + (8,)) if l_rk is None else np.array(l_rk, dtype=np.float64, order="C")
+        if buf.shape != cfg.shape + (8,):
+            raise ValueError(f"l_rk must have shape {cfg.shape + (8,)}")
+        _check(load_library().cumc3d_spatial(self._h, _p(buf)))
+        return {f"lrk{i + 1}": np.ascontiguousarray(buf[..., i]) for i in range(8)}
 
     def run(self, max_steps: int, total_time: float | None = None, sync_each_step: bool = True) -> np.ndarray:
         """main.f90's loop on the device; returns the dt of every step taken."""

@@ -126,6 +126,25 @@ int cumc3d_finddt(cumc3d_ctx *ctx, double *dt);
 /* RUNGEKUTTA (rungekutta.f90:12-208): one SSP-RK3 step with the given dt. */
 int cumc3d_rk3_step(cumc3d_ctx *ctx, double dt);
 
+/* One stage of RUNGEKUTTA, for checking a step stage by stage: stage 1
+ * (rungekutta.f90:37-103: u_old backup, SPATIAL, U1 = u_old + dt L,
+ * FROMUTORVE, CUSTOM_CHECKRHO, FROMRVETOU, BOUNDARY, UPDATE(1)), stage 2
+ * (:105-147) or stage 3 (:149-199, which ends the step).  The stages of a
+ * step run 1, 2, 3 in order with the step's dt (CUMC3D_ESTATE otherwise);
+ * cumc3d_rk3_step and cumc3d_run refuse to start while a step is part done.
+ * Stages 1-3 in a row give exactly cumc3d_rk3_step. */
+int cumc3d_stage(cumc3d_ctx *ctx, int stage, double dt);
+
+/* SPATIAL (spatial.f90:11-162) on the current state, for checking the
+ * operator on its own: l_rk(1:8, -2:nx+3, -2:ny+3, -2:nz+3) in the AoS layout
+ * above.  Written: l_rk(1:5) of cells 1..n, and l_rk(6:8) of the faces the
+ * update advances (index 0..n along the field component, 1..n across);
+ * every other element keeps the value it has in l_rk on entry.  The state
+ * is not changed.  Like the reference's SPATIAL it rewrites the fluxes and
+ * EMFs and advances the HLLD carry (CUMC3D_BN_CARRY), which the next stage
+ * reads. */
+int cumc3d_spatial(cumc3d_ctx *ctx, double *l_rk);
+
 /* The time loop of main.f90:381-399 on the device: up to max_steps steps of
  * FINDDT, dt clip to total_time, RUNGEKUTTA, global_time += dt, stopping once
  * global_time >= total_time.  With sync_each_step = 0 the steps are queued

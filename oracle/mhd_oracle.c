@@ -1562,41 +1562,52 @@ double oracle_finddt(oracle *o)
     return fmin2(1.0e5, dt_out);
 }
 
-/* RUNGEKUTTA, rungekutta.f90:37-199 */
-void oracle_rungekutta(oracle *o, double dt)
+/* SPATIAL on its own (spatial.f90:11-162): l_rk of the current state, with
+ * SPATIAL's side effects (the fluxes, the HLLD carry, the collapsar's mflux) */
+void oracle_spatial(oracle *o) { spatial(o); }
+
+/* One stage of RUNGEKUTTA (rungekutta.f90:37-199): stage 1 backs up u_old
+ * (:37-50), every stage runs SPATIAL, its combine, FROMUTORVE, CUSTOM_CHECKRHO
+ * (twice at stage 3, around the empty OPERATOR_SPLIT), FROMRVETOU, BOUNDARY
+ * and UPDATE(stage).  Stage 3 ends the step (n_step, main.f90:112). */
+void oracle_stage(oracle *o, int stage, double dt)
 {
     const double rk20 = 3.0 / 4.0, rk21 = 1.0 / 4.0, rk22 = 1.0 / 4.0;
     const double rk30 = 1.0 / 3.0, rk31 = 2.0 / 3.0, rk32 = 2.0 / 3.0;
-    for (int stage = 1; stage <= 3; ++stage) {
-        if (stage == 1) {
-            for (int l = 0; l <= o->nz; ++l)
-                for (int k = 0; k <= o->ny; ++k)
-                    for (int j = 0; j <= o->nx; ++j)
-                        for (int i = 1; i <= 8; ++i) {
-                            long c = CI(o, j, k, l);
-                            o->u_old[i][c] = o->cons[i][c];
-                        }
-        }
-        spatial(o);
+    if (stage == 1) {
         for (int l = 0; l <= o->nz; ++l)
             for (int k = 0; k <= o->ny; ++k)
                 for (int j = 0; j <= o->nx; ++j)
                     for (int i = 1; i <= 8; ++i) {
                         long c = CI(o, j, k, l);
-                        double uo = o->u_old[i][c], u = o->cons[i][c], lr = o->l_rk[i][c];
-                        if (stage == 1) o->cons[i][c] = uo + dt * lr;
-                        else if (stage == 2) o->cons[i][c] = rk20 * uo + rk21 * u + rk22 * dt * lr;
-                        else o->cons[i][c] = rk30 * uo + rk31 * u + rk32 * dt * lr;
+                        o->u_old[i][c] = o->cons[i][c];
                     }
-        fromutorve(o);
-        checkrho(o);
-        if (stage == 3) checkrho(o);   /* OPERATOR_SPLIT is empty in both models */
-        fromrvetou(o);
-        boundary(o);
-        update(o);
-        custom_update(o, stage, dt);
     }
-    o->nstep = o->nstep + 1;   /* main.f90:112, after the step */
+    spatial(o);
+    for (int l = 0; l <= o->nz; ++l)
+        for (int k = 0; k <= o->ny; ++k)
+            for (int j = 0; j <= o->nx; ++j)
+                for (int i = 1; i <= 8; ++i) {
+                    long c = CI(o, j, k, l);
+                    double uo = o->u_old[i][c], u = o->cons[i][c], lr = o->l_rk[i][c];
+                    if (stage == 1) o->cons[i][c] = uo + dt * lr;
+                    else if (stage == 2) o->cons[i][c] = rk20 * uo + rk21 * u + rk22 * dt * lr;
+                    else o->cons[i][c] = rk30 * uo + rk31 * u + rk32 * dt * lr;
+                }
+    fromutorve(o);
+    checkrho(o);
+    if (stage == 3) checkrho(o);   /* OPERATOR_SPLIT is empty in both models */
+    fromrvetou(o);
+    boundary(o);
+    update(o);
+    custom_update(o, stage, dt);
+    if (stage == 3) o->nstep = o->nstep + 1;   /* main.f90:112, after the step */
+}
+
+/* RUNGEKUTTA, rungekutta.f90:12-208 */
+void oracle_rungekutta(oracle *o, double dt)
+{
+    for (int stage = 1; stage <= 3; ++stage) oracle_stage(o, stage, dt);
 }
 
 int oracle_run(oracle *o, int max_steps, double total_time, double *t, double *dts)

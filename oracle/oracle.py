@@ -54,6 +54,8 @@ def lib():
         L.oracle_finddt.argtypes = [C.c_void_p]
         L.oracle_finddt.restype = C.c_double
         L.oracle_rungekutta.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_stage.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.oracle_spatial.argtypes = [C.c_void_p]
         L.oracle_run.argtypes = [C.c_void_p, C.c_int, C.c_double, PD, PD]
         L.oracle_run.restype = C.c_int
         L.oracle_find_divb.argtypes = [C.c_void_p]
@@ -136,6 +138,15 @@ class Oracle:
 
     def rk3(self, dt: float) -> None:
         lib().oracle_rungekutta(self._h, dt)
+
+    def stage(self, s: int, dt: float) -> None:
+        """One stage of RUNGEKUTTA; stages run 1, 2, 3 in order."""
+        lib().oracle_stage(self._h, s, dt)
+
+    def spatial(self) -> dict:
+        """SPATIAL on the current state: l_rk as lrk1..lrk8 on the ghosted box."""
+        lib().oracle_spatial(self._h)
+        return {f"lrk{i}": self.get(f"lrk{i}") for i in range(1, 9)}
 
     def run(self, max_steps: int, total_time: float | None = None) -> np.ndarray:
         T = self.cfg.total_time if total_time is None else total_time

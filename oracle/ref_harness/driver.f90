@@ -14,6 +14,8 @@
 ! (getgrid.f90:26-98, after CUSTOM_GRID and the FIXPOLE override), and with
 ! -DCUMC3D_COLLAPSAR OUT_FILE.grav: m_bh, r_sh and the potential phi of
 ! module CUSTOM_DEF (model/collapsar/src/def.f90:16-36) after the last step.
+! It then runs SPATIAL once more on the final state (with -DHIPDROP the
+! drop-in's CUMC3D_HIP_SPATIAL on the device) and writes l_rk to OUT_FILE.lrk.
 !
 ! usage: cumc3d_ref IC_FILE OUT_FILE MAX_STEPS [THREADS]
 PROGRAM REF_HARNESS
@@ -24,7 +26,7 @@ USE HARNESS_IO
 USE CUSTOM_DEF, ONLY : phi, m_bh, r_sh
 #endif
 #ifdef HIPDROP
-USE CUMC3D_HIP_STATE, ONLY : CUMC3D_HIP_SYNC_HOST, CUMC3D_HIP_DETACH
+USE CUMC3D_HIP_STATE, ONLY : CUMC3D_HIP_SYNC_HOST, CUMC3D_HIP_SPATIAL, CUMC3D_HIP_DETACH
 #endif
 IMPLICIT NONE
 CHARACTER(len=1024) :: out_path, arg
@@ -97,6 +99,17 @@ WRITE(u) bcell
 WRITE(u) cs
 WRITE(u) cons
 WRITE(u) dts(1:cnt)
+CLOSE(u)
+! SPATIAL once more, on the final state (spatial.f90:11-162), for its l_rk
+! on its own: after the dump, so nothing written above depends on it.
+#ifdef HIPDROP
+CALL CUMC3D_HIP_SPATIAL
+#else
+CALL SPATIAL
+#endif
+OPEN(NEWUNIT=u, FILE=TRIM(out_path)//'.lrk', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE', &
+     ACTION='WRITE')
+WRITE(u) l_rk
 CLOSE(u)
 #ifdef CUMC3D_COLLAPSAR
 OPEN(NEWUNIT=u, FILE=TRIM(out_path)//'.grav', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE', &

@@ -118,6 +118,10 @@ def run_ref(case: str, ic_path: str, steps: int, threads: int = 1, timeout: floa
             res["grid"] = read_grid(out + ".grid")
         if os.path.exists(out + ".grav"):
             res["grav"] = read_grav(out + ".grav", cfg.shape)
+        if os.path.exists(out + ".lrk"):
+            # l_rk(1:8, -2:n+3, ...) of one more SPATIAL on the final state
+            lrk = np.fromfile(out + ".lrk", dtype=np.float64).reshape(tuple(cfg.shape) + (8,))
+            res["lrk"] = {f"lrk{i + 1}": np.ascontiguousarray(lrk[..., i]) for i in range(8)}
         if os.path.exists(out + ".divb"):
             # FIND_DIVB's log record, WRITE(999,*) global_time, maxDivB
             with open(out + ".divb") as f:

@@ -23,6 +23,37 @@ def interior_window(cfg):
     return tuple(sl)
 
 
+def lrk_mask(cfg, i):
+    """Where the library forms l_rk(i) (1..8), on the fixture window of
+    interior_window: l_rk(1:5) on cells 1..n, l_rk(6:8) on the faces the update
+    advances (0..n along the component's own axis, 1..n across).  The reference
+    also forms the face values on the ghost rows across (spatial.f90:639-702),
+    which no stage reads back."""
+    own = {6: 2, 7: 1, 8: 0}.get(i)    # array axis (z, y, x) of the face normal
+    shape, sl = [], []
+    for a, (n, active) in enumerate(((cfg.nz, cfg.n_dim > 2), (cfg.ny, cfg.n_dim > 1), (cfg.nx, True))):
+        if not active:
+            shape.append(1)
+            sl.append(slice(0, 1))
+        else:
+            shape.append(n + 2)
+            sl.append(slice(0 if a == own else 1, n + 1))
+    m = np.zeros(shape, dtype=bool)
+    m[tuple(sl)] = True
+    return m
+
+
+def check_lrk(got, gold, cfg, label=""):
+    """got: lrk1..lrk8 over the ghosted box; gold: over the fixture window."""
+    w = interior_window(cfg)
+    for i in range(1, 9):
+        m = lrk_mask(cfg, i)
+        a, b = got[f"lrk{i}"][w][m], gold[f"lrk{i}"][m]
+        if not np.array_equal(a, b):
+            d = np.abs(a - b)
+            raise AssertionError(f"{label}l_rk({i}): {np.count_nonzero(d)} differ, max {np.nanmax(d):.3e}")
+
+
 def load_golden(name):
     path = os.path.join(GOLDEN, f"{name}.npz")
     if not os.path.exists(path):

@@ -5,7 +5,9 @@ Runs oracle/_ref/<case>/cumc3d_ref (the reference Fortran compiled from its own
 sources by oracle/build_ref.py) single-threaded with zero-filled allocations
 (oracle/refio.run_ref) on each case of oracle/cases.py and stores, per case,
 the dt sequence and the final state on cells 0..n+1 of every axis:
-rho, vx, vy, vz, p, bx, by, bz (face), eps, bcx, bcy, bcz; for spherical
+rho, vx, vy, vz, p, bx, by, bz (face), eps, bcx, bcy, bcz; lrk1..lrk8, the
+l_rk of one more SPATIAL on the final state (the harness calls it after its
+dump), on the same cells; for spherical
 cases also the GETGRID metric arrays the step reads (grid_*); for the
 collapsar also m_bh, r_sh and the potential phi on cells 0..n+1.
 
@@ -56,6 +58,10 @@ def main(names):
             # the reference's GETGRID arrays (after FIXPOLE), -3..n+3 at j+3
             for k, v in ref["grid"].items():
                 out["grid_" + k] = v
+        if "lrk" in ref:
+            # l_rk of one more SPATIAL on the final state, same cells
+            for k, v in ref["lrk"].items():
+                out[k] = v[w]
         if "grav" in ref:
             cfg = case.cfg
             out["m_bh"] = np.array(ref["grav"]["m_bh"])

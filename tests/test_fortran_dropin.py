@@ -14,7 +14,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from conftest import check_profile, interior_window, load_golden
+from conftest import check_lrk, check_profile, interior_window, load_golden
 from cumc3d_amd import ic
 from oracle import oracle as O
 from oracle.build_ref import DROPIN_CASES
@@ -52,6 +52,8 @@ def test_dropin_matches_reference(name):
         g = got["grav"]
         assert g["m_bh"] == float(gold["m_bh"]) and g["r_sh"] == float(gold["r_sh"])
         assert np.array_equal(g["phi"][2:cfg.nz + 4, 2:cfg.ny + 4, 2:cfg.nx + 4], gold["phi"])
+    # CUMC3D_HIP_SPATIAL on the final state into the module's l_rk
+    check_lrk(got["lrk"], gold, case.cfg, label=f"{name} (CUMC3D_HIP_SPATIAL): ")
     check_profile(got["profiles"]["rkiter-1-nm.hdf5"], case.cfg, got, got["t"])
     o = O.Oracle(case.cfg, st)         # the state after INITIAL_UPDATE
     check_profile(got["profiles"]["rkiter-0-nm.hdf5"], case.cfg, {f: o.get(f) for f in FIELDS[:9]}, 0.0)

@@ -4,7 +4,7 @@ bit for bit.  Needs an MI355X."""
 import numpy as np
 import pytest
 
-from conftest import check_profile, interior_window, load_golden
+from conftest import check_lrk, check_profile, interior_window, load_golden, lrk_mask
 from cumc3d_amd import hydroprofile
 from cumc3d_amd import ic
 from cumc3d_amd.config import Config, collapsar, orszag_tang_3d, torus_cyl, torus_sph
@@ -45,6 +45,72 @@ def test_gpu_matches_reference_fixture(name):
         cfg = case.cfg
         box = (slice(2, cfg.nz + 4), slice(2, cfg.ny + 4), slice(2, cfg.nx + 4))
         assert np.array_equal(g["phi"][box], gold["phi"]), "phi differs"
+    # SPATIAL once more on the final state (cumc3d_spatial)
+    check_lrk(s.spatial(), gold, case.cfg, label=name + ": ")
+
+
+@pytest.mark.parametrize("name", ["ot_hlld_ppm", "ot2d_hlld_ppm", "torus_hlld_ppm", "sph_torus_ppm",
+                                  "collapsar2d_npot2"])
+def test_gpu_stages_match_oracle(name):
+    """cumc3d_stage: the state after every stage bit for bit against the
+    oracle's stages, the three stages in a row against cumc3d_rk3_step, and
+    cumc3d_spatial between two stages against the oracle's SPATIAL there."""
+    case = CASES[name]
+    cfg = case.cfg
+    st0 = ic.GENERATORS[case.ic](cfg)
+    print(f"case {name} stage by stage", flush=True)
+    s, whole, o = make_solver(cfg, st0), make_solver(cfg, st0), O.Oracle(cfg, st0)
+    w = interior_window(cfg)
+    for step in range(3):
+        dt = s.finddt()
+        assert dt == o.finddt() and dt == whole.finddt()
+        for k in (1, 2, 3):
+            s.stage(k, dt)
+            o.stage(k, dt)
+            assert_same(s.state(), o.state(), w, label=f"step {step} stage {k}: ")
+            if step == 2 and k == 1:
+                # the carry SPATIAL advances is the same on both sides
+                lr, lo = s.spatial(), o.spatial()
+                check_lrk(lr, {f: v[w] for f, v in lo.items()}, cfg, label="mid-step SPATIAL: ")
+        if step < 2:
+            whole.rungekutta(dt)
+            assert_same(s.state(), whole.state(), label=f"step {step} vs cumc3d_rk3_step: ")
+
+
+def test_gpu_stage_order_and_spatial_keeps_the_rest():
+    """Stages run 1, 2, 3; a whole step or run cannot start in the middle of
+    one.  cumc3d_spatial leaves the elements it does not form as given and
+    does not change the state."""
+    from cumc3d_amd.solver import CumcError
+    case = CASES["ot_hlld_ppm"]
+    cfg = case.cfg
+    s = make_solver(cfg, ic.GENERATORS[case.ic](cfg))
+    dt = s.finddt()
+    with pytest.raises(CumcError):
+        s.stage(2, dt)
+    with pytest.raises(CumcError):
+        s.stage(4, dt)
+    s.stage(1, dt)
+    with pytest.raises(CumcError):
+        s.rungekutta(dt)
+    with pytest.raises(CumcError):
+        s.run(1)
+    with pytest.raises(CumcError):
+        s.stage(3, dt)
+    before = s.state()
+    fill = np.full(cfg.shape + (8,), 7.25)
+    lr = s.spatial(fill)
+    assert_same(s.state(), before, label="state after SPATIAL: ")
+    w = interior_window(cfg)
+    for i in range(1, 9):
+        outside = lr[f"lrk{i}"].copy()
+        inside = np.zeros(cfg.shape, dtype=bool)
+        inside[w] = lrk_mask(cfg, i)
+        assert np.all(outside[~inside] == 7.25), i
+        assert np.all(np.isfinite(outside[inside])) and np.any(outside[inside] != 7.25), i
+    s.stage(2, dt)
+    s.stage(3, dt)
+    s.rungekutta(s.finddt())
 
 
 @pytest.mark.parametrize("threads", [1, 37])

@@ -38,6 +38,11 @@ def test_oracle_matches_reference_fixture(name):
     assert o.divb() == float(gold["maxdivb"])
     if "phi" in gold:   # collapsar: black hole and potential after the last UPDATE
         check_gravity(o.gravity(), gold, CASES[name].cfg)
+    # SPATIAL once more on the final state: its l_rk, every variable
+    lrk = o.spatial()
+    for i in range(1, 9):
+        got = lrk[f"lrk{i}"][w]
+        assert np.array_equal(got, gold[f"lrk{i}"]), f"l_rk({i}): max diff {np.nanmax(np.abs(got - gold[f'lrk{i}']))}"
 
 
 def check_gravity(g, gold, cfg):
@@ -294,6 +299,55 @@ def test_divb_zero_away_from_frozen_faces():
     bmag = np.abs(bx).max()
     assert np.abs(div).max() * h / bmag < 1e-12
     assert o.divb() > 1e-3    # the frozen lower faces break it at cells 1
+
+
+def test_spatial_closed_forms():
+    """SPATIAL's l_rk on its own (spatial.f90:11-162): exactly zero for a
+    uniform state; with no field on a periodic box the hydro part telescopes
+    (its sum over the cells vanishes to rounding); and the face-B part, the
+    discrete curl of the edge EMFs, has zero discrete divergence in every
+    cell 1..n, the frozen lowest faces included (they change the state, not
+    the operator).  Face values are checked on the faces of cells 1..n; the
+    reference also forms l_rk on the faces of the ghost rows (index 0 across
+    the face), which no stage reads back."""
+    cfg = Config(nx=8, ny=8, nz=8, n_dim=3, recon="ppm", riemann="hlld", boundary_flag=(0,) * 6)
+    st = ic.empty_state(cfg)
+    for f, v in (("rho", 1.3), ("vx", 0.2), ("vy", -0.1), ("vz", 0.05), ("p", 0.7),
+                 ("bx", 0.4), ("by", -0.3), ("bz", 0.2)):
+        st[f][:] = v
+    st["eps"][:] = st["p"] / st["rho"] / (cfg.ggas - 1)
+    o = O.Oracle(cfg, st)
+    lr = o.spatial()
+    I = (slice(3, -3),) * 3          # cells 1..n
+    for i in range(1, 6):
+        assert np.all(lr[f"lrk{i}"][I] == 0.0), i
+    for i in range(6, 9):            # faces 0..n along their own axis
+        F = list(I)
+        F[8 - i] = slice(2, -3)
+        assert np.all(lr[f"lrk{i}"][tuple(F)] == 0.0), i
+
+    case = CASES["ot_hlld_weno"]
+    st = ic.GENERATORS[case.ic](case.cfg)
+    for b in ("bx", "by", "bz"):
+        st[b][:] = 0.0
+    o = O.Oracle(case.cfg, st)
+    o.run(3)
+    lr = o.spatial()
+    for i in range(1, 6):
+        L = lr[f"lrk{i}"][I]
+        assert abs(L.sum()) <= 1e-13 * np.abs(L).sum(), i
+
+    case = CASES["ot_hlld_ppm"]
+    cfg = case.cfg
+    o = O.Oracle(cfg, ic.GENERATORS[case.ic](cfg))
+    o.run(4)
+    lr = o.spatial()
+    n = cfg.nx
+    c, m = slice(3, n + 3), slice(2, n + 2)     # cells 1..n and their lower faces
+    bx, by, bz = lr["lrk6"], lr["lrk7"], lr["lrk8"]
+    div = (bx[c, c, c] - bx[c, c, m]) + (by[c, c, c] - by[c, m, c]) + (bz[c, c, c] - bz[m, c, c])
+    scale = max(np.abs(bx[I]).max(), np.abs(by[I]).max(), np.abs(bz[I]).max())
+    assert scale > 0.1 and np.abs(div).max() <= 1e-13 * scale
 
 
 SPHERICAL = [n for n in CASES if CASES[n].cfg.coordinate_flag == 2]
