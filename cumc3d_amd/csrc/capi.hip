@@ -661,14 +661,21 @@ int cumc3d_finddt(cumc3d_ctx *x, double *dt) {
     if (!x->dt_pending) launch_finddt(x);
     x->dt_pending = false;
     unsigned long long bits = 0;
+    int bad = 0;
     HIPCHK(hipMemcpyAsync(&bits, &x->S->dtmin_bits, sizeof(bits), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipMemcpyAsync(&bad, &x->S->nonfinite, sizeof(bad), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     double m;
     memcpy(&m, &bits, sizeof(m));
     *dt = m < 1.0e5 ? m : 1.0e5;
     unsigned long long reset = 0x7E37E43C8800759CULL;
+    const int clear = 0;
     HIPCHK(hipMemcpyAsync(&x->S->dtmin_bits, &reset, sizeof(reset), hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(&x->S->nonfinite, &clear, sizeof(clear), hipMemcpyHostToDevice, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
+    if (bad)
+        return fail(CUMC3D_ENONFINITE,
+                    "a cell's CFL time step was NaN or non-positive (NaN or negative energy in the state)");
     if (!(*dt > 0.0) || !std::isfinite(*dt)) return fail(CUMC3D_ENONFINITE, "FINDDT produced a non-positive dt");
     return 0;
 }

@@ -120,7 +120,9 @@ int cumc3d_upload(cumc3d_ctx *ctx, const double *prim, const double *eps, const 
  * the potential solved from a zero guess (model/collapsar/src/custom.f90:481-649). */
 int cumc3d_initial_update(cumc3d_ctx *ctx);
 
-/* FINDDT (rungekutta.f90:221-332): CFL time step, returned to the host. */
+/* FINDDT (rungekutta.f90:221-332): CFL time step, returned to the host.
+ * CUMC3D_ENONFINITE when a cell's candidate is NaN or not positive (NaN or
+ * negative energy in the state), where the reference would step on. */
 int cumc3d_finddt(cumc3d_ctx *ctx, double *dt);
 
 /* RUNGEKUTTA (rungekutta.f90:12-208): one SSP-RK3 step with the given dt. */

@@ -192,6 +192,21 @@ def test_gpu_run_stops_at_total_time():
     assert np.array_equal(od, dts)
 
 
+def test_gpu_nan_state_is_reported():
+    """A NaN cell gives a NaN CFL step: FINDDT and the device loop return
+    CUMC3D_ENONFINITE instead of stepping on with the other cells' minimum."""
+    from cumc3d_amd.solver import CumcError
+    case = CASES["ot_hlld_ppm"]
+    st = ic.GENERATORS[case.ic](case.cfg)
+    st["rho"][6, 7, 8] = np.nan
+    with pytest.raises(CumcError, match="NaN"):
+        make_solver(case.cfg, st).finddt()
+    with pytest.raises(CumcError, match="NaN"):
+        make_solver(case.cfg, st).run(2)
+    good = make_solver(case.cfg, ic.GENERATORS[case.ic](case.cfg))
+    assert good.finddt() > 0.0
+
+
 def test_gpu_deterministic():
     case = CASES["ot_hlld_weno"]
     st = ic.GENERATORS[case.ic](case.cfg)
