@@ -48,6 +48,8 @@ SUBROUTINE CUMC3D_HIP_ATTACH
   REAL(C_DOUBLE), ALLOCATABLE :: floors(:,:,:,:)
   INTEGER :: o(16)
   IF (hip_attached) RETURN
+  ! prim, cons and l_rk cross the C-ABI as (8, -2:n+3, ...) blocks
+  IF (imin /= 1 .OR. imax /= 8) STOP 'cumc3d drop-in: the state must hold exactly the 8 MHD variables'
   d%nx = nx
   d%ny = ny
   d%nz = nz
