@@ -172,7 +172,6 @@ static SorArgs sor_args(cumc3d_ctx *x) {
     a.part_nan = x->part_nan;
     a.part_max = x->part_max;
     a.nblk = x->sor_nblk;
-    a.nbx = (int)nblk((long)x->nx * x->ny, SOR_SPAN);
     a.relax_max = x->grav_p.relax_max;
     a.pa = x->grav_p.prim_a[0];
     a.omega = x->grav_p.omega;
@@ -202,7 +201,7 @@ static int sor_solve(cumc3d_ctx *x, bool zero_guess) {
                        (const double *)x->P[0], x->sor_rho);
     const long hx = (x->nx + 1) / 2;
     // red: one more block column, whose block (0, 0) decides the previous pair
-    const dim3 gr(nblk(hx * x->ny) + 1, x->nz), gb(nblk((long)x->nx * x->ny, SOR_SPAN), x->nz);
+    const dim3 gr(nblk(hx * x->ny) + 1, x->nz), gb(nblk(hx * x->ny, SOR_SLOTS), x->nz);
     const dim3 gc(nblk((long)x->nx * x->ny), x->nz);
     hipLaunchKernelGGL(k_sor_copy, gc, dim3(256), 0, st, a, x->phi, x->sor_phi[0][0], x->sor_phi[0][1], 1);
     long queued = 0;
@@ -575,7 +574,7 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
             return fail(CUMC3D_EHIP, "accretion stream");
         }
         // phi and its second buffer (G each), the coefficients, the error blocks' partials
-        x->sor_nblk = (int)nblk((long)d.nx * d.ny, SOR_SPAN) * d.nz;
+        x->sor_nblk = (int)nblk((long)((d.nx + 1) / 2) * d.ny, SOR_SLOTS) * d.nz;
         const size_t Gp = ((size_t)x->G + 3) / 4 * 4;
         const size_t Hp = ((size_t)((d.nx + 1) / 2) * d.ny * d.nz + 3) / 4 * 4;   // one colour of phi
         const size_t bytes = (Gp + 4 * Hp + pc.size() + 2 * (size_t)x->sor_nblk) * sizeof(double);

@@ -41,7 +41,8 @@
 
 namespace cumc3d {
 
-constexpr int SOR_SPAN = 1024;   // interior cells per black block (4 per thread)
+// colour-split slots of each colour per black block (1024 cells, 4 per thread)
+constexpr int SOR_SLOTS = 512;
 
 // Per-cell SOR data are split by colour so that each sweep reads its cells
 // densely: colour c (0 red, 1 black) holds, for row (k, l), the cells
@@ -71,8 +72,7 @@ struct SorArgs {
     const double *zpm, *zpc, *zcm;      // [l]: zp1-zm1, zp1-zc, zc-zm1
     long long *part_nan;                // per black block: q of its last NaN term, -1 if none
     double *part_max;                   // per black block: max of the terms after it, -1 if none
-    int nblk;                           // black blocks: ceil(nx ny / SOR_SPAN) per plane, times nz
-    int nbx;                            // black blocks per plane
+    int nblk;                           // black blocks: ceil(hx ny / SOR_SLOTS) per plane, times nz
     int relax_max;
     int zero_ghosts;                    // the first pair after the zero guess: boundary values 0
     int combine;                        // red kernel: block 0 first decides the previous pair's exit
@@ -266,32 +266,46 @@ __device__ __forceinline__ double block_max_d(double v) {
     return m;
 }
 
+// Two partials of consecutive q ranges, the right one folded into the left:
+// the last NaN of the right range supersedes everything before it.
+__device__ __forceinline__ void fold_partial(long long &L, double &M, long long L2, double M2) {
+    if (L2 >= 0) {
+        L = L2;
+        M = M2;
+    } else {
+        M = M2 > M ? M2 : M;
+    }
+}
+
 // The previous pair's abserror from the black blocks' partials and the exit
-// decision (custom.f90:638, :644), by one block of 256 threads.
+// decision (custom.f90:638, :644), by one block of 256 threads.  The
+// partials are in q order (block index plane-major), so their left-to-right
+// fold is (the last NaN q, the maximum of the terms after it): each thread
+// folds a contiguous run of them, then the threads' results are folded in
+// thread order (lane i takes lane i + off at each level of the tree).
 __device__ void sor_decide(const SorArgs &a) {
-    const int nxy = a.nx * a.ny;
     DevScalars *S = a.S;
+    const int per = (a.nblk + 255) >> 8;
+    const int i0 = (int)threadIdx.x * per, i1 = i0 + per < a.nblk ? i0 + per : a.nblk;
     long long L = -1;
-    for (int i = threadIdx.x; i < a.nblk; i += blockDim.x) {
-        const long long v = a.part_nan[i];
-        L = v > L ? v : L;
-    }
-    L = block_max_ll(L);
-    // block i covers q = plane + (i % nbx) SOR_SPAN ... up to the plane's end
     double M = -1.0;
-    for (int i = threadIdx.x; i < a.nblk; i += blockDim.x) {
-        if (L >= 0) {
-            const long long plane = (long long)(i / a.nbx) * nxy;
-            long long last_q = plane + (long long)(i % a.nbx) * SOR_SPAN + SOR_SPAN - 1;
-            if (last_q > plane + nxy - 1) last_q = plane + nxy - 1;   // the plane's last block is partial
-            if (last_q < L) continue;   // wholly before the last NaN
-        }
-        const double v = a.part_max[i];
-        M = v > M ? v : M;
+    for (int i = i0; i < i1; ++i) fold_partial(L, M, a.part_nan[i], a.part_max[i]);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long L2 = __shfl_down(L, off, 64);
+        const double M2 = __shfl_down(M, off, 64);
+        if ((threadIdx.x & 63) + off < 64) fold_partial(L, M, L2, M2);
     }
-    M = block_max_d(M);
+    __shared__ long long wl[4];
+    __shared__ double wm[4];
+    if ((threadIdx.x & 63) == 0) {
+        wl[threadIdx.x >> 6] = L;
+        wm[threadIdx.x >> 6] = M;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
-    const long long N = (long long)nxy * a.nz;
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) fold_partial(L, M, wl[w], wm[w]);
+    const long long N = (long long)a.nx * a.ny * a.nz;
     double err;
     if (L == N - 1) err = 1.0e-50;   // the private result is NaN: the combine keeps 1e-50
     else err = M > 1.0e-50 ? M : 1.0e-50;
@@ -333,31 +347,37 @@ __global__ __launch_bounds__(256) void k_sor_red(SorArgs a) {
     a.rout[ic] = sor_cell(a, j, k, l, rho_in, nb, old);
 }
 
-// Every interior cell, x fastest: black cells relax from the new red values;
-// every cell forms its error term |dphi/phi_old| (custom.f90:593) from pin
-// and pout and the block folds them into its partial.  Grid (x: the plane's
-// cells / SOR_SPAN, y: l - 1): a block covers SOR_SPAN consecutive q of one
-// plane, SOR_SPAN / 256 cells per thread.
+// Every interior cell, in the colour-split order: block b of plane l takes
+// the split indices [b S, (b+1) S) of both colours (S = SOR_SLOTS), the cells
+// (k, j = 2i+1, 2i+2) of rows k at index i; together they are one contiguous
+// range of q.  A thread's slots 0-1 are black cells, which relax from the new
+// red values, and its slots 2-3 red ones, so the lanes of a wave share a
+// colour and neither branch diverges.  Every cell forms its error term
+// |dphi/phi_old| (custom.f90:593) from the pair's start and end values, and
+// the block folds them into its partial: the fold is a maximum, so the order
+// of the slots does not matter.
 __global__ __launch_bounds__(256) void k_sor_black(SorArgs a) {
     if (a.S->sor_state || !a.S->active) return;
-    constexpr int PER = SOR_SPAN / 256;
-    const int nxy = a.nx * a.ny;
+    const int hx = (a.nx + 1) >> 1, nh = hx * a.ny, nxy = a.nx * a.ny;
     const int l = 1 + blockIdx.y;
-    double e[PER];
-    long long qq[PER];
+    double e[4];
+    long long qq[4];
     long long last = -1;
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const unsigned t = blockIdx.x * SOR_SPAN + u * 256 + threadIdx.x;
+    for (int u = 0; u < 4; ++u) {
+        const int colour = u < 2 ? 1 : 0;
+        const int s = blockIdx.x * SOR_SLOTS + (u & 1) * 256 + (int)threadIdx.x;
         qq[u] = -1;
         e[u] = -1.0;
-        if (t >= (unsigned)nxy) continue;
-        qq[u] = (long long)(l - 1) * nxy + t;
-        const int k = 1 + (int)(t / (unsigned)a.nx), j = 1 + (int)(t - (unsigned)(k - 1) * a.nx);
-        const long ic = sor_i(a, j, k, l);
+        if (s >= nh) continue;
+        const int k = 1 + s / hx, i = s - (k - 1) * hx;
+        const int j = (((1 + k + l) & 1) == colour ? 1 : 2) + 2 * i;
+        if (j > a.nx) continue;   // the padded slot of an odd row
+        qq[u] = (long long)(l - 1) * nxy + (long long)(k - 1) * a.nx + (j - 1);
+        const long ic = (long)s + (long)nh * (l - 1);   // sor_i(a, j, k, l)
         double old, nw;
-        if ((j + k + l) & 1) {
-            const double rho_in = a.rho_in[sor_rec(a, 1, (j - 1) >> 1, k, l)];
+        if (colour) {
+            const double rho_in = a.rho_in[(long)nh * a.nz + ic];   // sor_rec(a, 1, i, k, l)
             old = a.bin[ic];
             double nb[6];
             sor_neigh(a, a.rout, j, k, l, old, nb);
@@ -368,12 +388,12 @@ __global__ __launch_bounds__(256) void k_sor_black(SorArgs a) {
             nw = a.rout[ic];
         }
         e[u] = fabs((nw - old) / old);
-        if (e[u] != e[u]) last = qq[u];   // q grows with u: the thread's last NaN
+        if (e[u] != e[u]) last = qq[u] > last ? qq[u] : last;
     }
     last = block_max_ll(last);
     double m = -1.0;
 #pragma unroll
-    for (int u = 0; u < PER; ++u)
+    for (int u = 0; u < 4; ++u)
         if (qq[u] > last && e[u] == e[u]) m = e[u] > m ? e[u] : m;
     m = block_max_d(m);
     if (threadIdx.x == 0) {
