@@ -248,7 +248,9 @@ def assert_gravity(s, o, cfg):
     (collapsar(48, 32, n_dim=2, recon="weno", riemann="hll"), 21),
     (collapsar(20, 12, 96, recon="ppm", riemann="hlld"), 11),
     (collapsar(24, 16, n_dim=2, recon="tvdmc", riemann="hlld").with_(n_pot=3), 10),
-], ids=["2d_48x32_weno_hll", "3d_20x12x96_ppm", "2d_npot3_tvdmc"])
+    # odd nx: padded colour slots, and a black SOR block boundary inside a row
+    (collapsar(41, 30, n_dim=2, recon="ppm", riemann="hllc"), 8),
+], ids=["2d_48x32_weno_hll", "3d_20x12x96_ppm", "2d_npot3_tvdmc", "2d_41x30_odd_ppm_hllc"])
 def test_gpu_collapsar_matches_oracle(cfg, steps):
     """Several Poisson solves (the initial update, steps 0, n_pot, ...) and the
     accretion of every stage against the C oracle: state, dt, m_bh, r_sh,
