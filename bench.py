@@ -439,8 +439,8 @@ def main(argv=None):
     ap.add_argument("--thr-weno", type=float, default=1e-2)
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print("bench.py measures one MI355X: this build has no multi-GPU path "
-              "(the halo-exchange design is described in DESIGN.md); run with --gpus 1.")
+        print("bench.py measures one MI355X: the step has no multi-GPU path "
+              "(no halo exchange between GPUs); run with --gpus 1.")
         return 2
 
     if args.workload == "collapsar":
