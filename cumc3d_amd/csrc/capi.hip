@@ -22,6 +22,30 @@ static int fail(int code, const std::string &msg) {
     return code;
 }
 
+// A device temporary of one entry point, freed on every return path
+// (hipFree waits for the work queued on it).
+struct DevTmp {
+    void *p = nullptr;
+    DevTmp() = default;
+    DevTmp(const DevTmp &) = delete;
+    DevTmp &operator=(const DevTmp &) = delete;
+    ~DevTmp() {
+        if (p) hipFree(p);
+    }
+};
+
+// Two timing events of one call, destroyed on every return path.
+struct EvPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EvPair() = default;
+    EvPair(const EvPair &) = delete;
+    EvPair &operator=(const EvPair &) = delete;
+    ~EvPair() {
+        if (a) hipEventDestroy(a);
+        if (b) hipEventDestroy(b);
+    }
+};
+
 #define HIPCHK(x)                                                                         \
     do {                                                                                  \
         hipError_t e_ = (x);                                                              \
@@ -188,11 +212,11 @@ static SorArgs sor_args(cumc3d_ctx *x) {
 // grav_error (the reference STOPs there); one that ran out continues.
 static int sor_solve(cumc3d_ctx *x, bool zero_guess) {
     hipStream_t st = x->stream;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    EvPair ev;
     if (x->timing) {
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, st));
+        HIPCHK(hipEventCreate(&ev.a));
+        HIPCHK(hipEventCreate(&ev.b));
+        HIPCHK(hipEventRecord(ev.a, st));
     }
     if (zero_guess) hipLaunchKernelGGL(k_fill0, dim3(nblk(x->G)), dim3(256), 0, st, x->phi, x->G);
     hipLaunchKernelGGL(k_sor_begin, dim3(1), dim3(1), 0, st, x->S);
@@ -219,7 +243,7 @@ static int sor_solve(cumc3d_ctx *x, bool zero_guess) {
             ++queued;
         }
         hipLaunchKernelGGL(k_sor_decide, dim3(1), dim3(256), 0, st, a);
-        if (e1) HIPCHK(hipEventRecord(e1, st));
+        if (ev.b) HIPCHK(hipEventRecord(ev.b, st));
         HIPCHK(hipMemcpyAsync(flags, &x->S->sor_state, sizeof(flags), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (flags[0]) break;
@@ -234,16 +258,14 @@ static int sor_solve(cumc3d_ctx *x, bool zero_guess) {
                            dim3(256), 0, st, x->phi, x->nx, x->ny, x->nz, x->NX, x->NY);
     }
     if (flags[2] == x->grav_p.relax_max) x->grav_error = 1;
-    if (e0) {
+    if (ev.a) {
         float ms = 0;
-        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess && flags[1] > 0) {
+        if (hipEventElapsedTime(&ms, ev.a, ev.b) == hipSuccess && flags[1] > 0) {
             x->tsum[8] += ms;
             x->tcnt[8] += 1;
             x->tsum[9] += flags[1];
             x->tcnt[9] += 1;
         }
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
     }
     return 0;
 }
@@ -604,8 +626,9 @@ int cumc3d_upload(cumc3d_ctx *x, const double *prim, const double *eps, const do
     if (!x || !prim || !eps) return fail(CUMC3D_EINVAL, "null argument");
     if ((x->d.model == CUMC3D_MODEL_TORUS || x->d.model == CUMC3D_MODEL_TORUS_SPH) && !floors3)
         return fail(CUMC3D_EINVAL, "torus models need floor arrays");
-    double *tmp = nullptr;
-    HIPCHK(hipMalloc(&tmp, (size_t)9 * x->G * sizeof(double)));
+    DevTmp buf;
+    HIPCHK(hipMalloc(&buf.p, (size_t)9 * x->G * sizeof(double)));
+    double *tmp = (double *)buf.p;
     HIPCHK(hipMemcpyAsync(tmp, prim, (size_t)8 * x->G * sizeof(double), hipMemcpyHostToDevice, x->stream));
     HIPCHK(hipMemcpyAsync(tmp + (size_t)8 * x->G, eps, x->G * sizeof(double), hipMemcpyHostToDevice, x->stream));
     x->cur = 0;
@@ -619,7 +642,6 @@ int cumc3d_upload(cumc3d_ctx *x, const double *prim, const double *eps, const do
             HIPCHK(hipMemcpyAsync(x->floors[i], floors3 + (size_t)i * x->G, x->G * sizeof(double),
                                   hipMemcpyHostToDevice, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
-    HIPCHK(hipFree(tmp));
     x->uploaded = true;
     x->initialized = false;
     return 0;
@@ -724,23 +746,17 @@ int cumc3d_spatial(cumc3d_ctx *x, double *l_rk) {
     if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
     if (int rc = grav_check(x)) return rc;
     const size_t bytes = (size_t)8 * x->G * sizeof(double);
-    double *d = nullptr;
-    HIPCHK(hipMalloc(&d, bytes));
+    DevTmp buf;
+    HIPCHK(hipMalloc(&buf.p, bytes));
+    double *d = (double *)buf.p;
     // the entries SPATIAL does not form keep the caller's values
-    int rc = 0;
-    if (hipMemcpyAsync(d, l_rk, bytes, hipMemcpyHostToDevice, x->stream) != hipSuccess) rc = CUMC3D_EHIP;
-    if (!rc) {
-        hipLaunchKernelGGL(k_activate, dim3(1), dim3(1), 0, x->stream, x->S);
-        if (x->fp32) f32::spatial(x, d);
-        else spatial_t<double>(x, d);
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(l_rk, d, bytes, hipMemcpyDeviceToHost, x->stream) != hipSuccess ||
-            hipStreamSynchronize(x->stream) != hipSuccess)
-            rc = CUMC3D_EHIP;
-    }
-    hipStreamSynchronize(x->stream);
-    hipFree(d);
-    if (rc) return fail(rc, "SPATIAL failed on the device");
+    HIPCHK(hipMemcpyAsync(d, l_rk, bytes, hipMemcpyHostToDevice, x->stream));
+    hipLaunchKernelGGL(k_activate, dim3(1), dim3(1), 0, x->stream, x->S);
+    if (x->fp32) f32::spatial(x, d);
+    else spatial_t<double>(x, d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(l_rk, d, bytes, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
     return 0;
 }
 
@@ -749,8 +765,9 @@ int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_st
     if (!x || !t) return fail(CUMC3D_EINVAL, "null argument");
     if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
     if (x->next_stage != 1) return fail(CUMC3D_ESTATE, "a step run by cumc3d_stage is not finished");
-    double *dlog = nullptr;
-    if (dts && max_steps > 0) HIPCHK(hipMalloc(&dlog, (size_t)max_steps * sizeof(double)));
+    DevTmp dbuf;
+    if (dts && max_steps > 0) HIPCHK(hipMalloc(&dbuf.p, (size_t)max_steps * sizeof(double)));
+    double *dlog = (double *)dbuf.p;
     // set t and total_time
     DevScalars h;
     HIPCHK(hipMemcpy(&h, x->S, sizeof(h), hipMemcpyDeviceToHost));
@@ -768,11 +785,11 @@ int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_st
         x->sel_timed.assign(TSTAGES, 0);
         x->tstage = 0;
     }
-    hipEvent_t s0 = nullptr, s1 = nullptr;
+    EvPair ev;
     if (x->timing) {
-        HIPCHK(hipEventCreate(&s0));
-        HIPCHK(hipEventCreate(&s1));
-        HIPCHK(hipEventRecord(s0, x->stream));
+        HIPCHK(hipEventCreate(&ev.a));
+        HIPCHK(hipEventCreate(&ev.b));
+        HIPCHK(hipEventRecord(ev.a, x->stream));
     }
     int n = 0, grc = 0;
     double tcur = *t;
@@ -791,15 +808,13 @@ int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_st
         }
     }
     if (x->timing) {
-        HIPCHK(hipEventRecord(s1, x->stream));
-        HIPCHK(hipEventSynchronize(s1));
+        HIPCHK(hipEventRecord(ev.b, x->stream));
+        HIPCHK(hipEventSynchronize(ev.b));
         float ms = 0;
-        hipEventElapsedTime(&ms, s0, s1);
+        hipEventElapsedTime(&ms, ev.a, ev.b);
         accumulate_timing(x);
         x->tsum[6] = ms;
         x->tcnt[6] = n > 0 ? n : 1;
-        hipEventDestroy(s0);
-        hipEventDestroy(s1);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -812,7 +827,6 @@ int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_st
         HIPCHK(hipMemcpy(dts, dlog, (size_t)(done < max_steps ? done : max_steps) * sizeof(double),
                          hipMemcpyDeviceToHost));
     }
-    if (dlog) HIPCHK(hipFree(dlog));
     if (grc) return grc;
     if (h.nonfinite) {
         h.nonfinite = 0;
@@ -825,9 +839,10 @@ int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_st
 
 int cumc3d_download(cumc3d_ctx *x, double *prim, double *eps, double *bcell, double *cs) {
     if (!x) return fail(CUMC3D_EINVAL, "null context");
-    double *tmp = nullptr;
-    if (prim || bcell) HIPCHK(hipMalloc(&tmp, (size_t)8 * x->G * sizeof(double)));
-    if (!tmp && x->fp32 && (eps || cs)) HIPCHK(hipMalloc(&tmp, (size_t)x->G * sizeof(double)));
+    DevTmp buf;
+    if (prim || bcell) HIPCHK(hipMalloc(&buf.p, (size_t)8 * x->G * sizeof(double)));
+    if (!buf.p && x->fp32 && (eps || cs)) HIPCHK(hipMalloc(&buf.p, (size_t)x->G * sizeof(double)));
+    double *tmp = (double *)buf.p;
     const unsigned gg = nblk(x->G);
     if (prim) {
         if (x->fp32) f32::join_prim(x, tmp);
@@ -856,7 +871,6 @@ int cumc3d_download(cumc3d_ctx *x, double *prim, double *eps, double *bcell, dou
         }
         HIPCHK(hipStreamSynchronize(x->stream));
     }
-    if (tmp) HIPCHK(hipFree(tmp));
     return 0;
 }
 
@@ -865,8 +879,9 @@ int cumc3d_profile(cumc3d_ctx *x, double *primitive, double *epsilon, double *bf
     if (!x->uploaded) return fail(CUMC3D_ESTATE, "upload an initial model first");
     const size_t np = (size_t)(x->nx + 2) * (x->ny + 2) * (x->nz + 2);
     const size_t nb = (size_t)(x->nx + 3) * (x->ny + 3) * (x->nz + 3);
-    double *tmp = nullptr;
-    HIPCHK(hipMalloc(&tmp, (5 * np + np + 3 * nb) * sizeof(double)));
+    DevTmp buf;
+    HIPCHK(hipMalloc(&buf.p, (5 * np + np + 3 * nb) * sizeof(double)));
+    double *tmp = (double *)buf.p;
     double *dp = primitive ? tmp : nullptr, *de = epsilon ? tmp + 5 * np : nullptr,
            *db = bfield ? tmp + 6 * np : nullptr;
     if (x->fp32)
@@ -879,7 +894,6 @@ int cumc3d_profile(cumc3d_ctx *x, double *primitive, double *epsilon, double *bf
     if (epsilon) HIPCHK(hipMemcpyAsync(epsilon, de, np * sizeof(double), hipMemcpyDeviceToHost, x->stream));
     if (bfield) HIPCHK(hipMemcpyAsync(bfield, db, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
-    HIPCHK(hipFree(tmp));
     return 0;
 }
 
@@ -898,14 +912,15 @@ int cumc3d_gravity(cumc3d_ctx *x, double *phi, double *m_bh, double *r_sh, int *
 
 int cumc3d_divb(cumc3d_ctx *x, double *maxdivb) {
     if (!x || !maxdivb) return fail(CUMC3D_EINVAL, "null argument");
-    unsigned long long *d = nullptr, h = 0;
-    HIPCHK(hipMalloc(&d, sizeof(*d)));
+    DevTmp buf;
+    unsigned long long h = 0;
+    HIPCHK(hipMalloc(&buf.p, sizeof(h)));
+    unsigned long long *d = (unsigned long long *)buf.p;
     HIPCHK(hipMemsetAsync(d, 0, sizeof(*d), x->stream));
     hipLaunchKernelGGL(k_divb<double>, dim3(nblk((long)x->nx * x->ny * x->nz)), dim3(256), 0, x->stream,
                        make_args<double>(x), d);
     HIPCHK(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
-    HIPCHK(hipFree(d));
     memcpy(maxdivb, &h, sizeof(double));
     return 0;
 }
