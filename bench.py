@@ -18,16 +18,19 @@ The same steps are timed three times from the same initial state:
   fp32_sel  the FP32 split with the learned selector choosing TVD-MC, PPM or
             WENO-Z per cell and direction, retrained here on samples the solver
             generates from a 64^3 run of the same physics.
-value = zone-updates/s of fp32_sel; the others and the ratio are reported
-beside it, with the density L1 error of both FP32 variants against fp64 after
-the same steps with the same dt sequence.
+Each variant is timed --repeats times (default 5), every run from a fresh
+solver on the same initial state with its own warmup; ms_per_step is the
+median run's (all runs are listed).  value = zone-updates/s of fp32_sel; the
+others and the ratio are reported beside it, with the density L1 error of
+both FP32 variants against fp64 after the same steps with the same dt
+sequence.
 
 The roofline object is for the dominant kernel of the fp32_sel run:
 HIP-event durations on the library's stream, algorithmic bytes per launch.
 The CPU baseline is the reference Fortran (oracle/_ref, built from its own
 sources) on a bounded sample of the same physics.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--repeats R] [--n 256]
                        [--workload ot|blast|turbulence] [--no-cpu]
 """
 from __future__ import annotations
@@ -422,6 +425,9 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="timed runs of each variant, each from a fresh solver on the same initial state; "
+                         "value and ms_per_step are their median")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--workload", default="ot",
                     choices=["ot", "blast", "turbulence", "torus_sph", "torus_cyl", "collapsar"])
@@ -456,11 +462,18 @@ def main(argv=None):
                                      iters=args.train_iters)
     sel_info["refresh_every"] = SEL_EVERY
     print(f"[bench] selector: {sel_info}", file=sys.stderr, flush=True)
-    res = {}
+    res, walls = {}, {}
     for var in args.variants.split(","):
-        print(f"[bench] {label}: {var} warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
-        res[var] = time_variant(cfg, st, var, args.warmup, args.steps, sel_w)
-        print(f"[bench] {var}: {1e3 * res[var][0] / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
+        runs = []
+        for r in range(max(1, args.repeats)):
+            print(f"[bench] {label}: {var} run {r + 1}: warmup {args.warmup}, timed {args.steps}",
+                  file=sys.stderr, flush=True)
+            runs.append(time_variant(cfg, st, var, args.warmup, args.steps, sel_w))
+            print(f"[bench] {var}: {1e3 * runs[-1][0] / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
+        walls[var] = sorted(w for w, _, _ in runs)
+        # the median run's wall time; its kernel timing and selector fractions
+        # are the first run's
+        res[var] = (walls[var][len(runs) // 2], runs[0][1], runs[0][2])
 
     def rate(v):
         return zones * args.steps / res[v][0]
@@ -506,7 +519,9 @@ def main(argv=None):
         "data": f"synthetic (seeded IC, cumc3d_amd.ic.{ {'ot': 'orszag_tang', 'torus_cyl': 'torus'}.get(args.workload, args.workload) })",
         "config": {"workload": label + ", FP32 split + learned selector (FP64 accumulators)", "zones": zones,
                    "riemann": cfg.riemann, "recon": "selector: TVD-MC / PPM / WENO-Z per cell"},
-        "variants": {v: {"value": rate(v), "ms_per_step": 1e3 * res[v][0] / args.steps} for v in res},
+        "variants": {v: {"value": rate(v), "ms_per_step": 1e3 * res[v][0] / args.steps,
+                         "runs_ms_per_step": [round(1e3 * w / args.steps, 4) for w in walls[v]]} for v in res},
+        "repeats": max(1, args.repeats),
         "speedup_vs_fp64": value / rate("fp64") if "fp64" in res else None,
         "selector": dict(sel_info, class_fractions=frac),
         "l1_vs_fp64": err,
