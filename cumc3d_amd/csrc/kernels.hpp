@@ -860,6 +860,15 @@ __device__ __forceinline__ double cfl_dt_rt(const A &a, int j, int k, int l, dou
     return cfl_dt<0>(a, j, k, l, cs, rho, v, b);
 }
 
+// Stores of the update's full-row double-precision outputs (the new face
+// field, the hydro u_old, the FP32 split's conserved state), which no thread
+// of the same launch reads: non-temporal, so they do not displace the
+// neighbouring rows and planes the update reads from L2.  (Measured on
+// MI355X at 256^3: update 1.15 -> 1.12 ms and the ghost fill after it
+// 0.109 -> 0.100 ms in the FP32 split.  The single-precision outputs and the
+// u_old face stores stay ordinary: non-temporal, they were slower.)
+__device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+
 // One interior cell: FLUX_DIFF + sources + l_operator + RK combine +
 // FROMUTORVE + CUSTOM_CHECKRHO + FROMRVETOU + FINDPRESSURE; returns the
 // cell's CFL time step in stage 3.  STAGE 0 is SPATIAL alone: it stores the
@@ -896,7 +905,7 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
         u[3] = vz * rho;
         u[4] = rho * (eps + 0.5 * vsq) + 0.5 * bsq;
         if (STAGE == 1) {
-            for (int i = 0; i < 5; ++i) { uo[i] = u[i]; a.Uo[i][c] = u[i]; }
+            for (int i = 0; i < 5; ++i) { uo[i] = u[i]; st_nt(&a.Uo[i][c], u[i]); }
         } else {
             for (int i = 0; i < 5; ++i) uo[i] = a.Uo[i][c];
         }
@@ -1064,7 +1073,7 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
     a.cs[c] = (W)cs;
     for (int d = 0; d < 3; ++d) {
         a.Bc[d][c] = (W)bcf[d];
-        a.Bout[d][c] = bn_up[d];
+        st_nt(&a.Bout[d][c], bn_up[d]);
     }
     // cell EMFs -v x B of the new state (flux_ct, mhd_module.f90:316-318),
     // from the values just stored, for the next stage's edge EMFs
@@ -1075,11 +1084,11 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
         double vsq2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
         double bsq2 = bcf[0] * bcf[0] + bcf[1] * bcf[1] + bcf[2] * bcf[2];
         double *const *dst = STAGE == 3 ? a.U : a.Uo;
-        dst[0][c] = rn;
-        dst[1][c] = v[0] * rn;
-        dst[2][c] = v[1] * rn;
-        dst[3][c] = v[2] * rn;
-        dst[4][c] = rn * (en + 0.5 * vsq2) + 0.5 * bsq2;
+        st_nt(&dst[0][c], rn);
+        st_nt(&dst[1][c], v[0] * rn);
+        st_nt(&dst[2][c], v[1] * rn);
+        st_nt(&dst[3][c], v[2] * rn);
+        st_nt(&dst[4][c], rn * (en + 0.5 * vsq2) + 0.5 * bsq2);
     }
     return STAGE == 3 ? cfl_dt<G>(a, j, k, l, cs, rn, v, bcf) : 1.0e5;
 }
