@@ -1098,11 +1098,11 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
         double vsq2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
         double bsq2 = bcf[0] * bcf[0] + bcf[1] * bcf[1] + bcf[2] * bcf[2];
         double *const *dst = STAGE == 3 ? a.U : a.Uo;
-        st_nt(&dst[0][c], rn);
-        st_nt(&dst[1][c], v[0] * rn);
-        st_nt(&dst[2][c], v[1] * rn);
-        st_nt(&dst[3][c], v[2] * rn);
-        st_nt(&dst[4][c], rn * (en + 0.5 * vsq2) + 0.5 * bsq2);
+        const double un2[5] = {rn, v[0] * rn, v[1] * rn, v[2] * rn, rn * (en + 0.5 * vsq2) + 0.5 * bsq2};
+        for (int i = 0; i < 5; ++i) {
+            if (STAGE == 1) dst[i][c] = un2[i];   // ordinary: non-temporal was slower at stage 1
+            else st_nt(&dst[i][c], un2[i]);
+        }
     }
     return STAGE == 3 ? cfl_dt<G>(a, j, k, l, cs, rn, v, bcf) : 1.0e5;
 }
