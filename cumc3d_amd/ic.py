@@ -169,11 +169,13 @@ def blast(cfg: Config, seed: int = 20261202, edge_cells: float = 1.0) -> dict:
     """Magnetised blast: rho=1 (+-1% noise), p=0.1 outside and 10 inside r<0.1
     around the box centre, v=0, uniform B=(10/sqrt2, 10/sqrt2, 0)/sqrt(4 pi).
 
-    The pressure edge is a tanh of width ``edge_cells`` cells: with a one-cell
-    step, WENO-Z reconstruction of the primitives (weno_module.f90:315-408)
-    overshoots to negative face pressures at the 100x jump and the HLLD
-    solver returns NaN within the first step, in the reference algorithm as in
-    this one."""
+    The pressure edge is a tanh of width ``edge_cells`` cells;
+    ``edge_cells=0`` gives the sharp step the configuration specifies (p=10
+    where r<0.1).  With the sharp step, WENO-Z reconstruction of the
+    primitives (weno_module.f90:315-408) overshoots to negative face
+    pressures at the 100x jump and the HLLD solver returns NaN within the
+    first step, in the reference algorithm as in this one (pinned by
+    tests/test_oracle.py); the default smooths the edge over one cell."""
     g = Grid(cfg)
     X, Y, Z = g.centres()
     st = empty_state(cfg)
@@ -183,8 +185,11 @@ def blast(cfg: Config, seed: int = 20261202, edge_cells: float = 1.0) -> dict:
     cy = 0.5 * (cfg.y_start + cfg.y_end)
     cz = 0.5 * (cfg.z_start + cfg.z_end)
     r = np.sqrt((X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2)
-    w = edge_cells * (cfg.x_end - cfg.x_start) / cfg.nx
-    st["p"][:] = 0.1 + 9.9 * 0.5 * (1.0 - np.tanh((r - 0.1) / w))
+    if edge_cells > 0:
+        w = edge_cells * (cfg.x_end - cfg.x_start) / cfg.nx
+        st["p"][:] = 0.1 + 9.9 * 0.5 * (1.0 - np.tanh((r - 0.1) / w))
+    else:
+        st["p"][:] = np.where(r < 0.1, 10.0, 0.1)
     b = 10.0 / math.sqrt(2.0) / math.sqrt(4 * math.pi)
     st["bx"][:] = b
     st["by"][:] = b

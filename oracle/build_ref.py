@@ -15,7 +15,8 @@ Recipe (test infrastructure only):
     initial model (oracle/ref_harness/ic_plain.f90 / ic_torus.f90 /
     ic_collapsar.f90); the collapsar's CUSTOM_GRID reads the case's faces
     from ./profile/*_grid.dat, which oracle/refio.py writes next to the run;
-  * flang -O2 -ffp-contract=off -fopenmp, as planned for the CPU baseline.
+  * flang -O2 -ffp-contract=off -fopenmp for the parity cases; the CPU-baseline
+    samples (BENCH_CASES) with -O3 -ffp-contract=off -fopenmp.
 Outputs: oracle/_ref/<case>/cumc3d_ref.
 
 For the cases in DROPIN_CASES it also links oracle/_ref/dropin/<case>/cumc3d_dropin:
@@ -36,9 +37,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
-from cases import BENCH_CASES, CASES  # noqa: E402
-
-ALL_CASES = {**CASES, **BENCH_CASES}
+from cases import ALL_CASES, BENCH_CASES  # noqa: E402
 
 REF = os.environ.get("CUMC3D_REFERENCE", "/root/reference")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
@@ -51,6 +50,7 @@ DROPIN_SRC = [os.path.join(REPO, "cumc3d_amd", "fortran", f)
 DROPIN_CASES = ("c1_briowu", "ot_hlld_weno", "ot_bc_mixed", "torus_hlld_ppm", "ot2d_hlld_ppm",
                 "sph_torus_ppm", "sph_blast_weno", "collapsar2d_ppm", "collapsar_ppm")
 FFLAGS = ["-cpp", "-O2", "-ffp-contract=off", "-fopenmp", "-DDIVB", "-DFIXPOLE"]
+FFLAGS_BENCH = ["-cpp", "-O3", "-ffp-contract=off", "-fopenmp", "-DDIVB", "-DFIXPOLE"]
 
 # solver/mhd/source.list minus main.f90, dataio.f90 (HDF5) and openacc.f90
 # (device data regions only), in module dependency order.
@@ -180,6 +180,7 @@ def build_case(name: str) -> str:
         # no extra gravity source (custom.f90:143).
         with open(os.path.join(work, "param.h"), "w") as f:
             f.write("INTEGER, PARAMETER :: test_model = 0\n")
+    fflags = FFLAGS_BENCH if name in BENCH_CASES else FFLAGS
     srcs = [(os.path.join(REF, "solver/mhd", s), s) for s in SOLVER]
     srcs += [(os.path.join(REF, mdir, s), "model_" + s) for s in msrc]
     ic_src = {"torus": "ic_torus.f90", "torus_sph": "ic_torus.f90", "collapsar": "ic_collapsar.f90"}
@@ -192,11 +193,11 @@ def build_case(name: str) -> str:
         os.symlink(path, os.path.join(work, link))
         extra = ["-DNOT=.NOT."] if link == "getgrid.f90" else []
         obj = link.replace(".f90", ".o")
-        run([FLANG, *FFLAGS, *extra, "-c", link, "-o", obj], work)
+        run([FLANG, *fflags, *extra, "-c", link, "-o", obj], work)
         objs.append(obj)
     for h in harness:
         obj = h.replace(".f90", ".o")
-        run([FLANG, *FFLAGS, *(dflags if h == "driver.f90" else []), "-c", os.path.join(HERE, "ref_harness", h),
+        run([FLANG, *fflags, *(dflags if h == "driver.f90" else []), "-c", os.path.join(HERE, "ref_harness", h),
              "-o", obj], work)
         objs.append(obj)
     outdir = os.path.join(OUT, name)

@@ -17,7 +17,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from cumc3d_amd.config import Config, briowu_1d, collapsar, torus_sph  # noqa: E402
+from cumc3d_amd.config import (Config, briowu_1d, collapsar, orszag_tang_3d, torus_sph,  # noqa: E402
+                               turbulence_3d)
 
 
 @dataclasses.dataclass
@@ -48,6 +49,9 @@ CASES = {c.name: c for c in [
     Case("ot_hllc_ppm", _ot(12, "ppm", "hllc"), "orszag_tang", 12),
     # outflow and reflecting boundaries in 3D
     Case("ot_bc_mixed", _ot(12, "weno", "hlld", bc=(2, 2, 1, 1, 2, 1)), "orszag_tang", 10),
+    # decaying turbulence (config 4 physics at 24^3, HLLD + WENO-Z): the last
+    # step before the reference scheme's first non-finite state (step 12)
+    Case("c4_turb24_weno", turbulence_3d(24, recon="weno", riemann="hlld"), "turbulence", 11),
     # magnetised blast (config 3 physics at 16^3)
     Case("blast_weno", Config(nx=16, ny=16, nz=16, n_dim=3, cfl=0.4, ggas=5.0 / 3.0,
                               riemann="hlld", recon="weno", boundary_flag=(0,) * 6),
@@ -113,9 +117,24 @@ CASES = {c.name: c for c in [
 ]}
 
 
+# Production-size parity cases: the reference run single-threaded (its HLLD
+# carry is per-thread under OpenMP), its state after the last step stored as
+# per-field SHA-256 digests plus per-plane sums (tests/golden/large_*.json,
+# written by tests/golden/make_large.py), not as arrays.
+LARGE_CASES = {c.name: c for c in [
+    # configuration 2 exactly: 128^3 Orszag-Tang, HLLD + PPM, FP64
+    Case("c2_ot128_ppm", orszag_tang_3d(128, recon="ppm", riemann="hlld"), "orszag_tang", 20),
+    # configuration 4's physics (256^3 decaying turbulence) at 128^3, HLLD + WENO-Z
+    Case("c4_turb128_weno", turbulence_3d(128, recon="weno", riemann="hlld"), "turbulence", 12),
+]}
+
 # CPU-baseline samples for bench.py: the reference solver on the benchmark
-# physics at a grid that one host run covers in about 10-30 s.
+# physics at a grid that one host run covers in about 10-30 s.  They are
+# built with -O3 (oracle/build_ref.py), the parity cases with -O2.
 BENCH_CASES = {c.name: c for c in [
+    # the headline workload's physics and scheme (256^3 decaying turbulence,
+    # HLLD + WENO-Z) at 128^3
+    Case("bench_turb128", turbulence_3d(128, recon="weno", riemann="hlld"), "turbulence", 2),
     Case("bench_blast128", Config(nx=128, ny=128, nz=128, n_dim=3, cfl=0.4, ggas=5.0 / 3.0,
                                   total_time=0.02, riemann="hlld", recon="weno",
                                   boundary_flag=(0,) * 6), "blast", 3),
@@ -124,4 +143,4 @@ BENCH_CASES = {c.name: c for c in [
     Case("bench_collapsar156", collapsar(156, 44, 128, recon="ppm", riemann="hlld"), "collapsar", 22),
 ]}
 
-ALL_CASES = {**CASES, **BENCH_CASES}
+ALL_CASES = {**CASES, **LARGE_CASES, **BENCH_CASES}

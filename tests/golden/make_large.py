@@ -1,0 +1,74 @@
+#!/usr/bin/env python3
+"""Write the production-size parity fixtures: the reference solver's state on
+the LARGE_CASES of oracle/cases.py (128^3 grids), stored as digests.
+
+Runs oracle/_ref/<case>/cumc3d_ref (the reference Fortran compiled from its own
+sources by oracle/build_ref.py) single-threaded (oracle/refio.run_ref) and
+stores in tests/golden/large_<case>.json:
+  * the dt sequence (hex floats, exact);
+  * t and max div B after the last step (hex floats);
+  * per field (rho, vx, vy, vz, p, bx, by, bz, eps, bcx, bcy, bcz), on cells
+    0..n+1 of every axis as C-order float64: the SHA-256 of the bytes, and the
+    sum over each z plane (hex floats, to locate a difference);
+  * the same digests of l_rk from one more SPATIAL on the final state.
+The arrays themselves (hundreds of MB at 128^3) are not stored.
+
+usage: python tests/golden/make_large.py [case ...]
+"""
+import hashlib
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from cumc3d_amd import ic  # noqa: E402
+from oracle.cases import LARGE_CASES  # noqa: E402
+from oracle.refio import have_ref, run_ref  # noqa: E402
+
+FIELDS = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz", "eps", "bcx", "bcy", "bcz")
+
+
+def window(cfg):
+    """cells 0..n+1 of every axis of a 3D box"""
+    return tuple(slice(2, n + 4) for n in (cfg.nz, cfg.ny, cfg.nx))
+
+
+def digest(a: np.ndarray) -> dict:
+    """SHA-256 of the C-order float64 bytes and the sum over each leading-axis plane."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    planes = [float(p.sum()).hex() for p in a]
+    return {"sha256": hashlib.sha256(a.tobytes()).hexdigest(), "plane_sums": planes}
+
+
+def main(names):
+    for name in names or list(LARGE_CASES):
+        case = LARGE_CASES[name]
+        if not have_ref(name):
+            print("skip", name, "(reference binary not built)")
+            continue
+        st = ic.GENERATORS[case.ic](case.cfg)
+        with tempfile.TemporaryDirectory() as d:
+            p = os.path.join(d, "ic.bin")
+            ic.write_ic(p, case.cfg, st)
+            ref = run_ref(name, p, case.steps, threads=1, timeout=4 * 3600)
+        w = window(case.cfg)
+        out = {"case": name, "steps": int(ref["nstep"]), "dts": [float(v).hex() for v in ref["dts"]],
+               "t": float(ref["t"]).hex(), "maxdivb": float(ref["maxdivb"]).hex(),
+               "fields": {f: digest(ref[f][w]) for f in FIELDS},
+               "lrk": {k: digest(v[w]) for k, v in ref["lrk"].items()},
+               "finite": bool(all(np.all(np.isfinite(ref[f][w])) for f in FIELDS)),
+               "reference_seconds": float(ref["seconds"])}
+        path = os.path.join(HERE, f"large_{name}.json")
+        with open(path, "w") as f:
+            json.dump(out, f, indent=0)
+        print(f"{name}: {ref['nstep']} steps, finite={out['finite']} -> {path} ({os.path.getsize(path) // 1024} KB)")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
