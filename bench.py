@@ -2,36 +2,52 @@
 """Throughput of the MHD update step on one MI355X.
 
 A step is one FINDDT + one SSP-RK3 RUNGEKUTTA over the whole grid (the
-reference's time-loop body, solver/mhd/main.f90:384-392), with the state
-resident in HBM.  Workload (default): the 3D periodic Orszag-Tang vortex of
-the 128^3 configuration run at 256^3 with HLLD + WENO-Z, gamma=5/3, seeded
-+-1% density noise (cumc3d_amd.ic.orszag_tang).  The magnetised blast at 256^3
-is available with --workload blast, but the reference scheme produces NaN in
-it after 7 steps (the reference Fortran does the same at 128^3 after 8), so a
-20-step timing of it would time NaN arithmetic.
+reference's time-loop body, solver/mhd/main.f90:98-113), with the state
+resident in HBM.
 
-The same steps are timed three times from the same initial state:
-  fp64      the FP64 mode, bit-exact with the reference (WENO-Z everywhere);
-  fp32      the FP32 split: single-precision reconstruction, Riemann solver and
-            EMFs; double-precision RK accumulators and energy-to-pressure
-            recovery (WENO-Z everywhere);
-  fp32_sel  the FP32 split with the learned selector choosing TVD-MC, PPM or
-            WENO-Z per cell and direction, retrained here on samples the solver
-            generates from a 64^3 run of the same physics.
-Each variant is timed --repeats times (default 5), every run from a fresh
-solver on the same initial state with its own warmup; ms_per_step is the
-median run's (all runs are listed).  value = zone-updates/s of fp32_sel; the
-others and the ratio are reported beside it, with the density L1 error of
-both FP32 variants against fp64 after the same steps with the same dt
-sequence.
+Workload (default): 3D periodic decaying MHD turbulence at 256^3
+(cumc3d_amd.ic.turbulence: rho=1, beta=1, rms Mach 1 solenoidal velocity with
+|k|<=8 and E(k)~k^-2, mean B along z plus a 50% solenoidal dB), HLLD.  The
+same steps are timed for:
+  fp64        the FP64 mode, bit-exact with the reference (WENO-Z everywhere):
+              the plain FP64 high-order path;
+  fp32        the FP32 split (single-precision reconstruction, Riemann solver
+              and EMFs; double-precision RK accumulators and energy-to-pressure
+              recovery), WENO-Z everywhere;
+  fp32_sel    the FP32 split with the learned selector choosing TVD-MC, PPM or
+              WENO-Z per cell and direction, trained here on samples the
+              solver generates from a 128^3 run of the same physics;
+  fp32_tvdmc, fp32_ppm
+              controls: the FP32 split with TVD-MC or PPM everywhere.
+value = zone-updates/s of fp32_sel.
 
-The roofline object is for the dominant kernel of the fp32_sel run:
-HIP-event durations on the library's stream, algorithmic bytes per launch.
+The reference scheme stays finite on this workload for a limited number of
+steps only (26 at 256^3: the recovered pressure collapses in field-dominated
+cells and NaN enters through c_s = sqrt(<0); tests/test_configs.py pins the
+same on the CPU).  bench.py therefore measures that window first (the FP64
+run from the initial state until FINDDT reports a non-finite cell) and times
+every variant inside it: each timed run starts from the initial state with
+its own `warmup` steps, and `steps` are timed in segments of at most
+window - warmup steps (one segment for the defaults, 5 + 20 <= 26).  A timed
+segment that ends non-finite is an error, not a number.  The density L1
+errors are those of the FP32 variants against fp64 at the last finite FP64
+step, every variant driven with FP64's dt sequence.
+
+The roofline object is for the dominant kernel (the fused update) of the
+fp32_sel run, on the algorithmic byte count of one RK3 step: 512 B per
+zone-update (stage 1 reads U and writes U^1, 16 doubles; stages 2 and 3 read
+U^(s-1) and U^0 and write U^s, 24 doubles each), i.e. 512/3 B per zone per
+update launch on average, over the launch's mean duration measured with HIP
+events on the library's stream during a timed run.  The same launch's
+per-kernel byte model (every field it touches read or written once) is
+reported beside it as kernel_model_bytes.
 The CPU baseline is the reference Fortran (oracle/_ref, built from its own
-sources) on a bounded sample of the same physics.
+sources with -O3 -ffp-contract=off -fopenmp) on a bounded sample of the same
+physics and scheme, on the host share of cores.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--repeats R] [--n 256]
-                       [--workload ot|blast|turbulence] [--no-cpu]
+                       [--workload turbulence|ot|blast|torus_sph|torus_cyl|collapsar]
+                       [--no-cpu] [--no-secondary]
 """
 from __future__ import annotations
 
@@ -41,6 +57,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -49,11 +67,14 @@ HBM_PEAK_GBS = 8000.0
 KERNELS = ["flux_x", "flux_y", "flux_z", "edge", "update", "ghost"]
 
 
-def kernel_bytes(cfg, name: str, w: int) -> float:
-    """Algorithmic HBM bytes of one launch: every field the kernel needs read
-    once and every field it produces written once.  w = bytes of the working
-    type (8 in the FP64 mode, 4 in the FP32 split); the face field, u_old and
-    the FP32 split's conserved state are always 8-byte."""
+def kernel_model_bytes(cfg, name: str, w: int) -> float:
+    """The per-kernel byte model of one launch: every field the kernel needs
+    read once and every field it produces written once.  w = bytes of the
+    working type (8 in the FP64 mode, 4 in the FP32 split); the face field,
+    u_old and the FP32 split's conserved state are always 8-byte.  This is a
+    model of the kernel's own data movement, larger than the step's
+    algorithmic bytes (step_bytes), because the decomposition into sweeps,
+    edge EMFs and the update materialises fluxes, EMFs and primitives."""
     nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
     n = nx * ny * nz
     split = w == 4
@@ -86,27 +107,40 @@ def kernel_bytes(cfg, name: str, w: int) -> float:
     raise KeyError(name)
 
 
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_ot256_hlld.md")
+# Algorithmic bytes of one RK3 step per zone-update: FP64 state of 8 words;
+# stage 1 reads U and writes U^1 (16 words), stages 2 and 3 read U^(s-1) and
+# U^0 and write U^s (24 words each), dt folded into stage 3.  The same for
+# both precisions: the RK accumulators stay FP64.
+STEP_BYTES_PER_ZONE = 512.0
+STAGE_BYTES_PER_ZONE = (128.0, 192.0, 192.0)
 
 
-def measured_traffic(kernel: str, w: int, n: int):
+def traffic_summary(workload: str, n: int) -> str:
+    return os.path.join(ROOT, "profiles", f"traffic_{workload}{n}_hlld.md")
+
+
+def measured_traffic(kernel: str, w: int, workload: str, n: int):
     """HBM bytes per launch of a kernel class from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (calibrated against a copy of known size),
-    averaged over the class's launches (the three RK stages of the update);
-    None when the summary does not cover this workload."""
-    if n != 256 or not os.path.isfile(TRAFFIC_SUMMARY):
+    FETCH_SIZE/WRITE_SIZE summary of this workload (calibrated against a copy
+    of known size), averaged over the class's launches (the three RK stages
+    of the update); None when no summary covers the workload."""
+    path = traffic_summary(workload, n)
+    if not os.path.isfile(path):
         return None
     prefix = {"flux_x": "k_flux_x<", "flux_y": "k_flux_yz<", "flux_z": "k_flux_yz<", "edge": "k_edge<",
               "update": "k_update<", "ghost": "k_ghost<"}[kernel]
     typ = "float" if w == 4 else "double"
     vals = []
-    for line in open(TRAFFIC_SUMMARY):
+    for line in open(path):
         cols = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
         if len(cols) < 6 or not cols[0].startswith(prefix) or typ not in cols[0]:
             continue
         if kernel in ("flux_y", "flux_z") and not cols[0].rstrip(">").endswith("1" if kernel == "flux_y" else "2"):
             continue
-        vals.append(float(cols[5]) * 1e6)
+        try:
+            vals.append(float(cols[5]) * 1e6)
+        except ValueError:
+            continue
     return sum(vals) / len(vals) if vals else None
 
 
@@ -230,22 +264,40 @@ def profile_io(cfg, st, variant: str, sel_w=None, steps: int = 3):
             "hdf5_write_GBs": size / (t3 - t2) / 1e9}
 
 
-def train_selector(n: int, workload: str, thr_ppm: float, thr_weno: float, seed: int = 1, iters: int = 600):
-    """Train the selector on samples the FP32 solver generates from a small run
-    of the same physics (n^3, snapshots every 3 steps); returns its parameters
-    and the training summary."""
-    from cumc3d_amd.solver import make_solver
+def train_selector(n: int, workload: str, thr_ppm: float, thr_weno: float, seed: int = 1, iters: int = 600,
+                   snapshots: int = 6, every: int = 3):
+    """Train the selector on samples the FP32 solver generates from a smaller
+    run of the same physics (n^3, a snapshot every `every` steps, sampled into
+    a reservoir of 2^20); returns its parameters and the training summary.
+    Snapshots stop at the run's first non-finite state (the reference scheme
+    fails on the turbulence after a few tens of steps)."""
+    from cumc3d_amd.solver import ENONFINITE, CumcError, make_solver
     cfg, st, _ = make_workload(workload, n)
     s = make_solver(cfg.with_(precision="fp32"), st)
     s.selector_init(capacity=1 << 20, seed=seed)
-    counts = None
-    for k in range(6):
-        s.run(3, total_time=1e300)
-        counts = s.selector_collect(stride=2, thr_ppm=thr_ppm, thr_weno=thr_weno, seed=k + 1)
+    counts, taken, offered = None, 0, 0
+    stride = max(1, (3 * cfg.zones) // (1 << 19))     # about 2^19 candidates per snapshot
+    for k in range(snapshots):
+        try:
+            s.run(every, total_time=1e300)
+            s.finddt()
+        except CumcError as e:
+            if e.code != ENONFINITE:
+                raise
+            break
+        counts = s.selector_collect(stride=stride, thr_ppm=thr_ppm, thr_weno=thr_weno, seed=k + 1)
+        taken += 1
+        offered += 3 * ((cfg.zones + stride - 1) // stride)
+    if counts is None:
+        raise RuntimeError("selector training: the training run went non-finite before its first snapshot")
+    if np.min(counts[1:]) == 0:
+        print(f"[bench] WARNING: selector training labels {counts[1:].tolist()} leave a class empty",
+              file=sys.stderr, flush=True)
     loss = s.selector_train(iters, blocks=128, lr=3e-3, class_weight=(1.0, 1.0, 2.0), seed=seed)
     w = s.selector_params()
     del s
-    info = {"train_grid": f"{n}^3", "samples": int(counts[0]), "label_counts": counts[1:].tolist(),
+    info = {"train_grid": f"{n}^3", "snapshots": taken, "snapshot_every": every, "stride": stride,
+            "candidates_offered": offered, "samples_held": int(counts[0]), "label_counts": counts[1:].tolist(),
             "thresholds": [thr_ppm, thr_weno], "loss_first": float(loss[:10].mean()),
             "loss_last": float(loss[-10:].mean()), "iters": iters}
     return w, info
@@ -269,10 +321,16 @@ def calibration_copy(nbytes: int = 1 << 30):
     hip.hipFree(b)
 
 
+VARIANT_RECON = {"fp32_tvdmc": "tvdmc", "fp32_ppm": "ppm"}
+
+
 def make_variant(cfg, st, variant: str, sel_w=None):
     from cumc3d_amd.solver import make_solver
     prec = "fp64" if variant == "fp64" else "fp32"
-    s = make_solver(cfg.with_(precision=prec), st)
+    c = cfg.with_(precision=prec)
+    if variant in VARIANT_RECON:
+        c = c.with_(recon=VARIANT_RECON[variant])
+    s = make_solver(c, st)
     if variant == "fp32_sel":
         s.selector_init(capacity=1024)
         s.selector_params(sel_w)
@@ -280,53 +338,101 @@ def make_variant(cfg, st, variant: str, sel_w=None):
     return s
 
 
-def time_variant(cfg, st, variant: str, warmup: int, steps: int, sel_w=None):
-    s = make_variant(cfg, st, variant, sel_w)
-    s.run(warmup, total_time=1e300, sync_each_step=False)
-    s.set_timing(True)
-    s.run(2, total_time=1e300, sync_each_step=False)
-    kt = s.timing()
-    s.set_timing(False)
-    t0 = time.perf_counter()
-    s.run(steps, total_time=1e300, sync_each_step=False)   # returns after a stream sync
-    wall = time.perf_counter() - t0
-    frac = None
+def restart(s, st, variant: str):
+    """Put a variant's context back to the initial state: upload, INITIAL_UPDATE,
+    and for the selector a mask refresh at the next step (as a fresh context)."""
+    s.initial_model(st)
+    s.initial_update()
     if variant == "fp32_sel":
-        import numpy as np
-        cnt = sum(np.bincount(s.selector_mask(d, refresh=False)[3:-3, 3:-3, 3:-3].ravel(), minlength=3)
-                  for d in range(cfg.n_dim))
-        frac = (cnt / cnt.sum()).tolist()
+        s.selector_enable(True, every=SEL_EVERY)
+
+
+def finite_window(cfg, st, cap: int) -> int:
+    """Steps after which the FP64 state is still finite, from the initial
+    state (FINDDT raising CUMC3D_ENONFINITE marks the first non-finite state);
+    `cap` if it stays finite that long."""
+    from cumc3d_amd.solver import ENONFINITE, CumcError
+    s = make_variant(cfg, st, "fp64")
+    k = 0
+    try:
+        while k < cap:
+            s.rungekutta(s.finddt())
+            k += 1
+        s.finddt()
+    except CumcError as e:
+        if e.code != ENONFINITE:
+            raise
+        k -= 1
     del s
-    return wall, kt, frac
+    return k
 
 
-def l1_errors(cfg, st, steps: int, sel_w=None):
-    """Density L1 (absolute, relative) and relative Linf of the FP32 split and
-    of FP32 + selector against FP64 after `steps` steps driven with FP64's dt
-    sequence."""
-    import numpy as np
+def segments(steps: int, seg: int):
+    out = []
+    while steps > 0:
+        out.append(min(seg, steps))
+        steps -= out[-1]
+    return out
+
+
+def time_variant(s, st, variant: str, warmup: int, steps: int, seg: int, kernel_timing: bool = False):
+    """Wall time of `steps` steps, in segments of at most `seg` steps, each
+    from the initial state after `warmup` untimed steps; with kernel_timing
+    the per-kernel HIP-event durations of the timed steps too."""
+    wall, kt = 0.0, None
+    for L in segments(steps, seg):
+        restart(s, st, variant)
+        s.run(warmup, total_time=1e300, sync_each_step=False)
+        if kernel_timing:
+            s.set_timing(True)
+        t0 = time.perf_counter()
+        s.run(L, total_time=1e300, sync_each_step=False)   # returns after a stream sync; raises if non-finite
+        wall += time.perf_counter() - t0
+        if kernel_timing:
+            kt = kt or s.timing()
+            s.set_timing(False)
+    return wall, kt
+
+
+def selector_fractions(s, cfg):
+    cnt = sum(np.bincount(s.selector_mask(d, refresh=False)[3:-3, 3:-3, 3:-3].ravel(), minlength=3)
+              for d in range(cfg.n_dim))
+    return (cnt / cnt.sum()).tolist()
+
+
+def l1_errors(cfg, st, steps: int, variants, sel_w=None):
+    """Density L1 (absolute, relative) and relative Linf of each FP32 variant
+    against FP64 after `steps` steps, every variant driven with FP64's dt
+    sequence; max normalised div B of every variant."""
+    from cumc3d_amd.solver import CumcError
     a = make_variant(cfg, st, "fp64")
-    others = {"fp32": make_variant(cfg, st, "fp32")}
-    if sel_w is not None:
-        others["fp32_sel"] = make_variant(cfg, st, "fp32_sel", sel_w)
-    for _ in range(steps):
+    others = {v: make_variant(cfg, st, v, sel_w) for v in variants if v != "fp64"}
+    failed = {}
+    for k in range(steps):
         dt = a.finddt()
         a.rungekutta(dt)
-        for b in others.values():
-            b.finddt()
-            b.rungekutta(dt)
+        for v, b in others.items():
+            if v in failed:
+                continue
+            try:
+                b.finddt()
+                b.rungekutta(dt)
+            except CumcError as e:
+                failed[v] = f"non-finite at step {k + 1}: {e}"
     w = (slice(3, -3),) * 3
     sa = a.state()
     x = sa["rho"][w]
-    out = {"fp64": {"divb_norm": divb_norm(cfg, sa)}}
-    for k, b in others.items():
+    out = {"steps": steps, "t": a.t, "fp64": {"divb_norm": divb_norm(cfg, sa)}}
+    for v, b in others.items():
+        if v in failed:
+            out[v] = {"error": failed[v]}
+            continue
         sb = b.state()
         y = sb["rho"][w]
         d = np.abs(x - y)
-        out[k] = {"rho_l1": float(d.mean()), "rho_l1_rel": float(d.mean() / np.abs(x).mean()),
-                  "rho_linf_rel": float(d.max() / np.abs(x).max()), "divb_norm": divb_norm(cfg, sb)}
-    out["steps"] = steps
-    out["t"] = a.t
+        out[v] = {"rho_l1": float(d.mean()), "rho_l1_rel": float(d.mean() / np.abs(x).mean()),
+                  "rho_linf_rel": float(d.max() / np.abs(x).max()), "divb_norm": divb_norm(cfg, sb),
+                  "finite": bool(np.all(np.isfinite(y)))}
     return out
 
 
@@ -334,7 +440,6 @@ def divb_norm(cfg, st):
     """max |div B| * h / <|B|> over cells 2..n of a uniform Cartesian grid (the
     lowest face of each axis stays frozen in the reference, so cells 1 are left
     out)."""
-    import numpy as np
     if cfg.coordinate_flag != 0:
         return None     # the curvilinear divergences are checked by the tests
     bx, by, bz = st["bx"], st["by"], st["bz"]
@@ -346,40 +451,79 @@ def divb_norm(cfg, st):
     return float(np.max(np.abs(d)) * h / bm)
 
 
-def cpu_baseline(max_threads: int = 16):
-    """The reference solver (oracle/_ref/bench_blast128) on 3 steps of the blast
-    at 128^3 (steps the reference completes before its NaN), all host threads."""
+def host_share() -> dict:
+    """CPUs this process may use: the affinity mask, capped by a cgroup CPU
+    quota and by OMP_NUM_THREADS when set; plus the CPU model."""
+    n = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(per)))
+    except (OSError, ValueError):
+        pass
+    threads = n if quota is None else min(n, quota)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        threads = min(threads, int(omp))
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"threads": threads, "affinity_cpus": n, "cgroup_quota_cpus": quota, "cpu_model": model}
+
+
+BENCH_CPU_CASE = {"turbulence": "bench_turb128", "blast": "bench_blast128"}
+
+
+def cpu_baseline(workload: str):
+    """The reference solver (oracle/_ref/<bench case>, built from its own
+    sources with -O3 -ffp-contract=off -fopenmp) on a bounded sample of the
+    workload's physics and scheme, on the host share of cores, timed by its
+    own loop clock (FINDDT + RUNGEKUTTA of every step)."""
     from cumc3d_amd import ic
     from oracle.cases import BENCH_CASES
     from oracle.refio import have_ref, run_ref
     import tempfile
-    name = "bench_blast128"
+    name = BENCH_CPU_CASE.get(workload)
+    hs = host_share()
+    if name is None:
+        return None
     case = BENCH_CASES[name]
-    threads = max(1, min(max_threads, os.cpu_count() or 1))
-    sample = f"reference Fortran (flang -O2 -fopenmp), blast 128^3 HLLD+WENO-Z, {case.steps} steps"
+    c = case.cfg
+    sample = (f"reference Fortran (flang -O3 -ffp-contract=off -fopenmp, {hs['threads']} threads), "
+              f"the workload's physics and scheme at {c.nx}x{c.ny}x{c.nz} ({c.riemann.upper()} + "
+              f"{ {'weno': 'WENO-Z', 'ppm': 'PPM', 'tvdmc': 'TVD-MC'}.get(c.recon, c.recon)}), "
+              f"{case.steps} steps")
+    base = {"unit": "zone-updates/s", "cores": hs["threads"], "kind": "reference", "cpu_model": hs["cpu_model"],
+            "host_affinity_cpus": hs["affinity_cpus"], "host_cgroup_quota_cpus": hs["cgroup_quota_cpus"]}
     if not have_ref(name):
-        return {"value": None, "unit": "zone-updates/s", "cores": threads, "kind": "reference",
-                "sample": sample + " (binary not built)"}
-    st = ic.GENERATORS[case.ic](case.cfg)
+        return dict(base, value=None, sample=sample + " (binary not built)")
+    st = ic.GENERATORS[case.ic](c)
     with tempfile.TemporaryDirectory() as d:
         p = os.path.join(d, "ic.bin")
-        ic.write_ic(p, case.cfg, st)
-        ref = run_ref(name, p, case.steps, threads=threads, timeout=900)
-    rate = case.cfg.zones * ref["nstep"] / ref["seconds"]
-    return {"value": rate, "unit": "zone-updates/s", "cores": threads, "kind": "reference",
-            "sample": sample + f", {ref['seconds']:.1f} s loop time"}
+        ic.write_ic(p, c, st)
+        ref = run_ref(name, p, case.steps, threads=hs["threads"], timeout=900)
+    rate = c.zones * ref["nstep"] / ref["seconds"]
+    return dict(base, value=rate, sample=sample + f", {ref['seconds']:.1f} s loop time")
 
 
-def trace_summary(path: str, n: int, workload: str = "ot") -> str:
+def trace_summary(path: str, n: int, workload: str = "turbulence") -> str:
     """Markdown table of a rocprofv3 --kernel-trace CSV: per kernel, the
     dispatches at its largest grid (the n^3 workload; the selector's training
-    run is smaller), with calls, mean/min/max duration and VGPRs, plus the
-    algorithmic bytes and GB/s of the stage kernels (and, for the collapsar,
-    of the SOR kernels: red 28 B and black 32 B per cell, see sor_kernel_bytes)."""
+    run is smaller), with calls, mean/min/max duration and VGPRs, the kernel
+    byte model (kernel_model_bytes) and its GB/s for the stage kernels, and
+    for the update the step's algorithmic bytes per launch (128 / 192 / 192 B
+    per zone in stages 1 / 2 / 3) with their fraction of the 8 TB/s peak (and,
+    for the collapsar, the SOR kernels: red 28 B and black 32 B per cell, see
+    sor_kernel_bytes)."""
     import csv
     from collections import defaultdict
-    from cumc3d_amd.config import orszag_tang_3d
-    cfg = make_workload(workload, n)[0] if workload != "ot" else orszag_tang_3d(n)
+    cfg = make_workload(workload, n)[0]
     groups = defaultdict(list)
     for r in csv.DictReader(open(path)):
         grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
@@ -392,52 +536,101 @@ def trace_summary(path: str, n: int, workload: str = "ot") -> str:
         durs = [d[1] for d in sel]
         rows.append((sum(durs), name, len(sel), gmax, sum(durs) / len(durs), min(durs), max(durs), sel[0][2], sel[0][3]))
     rows.sort(reverse=True)
-    cls = [("k_flux_x<", "flux_x"), ("k_flux_yz<", None), ("k_update<", "update")]
-    out = ["| kernel | calls | grid (threads) | mean us | min us | max us | VGPR | AGPR | algorithmic MB | GB/s |",
-           "|---|---|---|---|---|---|---|---|---|---|"]
+    out = ["| kernel | calls | grid (threads) | mean us | min us | max us | VGPR | AGPR | model MB | model GB/s "
+           "| algorithmic MB | frac of 8 TB/s |",
+           "|---|---|---|---|---|---|---|---|---|---|---|---|"]
     for tot, name, calls, grid, mean, mn, mx, vg, ag in rows:
         if calls < 2 and tot < 100:
             continue
         w = 4 if "<float" in name else 8
-        kb = None
+        kb, ab = None, None
         if "k_flux_x<" in name:
-            kb = kernel_bytes(cfg, "flux_x", w)
+            kb = kernel_model_bytes(cfg, "flux_x", w)
         elif "k_flux_yz<" in name:
-            kb = kernel_bytes(cfg, "flux_y" if name.rstrip(")").split(",")[3].strip().startswith("1") else "flux_z", w)
+            kb = kernel_model_bytes(cfg, "flux_y" if name.rstrip(")").split(",")[3].strip().startswith("1") else "flux_z", w)
         elif "k_update<" in name:
-            kb = kernel_bytes(cfg, "update", w)
+            kb = kernel_model_bytes(cfg, "update", w)
+            stage = int(name.split("k_update<")[1].split(",")[0])
+            if 1 <= stage <= 3:
+                ab = STAGE_BYTES_PER_ZONE[stage - 1] * cfg.zones
         elif "k_sor_red" in name or "k_sor_black" in name:
             kb = sor_kernel_bytes(cfg, "red" if "k_sor_red" in name else "black")
         mb = f"{kb / 1e6:.0f}" if kb else ""
         gbs = f"{kb / (mean * 1e-6) / 1e9:.0f}" if kb else ""
+        amb = f"{ab / 1e6:.0f}" if ab else ""
+        afr = f"{ab / (mean * 1e-6) / (HBM_PEAK_GBS * 1e9):.3f}" if ab else ""
         short = name.replace("cumc3d::", "").replace("void ", "")
         short = short.split("(")[0]
-        out.append(f"| `{short}` | {calls} | {grid} | {mean:.1f} | {mn:.1f} | {mx:.1f} | {vg} | {ag} | {mb} | {gbs} |")
+        out.append(f"| `{short}` | {calls} | {grid} | {mean:.1f} | {mn:.1f} | {mx:.1f} | {vg} | {ag} | {mb} | {gbs} "
+                   f"| {amb} | {afr} |")
     return "\n".join(out)
+
+
+VARIANTS = "fp64,fp32,fp32_sel,fp32_tvdmc,fp32_ppm"
+
+
+def measure(cfg, st, label, args, variants, sel_w, window, kernel_timing=True, repeats=None):
+    """Time each variant (median of `repeats` runs) inside the FP64 window;
+    per-kernel times from one more run with HIP events on."""
+    seg = window - args.warmup
+    if seg < 1:
+        raise SystemExit(f"warmup {args.warmup} leaves no step inside the {window}-step finite window")
+    repeats = max(1, args.repeats if repeats is None else repeats)
+    from cumc3d_amd.solver import ENONFINITE, CumcError
+    res, failed = {}, {}
+    for var in variants:
+        s = make_variant(cfg, st, var, sel_w)
+        walls = []
+        try:
+            for r in range(repeats):
+                print(f"[bench] {label}: {var} run {r + 1}: warmup {args.warmup}, timed {args.steps} "
+                      f"in segments {segments(args.steps, seg)}", file=sys.stderr, flush=True)
+                walls.append(time_variant(s, st, var, args.warmup, args.steps, seg)[0])
+                print(f"[bench] {var}: {1e3 * walls[-1] / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+            kt = None
+            if kernel_timing:
+                _, kt = time_variant(s, st, var, args.warmup, min(args.steps, seg), seg, kernel_timing=True)
+        except CumcError as e:
+            # a variant whose state goes non-finite inside the window is reported, not timed
+            if e.code != ENONFINITE:
+                raise
+            print(f"[bench] {var}: non-finite inside the window: {e}", file=sys.stderr, flush=True)
+            failed[var] = str(e)
+            del s
+            continue
+        frac = selector_fractions(s, cfg) if var == "fp32_sel" else None
+        del s
+        walls.sort()
+        res[var] = {"wall": walls[len(walls) // 2], "walls": walls, "kt": kt, "frac": frac}
+    return res, failed
 
 
 def main(argv=None):
     if argv is None and len(sys.argv) > 2 and sys.argv[1] == "--trace-summary":
         n = int(sys.argv[3]) if len(sys.argv) > 3 else 256
-        print(trace_summary(sys.argv[2], n, sys.argv[4] if len(sys.argv) > 4 else "ot"))
+        print(trace_summary(sys.argv[2], n, sys.argv[4] if len(sys.argv) > 4 else "turbulence"))
         return 0
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--repeats", type=int, default=5,
-                    help="timed runs of each variant, each from a fresh solver on the same initial state; "
+                    help="timed runs of each variant, each from the initial state; "
                          "value and ms_per_step are their median")
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--workload", default="ot",
-                    choices=["ot", "blast", "turbulence", "torus_sph", "torus_cyl", "collapsar"])
+    ap.add_argument("--workload", default="turbulence",
+                    choices=["turbulence", "ot", "blast", "torus_sph", "torus_cyl", "collapsar"])
     ap.add_argument("--profile-io", action="store_true",
                     help="also time the profile output path (device pack, copy, HDF5 write)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-l1", action="store_true")
-    ap.add_argument("--train-n", type=int, default=64)
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the secondary line (3D Orszag-Tang at the same size)")
+    ap.add_argument("--train-n", type=int, default=0, help="selector training grid (default n/2)")
     ap.add_argument("--train-iters", type=int, default=600)
-    ap.add_argument("--variants", default="fp64,fp32,fp32_sel")
+    ap.add_argument("--variants", default=VARIANTS)
+    ap.add_argument("--window-cap", type=int, default=200,
+                    help="longest FP64 run probed for the finite window")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="copy 1 GiB device to device first (the known-size kernel that scales the HBM counters)")
     ap.add_argument("--sel-every", type=int, default=8)
@@ -451,53 +644,67 @@ def main(argv=None):
 
     if args.workload == "collapsar":
         return bench_collapsar(args)
-    cfg, st, label = make_workload(args.workload, args.n)
-    zones = cfg.zones
-    print(f"[bench] training the selector on a {args.train_n}^3 run", file=sys.stderr, flush=True)
     global SEL_EVERY
     SEL_EVERY = args.sel_every
     if args.pmc_calibrate:
         calibration_copy()
-    sel_w, sel_info = train_selector(args.train_n, args.workload, args.thr_ppm, args.thr_weno,
-                                     iters=args.train_iters)
-    sel_info["refresh_every"] = SEL_EVERY
-    print(f"[bench] selector: {sel_info}", file=sys.stderr, flush=True)
-    res, walls = {}, {}
-    for var in args.variants.split(","):
-        runs = []
-        for r in range(max(1, args.repeats)):
-            print(f"[bench] {label}: {var} run {r + 1}: warmup {args.warmup}, timed {args.steps}",
-                  file=sys.stderr, flush=True)
-            runs.append(time_variant(cfg, st, var, args.warmup, args.steps, sel_w))
-            print(f"[bench] {var}: {1e3 * runs[-1][0] / args.steps:.2f} ms/step", file=sys.stderr, flush=True)
-        walls[var] = sorted(w for w, _, _ in runs)
-        # the median run's wall time; its kernel timing and selector fractions
-        # are the first run's
-        res[var] = (walls[var][len(runs) // 2], runs[0][1], runs[0][2])
+    cfg, st, label = make_workload(args.workload, args.n)
+    zones = cfg.zones
+    variants = args.variants.split(",")
+    train_n = args.train_n or max(16, args.n // 2)
+
+    print(f"[bench] {label}: FP64 finite window", file=sys.stderr, flush=True)
+    window = finite_window(cfg, st, max(args.window_cap, args.warmup + 1))
+    print(f"[bench] FP64 state finite for {window} steps", file=sys.stderr, flush=True)
+
+    sel_w, sel_info = None, None
+    if "fp32_sel" in variants:
+        print(f"[bench] training the selector on a {train_n}^3 run", file=sys.stderr, flush=True)
+        sel_w, sel_info = train_selector(train_n, args.workload, args.thr_ppm, args.thr_weno, iters=args.train_iters)
+        sel_info["refresh_every"] = SEL_EVERY
+        print(f"[bench] selector: {sel_info}", file=sys.stderr, flush=True)
+
+    res, failed = measure(cfg, st, label, args, variants, sel_w, window)
+    if "fp32_sel" in variants and "fp32_sel" not in res:
+        raise SystemExit(f"fp32_sel went non-finite inside the FP64 window: {failed['fp32_sel']}")
 
     def rate(v):
-        return zones * args.steps / res[v][0]
+        return zones * args.steps / res[v]["wall"]
 
-    main_var = "fp32_sel" if "fp32_sel" in res else list(res)[-1]
-    wall, kt, frac = res[main_var]
+    main_var = "fp32_sel" if "fp32_sel" in res else variants[-1]
+    wall, kt = res[main_var]["wall"], res[main_var]["kt"]
     value = rate(main_var)
     dom = max(KERNELS, key=lambda k: kt[k])
-    achieved = kernel_bytes(cfg, dom, 8 if main_var == "fp64" else 4) / (kt[dom] * 1e-3) / 1e9
-    traffic = measured_traffic(dom, 8 if main_var == "fp64" else 4, args.n) if args.workload == "ot" else None
+    w = 8 if main_var == "fp64" else 4
+    # the update runs once per stage: the step's algorithmic bytes over its three launches
+    alg = STEP_BYTES_PER_ZONE * zones / 3.0 if dom == "update" else None
+    achieved = alg / (kt[dom] * 1e-3) / 1e9 if alg else None
+    traffic = measured_traffic(dom, w, args.workload, args.n)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                "algorithmic_bytes": kernel_bytes(cfg, dom, 8 if main_var == "fp64" else 4),
-                "traffic_source": "profiles/traffic_ot256_hlld.md (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
-                if traffic else None,
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                "kernel": f"k_update (fused update, {'FP32 split' if w == 4 else 'FP64'}), mean of its 3 stage launches"
+                if dom == "update" else dom,
+                "algorithmic_bytes": alg,
+                "algorithmic_basis": "512 B per zone-update per RK3 step (128/192/192 B per zone in stages 1/2/3), "
+                                     "averaged over the update's 3 launches",
                 "kernel_ms": kt[dom],
-                "kernel_ms_all": {v: {k: res[v][1][k] for k in KERNELS + ["selector"]} for v in res},
-                "step_bytes_frac": 512.0 * value / (HBM_PEAK_GBS * 1e9)}
+                "kernel_model_bytes": kernel_model_bytes(cfg, dom, w),
+                "kernel_model_frac": kernel_model_bytes(cfg, dom, w) / (kt[dom] * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                "traffic_source": os.path.relpath(traffic_summary(args.workload, args.n), ROOT)
+                + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, per launch)" if traffic else None,
+                "kernel_ms_all": {v: {k: res[v]["kt"][k] for k in KERNELS + ["selector"]} for v in res if res[v]["kt"]},
+                "step_bytes_frac": STEP_BYTES_PER_ZONE * value / (HBM_PEAK_GBS * 1e9)}
 
     err = None
     if not args.no_l1:
-        nacc = args.warmup + 2 + args.steps
-        print(f"[bench] L1 against FP64 after {nacc} steps", file=sys.stderr, flush=True)
-        err = l1_errors(cfg, st, nacc, sel_w)
+        print(f"[bench] L1 against FP64 after {window} steps (the last finite FP64 state)", file=sys.stderr,
+              flush=True)
+        err = l1_errors(cfg, st, window, variants, sel_w)
+
+    secondary = None
+    if not args.no_secondary and args.workload == "turbulence":
+        print("[bench] secondary: Orszag-Tang", file=sys.stderr, flush=True)
+        secondary = secondary_line(args)
 
     io = None
     if args.profile_io:
@@ -508,30 +715,53 @@ def main(argv=None):
     if not args.no_cpu:
         print("[bench] cpu baseline (reference Fortran)", file=sys.stderr, flush=True)
         try:
-            cpu = cpu_baseline()
+            cpu = cpu_baseline(args.workload)
         except Exception as e:  # report, do not hide, a failed baseline
             cpu = {"value": None, "error": str(e)[:200]}
 
+    seg = window - args.warmup
     line = {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64 (bf16 selector)",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64 (bf16 selector MLP)",
         "data": f"synthetic (seeded IC, cumc3d_amd.ic.{ {'ot': 'orszag_tang', 'torus_cyl': 'torus'}.get(args.workload, args.workload) })",
         "config": {"workload": label + ", FP32 split + learned selector (FP64 accumulators)", "zones": zones,
-                   "riemann": cfg.riemann, "recon": "selector: TVD-MC / PPM / WENO-Z per cell"},
-        "variants": {v: {"value": rate(v), "ms_per_step": 1e3 * res[v][0] / args.steps,
-                         "runs_ms_per_step": [round(1e3 * w / args.steps, 4) for w in walls[v]]} for v in res},
+                   "riemann": cfg.riemann, "recon": "selector: TVD-MC / PPM / WENO-Z per cell and direction",
+                   "fp64_finite_steps": window,
+                   "timing": f"each timed run starts from the initial state: {args.warmup} warmup steps, then "
+                             f"{args.steps} timed steps in segments {segments(args.steps, seg)} (the FP64 state "
+                             f"is finite for {window} steps)"},
+        "variants": {v: {"value": rate(v), "ms_per_step": 1e3 * res[v]["wall"] / args.steps,
+                         "runs_ms_per_step": [round(1e3 * x / args.steps, 4) for x in res[v]["walls"]]} for v in res},
+        "variants_failed": failed or None,
         "repeats": max(1, args.repeats),
         "speedup_vs_fp64": value / rate("fp64") if "fp64" in res else None,
-        "selector": dict(sel_info, class_fractions=frac),
+        "selector": dict(sel_info, class_fractions=res["fp32_sel"]["frac"]) if sel_info else None,
         "l1_vs_fp64": err,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if secondary is not None:
+        line["secondary"] = secondary
     if io is not None:
         line["output"] = io
     print(json.dumps(line))
     return 0
+
+
+def secondary_line(args):
+    """3D Orszag-Tang at the same size (HLLD + WENO-Z): FP64 and FP32 +
+    selector (trained on a half-size run of the vortex), 2 runs each."""
+    cfg, st, label = make_workload("ot", args.n)
+    window = finite_window(cfg, st, args.warmup + args.steps)
+    sel_w, info = train_selector(max(16, args.n // 4), "ot", args.thr_ppm, args.thr_weno, iters=args.train_iters)
+    res, _ = measure(cfg, st, label, args, ["fp64", "fp32_sel"], sel_w, window, kernel_timing=False, repeats=2)
+    out = {"workload": label, "fp64_finite_steps_probed": window}
+    for v in res:
+        out[v] = {"value": cfg.zones * args.steps / res[v]["wall"], "ms_per_step": 1e3 * res[v]["wall"] / args.steps}
+    out["speedup_vs_fp64"] = out["fp32_sel"]["value"] / out["fp64"]["value"]
+    out["selector_label_counts"] = info["label_counts"]
+    return out
 
 
 if __name__ == "__main__":

@@ -939,7 +939,7 @@ int cumc3d_finalize(cumc3d_ctx *x) {
     if (x->ev_m) hipEventDestroy(x->ev_m);
     if (x->side) hipStreamDestroy(x->side);
     void *sel[] = {x->sel_w, x->sel_m, x->sel_v, x->sel_frag, x->mask, x->samp_feat, x->samp_label,
-                   x->samp_counts, x->grad_part, x->loss_part, x->loss_dev};
+                   x->samp_owner, x->samp_counts, x->grad_part, x->loss_part, x->loss_dev};
     for (void *q : sel)
         if (q) hipFree(q);
     if (x->stream) hipStreamDestroy(x->stream);
@@ -971,8 +971,14 @@ int cumc3d_selector_init(cumc3d_ctx *x, int capacity, unsigned seed) {
     if (x->samp_cap != capacity) {
         if (x->samp_feat) HIPCHK(hipFree(x->samp_feat));
         if (x->samp_label) HIPCHK(hipFree(x->samp_label));
+        if (x->samp_owner) HIPCHK(hipFree(x->samp_owner));
+        x->samp_feat = nullptr;
+        x->samp_label = nullptr;
+        x->samp_owner = nullptr;
+        x->samp_cap = 0;
         HIPCHK(hipMalloc(&x->samp_feat, (size_t)capacity * SEL_IN * sizeof(float)));
         HIPCHK(hipMalloc(&x->samp_label, (size_t)capacity));
+        HIPCHK(hipMalloc(&x->samp_owner, (size_t)capacity * sizeof(int)));
         x->samp_cap = capacity;
     }
     // He-uniform initial weights from a fixed-seed generator, zero biases
@@ -990,6 +996,7 @@ int cumc3d_selector_init(cumc3d_ctx *x, int capacity, unsigned seed) {
     HIPCHK(hipMemcpy(x->sel_m, zero.data(), SEL_NPARAM * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(x->sel_v, zero.data(), SEL_NPARAM * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(x->samp_counts, 0, 4 * sizeof(int)));
+    x->samp_seen = 0;
     HIPCHK(hipMemset(x->mask, 2, (size_t)3 * x->G));   // WENO-Z until the first inference
     x->adam_step = 0;
     if (int rc = sel_pack(x)) return rc;
@@ -1028,13 +1035,11 @@ int cumc3d_selector_collect(cumc3d_ctx *x, int stride, double thr_ppm, double th
     if (x->fp32) f32::sel_collect(x, stride, seed, (float)thr_ppm, (float)thr_weno);
     else sel_collect_t<double>(x, stride, seed, (float)thr_ppm, (float)thr_weno);
     HIPCHK(hipGetLastError());
+    // counts: samples held (at most the capacity; a reservoir over every
+    // candidate offered so far once full) and the class histogram of those held
     int h[4];
     HIPCHK(hipMemcpyAsync(h, x->samp_counts, sizeof(h), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
-    if (h[0] > x->samp_cap) {
-        // the buffer is full: later slots were dropped, the histogram counted them
-        h[0] = x->samp_cap;
-    }
     if (counts) memcpy(counts, h, sizeof(h));
     return 0;
 }

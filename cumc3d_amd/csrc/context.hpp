@@ -39,6 +39,8 @@ struct cumc3d_ctx {
     float *samp_feat = nullptr;
     unsigned char *samp_label = nullptr;
     int samp_cap = 0, *samp_counts = nullptr, adam_step = 0;
+    int *samp_owner = nullptr;              // reservoir slot claims of one collect
+    unsigned long long samp_seen = 0;       // candidates offered since cumc3d_selector_init
     float *grad_part = nullptr, *loss_part = nullptr, *loss_dev = nullptr;
     int grad_blocks = 0;
     int bfac[6][12];
@@ -408,20 +410,42 @@ template <class W> static void sel_infer_t(cumc3d_ctx *x) {
 
 template <class W> static void sel_collect_t(cumc3d_ctx *x, int stride, unsigned seed, float t1, float t2) {
     ArgsT<W> a = make_args<W>(x);
+    const long cap = x->samp_cap;
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(cap)), dim3(256), 0, x->stream, x->samp_owner, cap, 0);
+    long nt[3] = {0, 0, 0};
+    unsigned sd[3];
     for (int D = 0; D < x->d.n_dim; ++D) {
         long nq = (long)(x->nx + (D == 0 ? 2 : 0)) * (x->ny + (D == 1 ? 2 : 0)) * (x->nz + (D == 2 ? 2 : 0));
-        unsigned g = nblk((nq + stride - 1) / stride);
-        unsigned sd = seed * 3u + (unsigned)D;
-        if (D == 0)
-            hipLaunchKernelGGL((k_sel_collect<W, 0>), dim3(g), dim3(256), 0, x->stream, a, stride, sd, t1, t2,
-                               x->samp_feat, x->samp_label, x->samp_cap, x->samp_counts);
-        else if (D == 1)
-            hipLaunchKernelGGL((k_sel_collect<W, 1>), dim3(g), dim3(256), 0, x->stream, a, stride, sd, t1, t2,
-                               x->samp_feat, x->samp_label, x->samp_cap, x->samp_counts);
-        else
-            hipLaunchKernelGGL((k_sel_collect<W, 2>), dim3(g), dim3(256), 0, x->stream, a, stride, sd, t1, t2,
-                               x->samp_feat, x->samp_label, x->samp_cap, x->samp_counts);
+        nt[D] = (nq + stride - 1) / stride;
+        sd[D] = seed * 3u + (unsigned)D;
     }
+    // pass 1 (store or claim), then pass 2 (replace), over the three regions
+    for (int pass = 0; pass < 2; ++pass) {
+        unsigned long long base = x->samp_seen;
+        long id0 = 0;
+        for (int D = 0; D < x->d.n_dim; ++D) {
+            const dim3 g(nblk(nt[D]));
+#define SEL_COLLECT_ARGS a, stride, sd[D], t1, t2, x->samp_feat, x->samp_label, cap, base, id0
+            if (pass == 0) {
+                if (D == 0) hipLaunchKernelGGL((k_sel_collect<W, 0>), g, dim3(256), 0, x->stream, SEL_COLLECT_ARGS, x->samp_owner);
+                else if (D == 1) hipLaunchKernelGGL((k_sel_collect<W, 1>), g, dim3(256), 0, x->stream, SEL_COLLECT_ARGS, x->samp_owner);
+                else hipLaunchKernelGGL((k_sel_collect<W, 2>), g, dim3(256), 0, x->stream, SEL_COLLECT_ARGS, x->samp_owner);
+            } else {
+                if (D == 0) hipLaunchKernelGGL((k_sel_place<W, 0>), g, dim3(256), 0, x->stream, SEL_COLLECT_ARGS, x->samp_owner);
+                else if (D == 1) hipLaunchKernelGGL((k_sel_place<W, 1>), g, dim3(256), 0, x->stream, SEL_COLLECT_ARGS, x->samp_owner);
+                else hipLaunchKernelGGL((k_sel_place<W, 2>), g, dim3(256), 0, x->stream, SEL_COLLECT_ARGS, x->samp_owner);
+            }
+#undef SEL_COLLECT_ARGS
+            base += (unsigned long long)nt[D];
+            id0 += nt[D];
+        }
+    }
+    x->samp_seen += (unsigned long long)(nt[0] + nt[1] + nt[2]);
+    const long held = (long)(x->samp_seen < (unsigned long long)cap ? x->samp_seen : (unsigned long long)cap);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, x->stream, x->samp_counts, 4L, 0);
+    unsigned hb = nblk(held > 0 ? held : 1);
+    if (hb > 1024) hb = 1024;
+    hipLaunchKernelGGL(k_sel_hist, dim3(hb), dim3(256), 0, x->stream, x->samp_label, held, x->samp_counts);
 }
 
 

@@ -305,28 +305,87 @@ __device__ __forceinline__ unsigned sel_hash(unsigned x) {
     return x;
 }
 
-// Append (features, label) of every stride-th cell (from a hashed offset) of
-// the region along D to the sample buffer; counts[0] is the fill level,
-// counts[1..3] the class histogram.
-template <class W, int D>
-__global__ __launch_bounds__(256) void k_sel_collect(ArgsT<W> a, int stride, unsigned seed, float thr_ppm,
-                                                     float thr_weno, float *feat, unsigned char *label, int cap,
-                                                     int *counts) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long nq = sel_count<D>(a);
+// Training samples by reservoir sampling over every candidate offered since
+// cumc3d_selector_init: candidate s (the candidates of earlier collects plus
+// its index in this one, `base + t`) takes slot s while the buffer has room;
+// after that it replaces slot r = hash(s) % (s + 1) when r < cap, so each
+// candidate offered so far is held with probability cap / seen and a late
+// snapshot is represented as well as an early one.  The candidates are every
+// stride-th cell (from a hashed offset) of the region along D.
+__device__ __forceinline__ unsigned long long sel_slot(unsigned long long s, unsigned seed) {
+    const unsigned h = sel_hash((unsigned)s ^ sel_hash((unsigned)(s >> 32) ^ seed));
+    const unsigned long long r = ((unsigned long long)sel_hash(h ^ 0x5bd1e995U) << 32) | h;
+    return r % (s + 1);
+}
+
+template <int D, class A> __device__ __forceinline__ long sel_candidate(const A &a, long t, int stride, unsigned seed) {
     const long q = t * stride + (long)(sel_hash(seed ^ (unsigned)t) % (unsigned)stride);
-    if (q >= nq) return;
+    return q < sel_count<D>(a) ? q : -1;
+}
+
+template <class W, int D>
+__device__ __forceinline__ void sel_store_sample(const ArgsT<W> &a, long q, float thr_ppm, float thr_weno, float *feat,
+                                                 unsigned char *label, unsigned long long slot) {
     int i;
     const long c = sel_cell<D>(a, q, i);
     const long st = D == 0 ? 1L : (D == 1 ? a.NX : a.NX * a.NY);
-    const int lab = sel_label<W, D>(a, c, i, st, thr_ppm, thr_weno);
-    const int slot = atomicAdd(&counts[0], 1);
-    if (slot >= cap) return;
     float f[SEL_IN];
     sel_features(a, c, st, f);
-    for (int e = 0; e < SEL_IN; ++e) feat[(long)slot * SEL_IN + e] = f[e];
-    label[slot] = (unsigned char)lab;
-    atomicAdd(&counts[1 + lab], 1);
+    for (int e = 0; e < SEL_IN; ++e) feat[slot * SEL_IN + e] = f[e];
+    label[slot] = (unsigned char)sel_label<W, D>(a, c, i, st, thr_ppm, thr_weno);
+}
+
+// Pass 1: candidates below cap store their sample; later ones claim their
+// slot in owner[] (the largest candidate index of this collect wins, id =
+// index within the collect + 1; owner starts at 0).
+template <class W, int D>
+__global__ __launch_bounds__(256) void k_sel_collect(ArgsT<W> a, int stride, unsigned seed, float thr_ppm,
+                                                     float thr_weno, float *feat, unsigned char *label, long cap,
+                                                     unsigned long long base, long id0, int *owner) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long q = sel_candidate<D>(a, t, stride, seed);
+    if (q < 0) return;
+    const unsigned long long s = base + (unsigned long long)t;
+    if (s < (unsigned long long)cap) {
+        sel_store_sample<W, D>(a, q, thr_ppm, thr_weno, feat, label, s);
+        return;
+    }
+    const unsigned long long r = sel_slot(s, seed);
+    if (r < (unsigned long long)cap) atomicMax(&owner[r], (int)(id0 + t + 1));
+}
+
+// Pass 2: a candidate past cap that owns its slot replaces the sample there.
+template <class W, int D>
+__global__ __launch_bounds__(256) void k_sel_place(ArgsT<W> a, int stride, unsigned seed, float thr_ppm,
+                                                   float thr_weno, float *feat, unsigned char *label, long cap,
+                                                   unsigned long long base, long id0, const int *owner) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long q = sel_candidate<D>(a, t, stride, seed);
+    if (q < 0) return;
+    const unsigned long long s = base + (unsigned long long)t;
+    if (s < (unsigned long long)cap) return;
+    const unsigned long long r = sel_slot(s, seed);
+    if (r < (unsigned long long)cap && owner[r] == (int)(id0 + t + 1))
+        sel_store_sample<W, D>(a, q, thr_ppm, thr_weno, feat, label, r);
+}
+
+// counts[0] = samples held, counts[1..3] = their class histogram (counts zeroed before)
+static __global__ __launch_bounds__(256) void k_sel_hist(const unsigned char *__restrict__ label, long n, int *counts) {
+    __shared__ int h[3];
+    if (threadIdx.x < 3) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int c = label[i];
+        if (c < 3) atomicAdd(&h[c], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) atomicAdd(&counts[1 + threadIdx.x], h[threadIdx.x]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) counts[0] = (int)n;
+}
+
+static __global__ void k_fill_i32(int *p, long n, int v) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 constexpr int SEL_TB = 128;   // samples (threads) per training block

@@ -287,7 +287,11 @@ def torus(cfg: Config, seed: int = 20261202, r_in: float = 6.0, r_max: float = 1
     c_const = phi(r_in, 0.0) + l2 / (2.0 * r_in * r_in)
     h_max = c_const - phi(r_max, 0.0) - l2 / (2.0 * r_max * r_max)
     k_poly = h_max * (gam - 1.0) / gam        # rho_max = 1
-    h = c_const - phi(R, Zc) - l2 / (2.0 * R * R)
+    # the enthalpy only where the radius is positive: ghost cells of a grid
+    # starting near the axis can sit at r <= 0, where the potential and the
+    # centrifugal term are infinite (their state is the floor either way)
+    Rp = np.where(R > 0.0, R, 1.0)
+    h = np.where(R > 0.0, c_const - phi(Rp, Zc) - l2 / (2.0 * Rp * Rp), 0.0)
     h = np.maximum(h, 0.0)
     rho_t = (h * (gam - 1.0) / (gam * k_poly)) ** (1.0 / (gam - 1.0))
     inside = (rho_t > rho_floor) & (R >= r_in)
@@ -303,7 +307,8 @@ def torus(cfg: Config, seed: int = 20261202, r_in: float = 6.0, r_max: float = 1
     # A_phi at the (r, z) corners (xF(j), zF(l)); axisymmetric
     xf = g.x.faces[None, :]
     zf = g.z.faces[:, None]
-    hc = np.maximum(c_const - phi(xf, zf) - l2 / (2.0 * xf * xf), 0.0)
+    xfp = np.where(xf > 0.0, xf, 1.0)      # A_phi = 0 on corners at r <= 0 (ghost faces)
+    hc = np.where(xf > 0.0, np.maximum(c_const - phi(xfp, zf) - l2 / (2.0 * xfp * xfp), 0.0), 0.0)
     rc = (hc * (gam - 1.0) / (gam * k_poly)) ** (1.0 / (gam - 1.0))
     A = np.maximum(rc - a_cut, 0.0)           # A[l+3, j+3]
     dz = g.z.d[:, None]

@@ -70,7 +70,14 @@ def gravity_params(cfg: Config, gravity) -> GravityParams:
 
 
 class CumcError(RuntimeError):
-    pass
+    """A non-zero return of the library; ``code`` is the CUMC3D_E* value."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
+
+
+ENONFINITE = 5   # CUMC3D_ENONFINITE: a cell's CFL time step was NaN or not positive
 
 
 _lib = None
@@ -127,7 +134,7 @@ def _pf(a):
 def _check(rc: int):
     if rc != 0:
         L = load_library()
-        raise CumcError(f"{L.cumc3d_error_string(rc).decode()}: {L.cumc3d_last_error().decode()}")
+        raise CumcError(f"{L.cumc3d_error_string(rc).decode()}: {L.cumc3d_last_error().decode()}", rc)
 
 
 class Solver:

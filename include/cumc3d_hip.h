@@ -209,10 +209,14 @@ enum { CUMC3D_SEL_NPARAM = 1699 };
 int cumc3d_selector_init(cumc3d_ctx *ctx, int capacity, unsigned seed);
 int cumc3d_selector_set_params(cumc3d_ctx *ctx, const float *w, int n);
 int cumc3d_selector_get_params(cumc3d_ctx *ctx, float *w, int n);
-/* Append training samples from the current state: every stride-th cell of each
+/* Offer training samples from the current state: every stride-th cell of each
  * sweep region, labelled 2 (WENO-Z) where the relative gap between TVD-MC and
  * WENO-Z faces of rho or p exceeds thr_weno, 1 (PPM) above thr_ppm, else 0.
- * counts (may be NULL) receives {samples held, class-0, class-1, class-2 seen}. */
+ * The buffer is a reservoir over every candidate offered since
+ * cumc3d_selector_init: once full, each new candidate replaces a random held
+ * sample with probability capacity / offered, so later snapshots are held as
+ * well as earlier ones.  counts (may be NULL) receives {samples held, and the
+ * class-0, class-1, class-2 counts of the samples held}. */
 int cumc3d_selector_collect(cumc3d_ctx *ctx, int stride, double thr_ppm, double thr_weno, unsigned seed,
                             int *counts);
 /* iters Adam steps on minibatches of blocks*128 samples, class-weighted

@@ -10,7 +10,9 @@ stores in tests/golden/large_<case>.json:
   * per field (rho, vx, vy, vz, p, bx, by, bz, eps, bcx, bcy, bcz), on cells
     0..n+1 of every axis as C-order float64: the SHA-256 of the bytes, and the
     sum over each z plane (hex floats, to locate a difference);
-  * the same digests of l_rk from one more SPATIAL on the final state.
+  * the same digests of l_rk from one more SPATIAL on the final state, over
+    the elements the library forms (conftest.lrk_mask: l_rk(1:5) on cells
+    1..n, l_rk(6:8) on the faces the update advances), flattened.
 The arrays themselves (hundreds of MB at 128^3) are not stored.
 
 usage: python tests/golden/make_large.py [case ...]
@@ -26,7 +28,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
 
+from conftest import lrk_mask  # noqa: E402
 from cumc3d_amd import ic  # noqa: E402
 from oracle.cases import LARGE_CASES  # noqa: E402
 from oracle.refio import have_ref, run_ref  # noqa: E402
@@ -40,9 +44,11 @@ def window(cfg):
 
 
 def digest(a: np.ndarray) -> dict:
-    """SHA-256 of the C-order float64 bytes and the sum over each leading-axis plane."""
+    """SHA-256 of the C-order float64 bytes and the sum over each leading-axis
+    plane (of blocks of 4096 elements for a flattened array)."""
     a = np.ascontiguousarray(a, dtype=np.float64)
-    planes = [float(p.sum()).hex() for p in a]
+    parts = a if a.ndim > 1 else [a[i:i + 4096] for i in range(0, a.size, 4096)]
+    planes = [float(p.sum()).hex() for p in parts]
     return {"sha256": hashlib.sha256(a.tobytes()).hexdigest(), "plane_sums": planes}
 
 
@@ -61,7 +67,7 @@ def main(names):
         out = {"case": name, "steps": int(ref["nstep"]), "dts": [float(v).hex() for v in ref["dts"]],
                "t": float(ref["t"]).hex(), "maxdivb": float(ref["maxdivb"]).hex(),
                "fields": {f: digest(ref[f][w]) for f in FIELDS},
-               "lrk": {k: digest(v[w]) for k, v in ref["lrk"].items()},
+               "lrk": {k: digest(v[w][lrk_mask(case.cfg, int(k[3:]))]) for k, v in ref["lrk"].items()},
                "finite": bool(all(np.all(np.isfinite(ref[f][w])) for f in FIELDS)),
                "reference_seconds": float(ref["seconds"])}
         path = os.path.join(HERE, f"large_{name}.json")
