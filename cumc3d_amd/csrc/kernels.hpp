@@ -357,11 +357,18 @@ __device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v
 // 64-lane wave reconstructs 64 consecutive cells and solves the 63 interfaces
 // between them, taking the right neighbour's left face with a lane shuffle.
 // Waves overlap by one cell, so every cell is reconstructed about 64/63 times.
+// Blocks take the rows from the last to the first: the first blocks hold the
+// largest serial keys of the HLLD carry, and a later block whose key does not
+// exceed the value already folded skips its atomic.  (In order, nearly every
+// one of the ~68k blocks of a 256^3 sweep with many double-star interfaces
+// raised the maximum, and the atomics on the one address serialised: FP32
+// TVD-MC x sweep 0.79 ms against 0.39 ms for the z sweep on turbulence.)
 template <class T, int R, int S>
 __global__ __launch_bounds__(256) void k_flux_x(ArgsT<T> a) {
     if (!a.S->active) return;
     const int lane = threadIdx.x & 63;
-    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+    const long wave = nwaves - 1 - (((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const long g = wave * 63 + lane;
     const long RL = a.nx + 2;
     const long gtot = RL * a.ny * a.nz;
@@ -398,7 +405,9 @@ __global__ __launch_bounds__(256) void k_flux_x(ArgsT<T> a) {
     if constexpr (S == 3) {
         if (a.bn_mode == 2) {
             key = block_max_u64(key);
-            if (threadIdx.x == 0 && key) atomicMax(&a.S->carry_key, key);
+            if (threadIdx.x == 0 && key &&
+                key > __hip_atomic_load(&a.S->carry_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(&a.S->carry_key, key);
         }
     }
 }
