@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -52,6 +53,38 @@ struct EvPair {
         if (e_ != hipSuccess)                                                             \
             return fail(CUMC3D_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));     \
     } while (0)
+
+// ---------------------------------------------------------------------------
+// Allocation-free time loop (cumc3d_run_noalloc): its per-step dt log lives in
+// this file-scope device array instead of a per-call hipMalloc.  Each GPU the
+// library runs on holds its own copy until the process ends (512 KiB).  Host
+// code never passes `scratch` itself to a kernel or a copy (on the host the
+// name is only a stand-in for the symbol): cumc3d_init resolves the array's
+// device address once per device, with that device current, and every use
+// goes through that address.  All calls on one GPU share the array and it
+// keeps the last call's contents; every slot a call reads back was written
+// by that call, and calls that overlap on one GPU race on it.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[CUMC3D_NOALLOC_MAX_STEPS];
+constexpr int MAX_DEVICES = 64;
+static double *g_scratch[MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// The current device's address of `scratch`, resolved on first use per device.
+static int scratch_address(double **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(CUMC3D_EHIP, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(CUMC3D_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch[dev] = (double *)p;
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
 
 // dt = MIN(1e5, cell minimum), clipped to total_time (main.f90:384-389); a
 // step that starts at or past total_time gets dt = 0 and is a no-op in time.
@@ -526,6 +559,7 @@ int cumc3d_init(const cumc3d_desc *desc, cumc3d_ctx **out) {
         delete x;
         return fail(CUMC3D_EHIP, "hipStreamCreate failed");
     }
+    if (int e = scratch_address(&x->scratch)) { cleanup(); return e; }
     if (hipMalloc(&x->met, hm.size() * sizeof(double)) != hipSuccess) { cleanup(); return fail(CUMC3D_ENOMEM, "metrics"); }
     if (hipMemcpy(x->met, hm.data(), hm.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
         cleanup();
@@ -760,14 +794,11 @@ int cumc3d_spatial(cumc3d_ctx *x, double *l_rk) {
     return 0;
 }
 
-int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_step, double *t, double *dts,
-               int *steps_done) {
-    if (!x || !t) return fail(CUMC3D_EINVAL, "null argument");
-    if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
-    if (x->next_stage != 1) return fail(CUMC3D_ESTATE, "a step run by cumc3d_stage is not finished");
-    DevTmp dbuf;
-    if (dts && max_steps > 0) HIPCHK(hipMalloc(&dbuf.p, (size_t)max_steps * sizeof(double)));
-    double *dlog = (double *)dbuf.p;
+// The time loop of cumc3d_run and cumc3d_run_noalloc: dlog (device, at
+// least max_steps doubles, or null when the caller wants no dt log) receives
+// each step's dt.
+static int run_loop(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_step, double *t, double *dts,
+                    int *steps_done, double *dlog) {
     // set t and total_time
     DevScalars h;
     HIPCHK(hipMemcpy(&h, x->S, sizeof(h), hipMemcpyDeviceToHost));
@@ -835,6 +866,30 @@ int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_st
     }
     if (!std::isfinite(h.dt) || h.dt < 0.0) return fail(CUMC3D_ENONFINITE, "time step became non-finite");
     return grav_check(x);
+}
+
+int cumc3d_run(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_step, double *t, double *dts,
+               int *steps_done) {
+    if (!x || !t) return fail(CUMC3D_EINVAL, "null argument");
+    if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
+    if (x->next_stage != 1) return fail(CUMC3D_ESTATE, "a step run by cumc3d_stage is not finished");
+    DevTmp dbuf;
+    if (dts && max_steps > 0) HIPCHK(hipMalloc(&dbuf.p, (size_t)max_steps * sizeof(double)));
+    return run_loop(x, max_steps, total_time, sync_each_step, t, dts, steps_done, (double *)dbuf.p);
+}
+
+int cumc3d_run_noalloc(cumc3d_ctx *x, int max_steps, double total_time, int sync_each_step, double *t,
+                       double *dts, int *steps_done) {
+    if (!x || !t) return fail(CUMC3D_EINVAL, "null argument");
+    if (!x->initialized) return fail(CUMC3D_ESTATE, "call cumc3d_initial_update first");
+    if (x->next_stage != 1) return fail(CUMC3D_ESTATE, "a step run by cumc3d_stage is not finished");
+    if (dts && max_steps > CUMC3D_NOALLOC_MAX_STEPS)
+        return fail(CUMC3D_EINVAL, "cumc3d_run_noalloc logs at most CUMC3D_NOALLOC_MAX_STEPS (65536) dts per call");
+    if (x->timing) return fail(CUMC3D_ESTATE, "cumc3d_run_noalloc does not time kernels (the events allocate): "
+                                              "turn timing off with cumc3d_set_timing");
+    if (!x->scratch) return fail(CUMC3D_ESTATE, "the scratch array's device address was not resolved");
+    return run_loop(x, max_steps, total_time, sync_each_step, t, dts, steps_done,
+                    dts && max_steps > 0 ? x->scratch : nullptr);
 }
 
 int cumc3d_download(cumc3d_ctx *x, double *prim, double *eps, double *bcell, double *cs) {

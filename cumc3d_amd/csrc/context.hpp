@@ -40,6 +40,7 @@ struct cumc3d_ctx {
     unsigned char *samp_label = nullptr;
     int samp_cap = 0, *samp_counts = nullptr, adam_step = 0;
     int *samp_owner = nullptr;              // reservoir slot claims of one collect
+    double *scratch = nullptr;              // this device's address of capi.hip's scratch array
     unsigned long long samp_seen = 0;       // candidates offered since cumc3d_selector_init
     float *grad_part = nullptr, *loss_part = nullptr, *loss_dev = nullptr;
     int grad_blocks = 0;

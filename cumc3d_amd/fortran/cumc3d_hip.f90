@@ -91,6 +91,18 @@ INTERFACE
     REAL(C_DOUBLE), INTENT(OUT), OPTIONAL :: dts(*)
     INTEGER(C_INT), INTENT(OUT) :: steps_done
   END FUNCTION
+  ! the same loop without device allocation (no reference counterpart: for
+  ! callers that cannot allocate; dts holds at most 65536 entries)
+  INTEGER(C_INT) FUNCTION cumc3d_run_noalloc(ctx, max_steps, total_time, sync_each_step, t, dts, steps_done) &
+      BIND(C, NAME='cumc3d_run_noalloc')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR), VALUE :: ctx
+    INTEGER(C_INT), VALUE :: max_steps, sync_each_step
+    REAL(C_DOUBLE), VALUE :: total_time
+    REAL(C_DOUBLE), INTENT(INOUT) :: t
+    REAL(C_DOUBLE), INTENT(OUT), OPTIONAL :: dts(*)
+    INTEGER(C_INT), INTENT(OUT) :: steps_done
+  END FUNCTION
   INTEGER(C_INT) FUNCTION cumc3d_download(ctx, prim, eps, bcell, cs) BIND(C, NAME='cumc3d_download')
     IMPORT :: C_INT, C_PTR, C_DOUBLE
     TYPE(C_PTR), VALUE :: ctx

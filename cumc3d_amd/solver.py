@@ -26,7 +26,8 @@ PD = C.POINTER(C.c_double)
 
 # symbols declared in include/cumc3d_hip.h
 EXPORTS = ("cumc3d_init", "cumc3d_upload", "cumc3d_initial_update", "cumc3d_finddt",
-           "cumc3d_rk3_step", "cumc3d_stage", "cumc3d_spatial", "cumc3d_run", "cumc3d_download", "cumc3d_profile", "cumc3d_gravity",
+           "cumc3d_rk3_step", "cumc3d_stage", "cumc3d_spatial", "cumc3d_run", "cumc3d_run_noalloc",
+           "cumc3d_download", "cumc3d_profile", "cumc3d_gravity",
            "cumc3d_divb",
            "cumc3d_finalize", "cumc3d_error_string", "cumc3d_last_error",
            "cumc3d_set_timing", "cumc3d_get_timing",
@@ -100,6 +101,7 @@ def load_library(path: str = LIB_PATH):
     L.cumc3d_stage.argtypes = [vp, C.c_int, C.c_double]
     L.cumc3d_spatial.argtypes = [vp, PD]
     L.cumc3d_run.argtypes = [vp, C.c_int, C.c_double, C.c_int, PD, PD, C.POINTER(C.c_int)]
+    L.cumc3d_run_noalloc.argtypes = [vp, C.c_int, C.c_double, C.c_int, PD, PD, C.POINTER(C.c_int)]
     L.cumc3d_download.argtypes = [vp, PD, PD, PD, PD]
     L.cumc3d_profile.argtypes = [vp, PD, PD, PD]
     L.cumc3d_gravity.argtypes = [vp, PD, PD, PD, C.POINTER(C.c_int)]
@@ -244,14 +246,17 @@ class Solver:
         _check(load_library().cumc3d_spatial(self._h, _p(buf)))
         return {f"lrk{i + 1}": np.ascontiguousarray(buf[..., i]) for i in range(8)}
 
-    def run(self, max_steps: int, total_time: float | None = None, sync_each_step: bool = True) -> np.ndarray:
-        """main.f90's loop on the device; returns the dt of every step taken."""
+    def run(self, max_steps: int, total_time: float | None = None, sync_each_step: bool = True,
+            noalloc: bool = False) -> np.ndarray:
+        """main.f90's loop on the device; returns the dt of every step taken.
+        noalloc: through cumc3d_run_noalloc (no device allocation in the call)."""
         T = self.cfg.total_time if total_time is None else total_time
         t = C.c_double(self.t)
         dts = np.zeros(max(max_steps, 1))
         n = C.c_int()
-        _check(load_library().cumc3d_run(self._h, max_steps, T, int(sync_each_step), C.byref(t), _p(dts),
-                                         C.byref(n)))
+        L = load_library()
+        fn = L.cumc3d_run_noalloc if noalloc else L.cumc3d_run
+        _check(fn(self._h, max_steps, T, int(sync_each_step), C.byref(t), _p(dts), C.byref(n)))
         self.t = t.value
         self.nstep += n.value
         return dts[:n.value]

@@ -155,6 +155,23 @@ int cumc3d_spatial(cumc3d_ctx *ctx, double *l_rk);
 int cumc3d_run(cumc3d_ctx *ctx, int max_steps, double total_time, int sync_each_step, double *t,
                double *dts, int *steps_done);
 
+/* The largest max_steps cumc3d_run_noalloc logs dts for. */
+#define CUMC3D_NOALLOC_MAX_STEPS 65536
+
+/* cumc3d_run without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, steps and results as cumc3d_run; the per-step dt log that
+ * cumc3d_run allocates lives in a static device array instead (one per GPU,
+ * 512 KiB, held until the process ends), so dts may receive at most
+ * CUMC3D_NOALLOC_MAX_STEPS entries (CUMC3D_EINVAL beyond that; with dts =
+ * NULL any max_steps runs).  It calls no hipMalloc/hipFree and refuses kernel
+ * timing (cumc3d_set_timing on: CUMC3D_ESTATE), whose events allocate.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (two host
+ * threads) race on it. */
+int cumc3d_run_noalloc(cumc3d_ctx *ctx, int max_steps, double total_time, int sync_each_step, double *t,
+                       double *dts, int *steps_done);
+
 /* Copy the device state back (replaces !$ACC UPDATE HOST, dataio.f90:26).
  * Any pointer may be NULL.  prim and bcell use the AoS layout above. */
 int cumc3d_download(cumc3d_ctx *ctx, double *prim, double *eps, double *bcell, double *cs);
