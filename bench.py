@@ -525,8 +525,28 @@ def trace_summary(path: str, n: int, workload: str = "turbulence") -> str:
     from collections import defaultdict
     cfg = make_workload(workload, n)[0]
     groups = defaultdict(list)
-    for r in csv.DictReader(open(path)):
-        grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+    rows_in = list(csv.DictReader(open(path)))
+
+    def grid_of(r):
+        return int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+
+    # the selector's training run (a smaller grid) launches the y/z march
+    # kernels with the same thread count as the n^3 run: leave out every
+    # dispatch inside the span of the smaller-grid update launches
+    upd = [r for r in rows_in if "k_update<" in r["Kernel_Name"]]
+    gmax_upd = max((grid_of(r) for r in upd), default=0)
+    small = [r for r in upd if grid_of(r) < gmax_upd]
+    span = None
+    if small:
+        s0 = min(int(r["Start_Timestamp"]) for r in small)
+        # from the end of the last full-size update before it (the training
+        # run's first sweeps precede its first update)
+        before = [int(r["End_Timestamp"]) for r in upd if grid_of(r) == gmax_upd and int(r["End_Timestamp"]) < s0]
+        span = (max(before) + 1 if before else 0, max(int(r["End_Timestamp"]) for r in small))
+    for r in rows_in:
+        if span and span[0] <= int(r["Start_Timestamp"]) <= span[1]:
+            continue
+        grid = grid_of(r)
         dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
         groups[r["Kernel_Name"]].append((grid, dur, int(r["VGPR_Count"]), int(r["Accum_VGPR_Count"])))
     rows = []
@@ -563,6 +583,101 @@ def trace_summary(path: str, n: int, workload: str = "turbulence") -> str:
         short = short.split("(")[0]
         out.append(f"| `{short}` | {calls} | {grid} | {mean:.1f} | {mn:.1f} | {mx:.1f} | {vg} | {ag} | {mb} | {gbs} "
                    f"| {amb} | {afr} |")
+    return "\n".join(out)
+
+
+def pmc_summary(paths, n: int, workload: str = "turbulence") -> str:
+    """Markdown table of rocprofv3 --pmc counter collections (one CSV per
+    counter group, all of the same program): per kernel at its largest grid
+    (the selector's smaller training run left out, as in trace_summary), the
+    mean per dispatch of each counter.  FETCH_SIZE / WRITE_SIZE (KiB) become
+    MB through the counters of the run's 1 GiB device copy (bench.py
+    --pmc-calibrate); FP64/FP32 flops become GFLOP/s over the dispatch's own
+    duration; VALUBusy, VALUUtilization and OccupancyPercent are rocprofv3's
+    derived metrics (%)."""
+    import csv
+    from collections import defaultdict
+    cfg = make_workload(workload, n)[0]
+    vals = defaultdict(lambda: defaultdict(list))    # kernel -> counter -> values
+    durs = defaultdict(list)
+    calib = {}
+
+    def grid_of(r):
+        return int(r["Grid_Size"])
+
+    for path in paths:
+        rows = list(csv.DictReader(open(path)))
+        for r in rows:
+            if r["Kernel_Name"].startswith("__amd_rocclr_copyBuffer") and grid_of(r) >= 131072:
+                calib[r["Counter_Name"]] = float(r["Counter_Value"]) / (1 << 20)   # KiB per GiB moved
+        upd = [r for r in rows if "k_update<" in r["Kernel_Name"]]
+        gmax = max((grid_of(r) for r in upd), default=0)
+        small = [r for r in upd if grid_of(r) < gmax]
+        span = None
+        if small:
+            s0 = min(int(r["Start_Timestamp"]) for r in small)
+            before = [int(r["End_Timestamp"]) for r in upd if grid_of(r) == gmax and int(r["End_Timestamp"]) < s0]
+            span = (max(before) + 1 if before else 0, max(int(r["End_Timestamp"]) for r in small))
+        gk = defaultdict(int)
+        for r in rows:
+            gk[r["Kernel_Name"]] = max(gk[r["Kernel_Name"]], grid_of(r))
+        seen = set()
+        for r in rows:
+            if span and span[0] <= int(r["Start_Timestamp"]) <= span[1]:
+                continue
+            name = r["Kernel_Name"]
+            if grid_of(r) != gk[name] or not name.startswith("void cumc3d::k_"):
+                continue
+            vals[name][r["Counter_Name"]].append(float(r["Counter_Value"]))
+            key = (path, r["Dispatch_Id"])
+            if key not in seen:
+                seen.add(key)
+                durs[name].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9)
+    fs, ws = calib.get("FETCH_SIZE", 1.0), calib.get("WRITE_SIZE", 1.0)
+
+    def mean(v):
+        return sum(v) / len(v) if v else None
+
+    out = [f"Calibration (1 GiB device copy): FETCH_SIZE factor {fs:.4f}, WRITE_SIZE factor {ws:.4f} "
+           "(KiB counted per KiB moved).", "",
+           "| kernel | dispatches per run | mean us (PMC run) | fetch MB | write MB | traffic MB | model MB | algorithmic MB "
+           "| VALUBusy % | VALUUtil % | occupancy % | FP64 GFLOP/s | FP32 GFLOP/s | VALU insts (M) | VMEM rd/wr (M) |",
+           "|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    rows_out = []
+    for name, cv in vals.items():
+        short = name.replace("void cumc3d::", "").split("(")[0]
+        d = mean(durs[name]) or 0.0
+        fmb = mean(cv.get("FETCH_SIZE", [])) and mean(cv["FETCH_SIZE"]) * 1024 / fs / 1e6
+        wmb = mean(cv.get("WRITE_SIZE", [])) and mean(cv["WRITE_SIZE"]) * 1024 / ws / 1e6
+        w = 4 if "<float" in name else 8
+        kb, ab = None, None
+        if "k_flux_x<" in name:
+            kb = kernel_model_bytes(cfg, "flux_x", w)
+        elif "k_flux_yz<" in name:
+            kb = kernel_model_bytes(cfg, "flux_y" if name.split("(")[0].rstrip(">").endswith("1") else "flux_z", w)
+        elif "k_update<" in name:
+            kb = kernel_model_bytes(cfg, "update", w)
+            stg = int(name.split("k_update<")[1].split(",")[0])
+            ab = STAGE_BYTES_PER_ZONE[stg - 1] * cfg.zones if 1 <= stg <= 3 else None
+        elif "k_edge<" in name:
+            kb = kernel_model_bytes(cfg, "edge", w)
+        f64 = mean(cv.get("SQ_INSTS_VALU_FLOPS_FP64", []))
+        f32 = mean(cv.get("SQ_INSTS_VALU_FLOPS_FP32", []))
+        vi = mean(cv.get("SQ_INSTS_VALU", []))
+        vr, vw = mean(cv.get("SQ_INSTS_VMEM_RD", [])), mean(cv.get("SQ_INSTS_VMEM_WR", []))
+
+        def f(x, fmt):
+            return format(x, fmt) if x is not None else ""
+        rows_out.append(((fmb or 0) + (wmb or 0), f"| `{short}` | {len(durs[name]) // len(paths)} | {d * 1e6:.1f} | {f(fmb, '.0f')} "
+                         f"| {f(wmb, '.0f')} | {f((fmb or 0) + (wmb or 0) if fmb is not None else None, '.0f')} "
+                         f"| {f(kb and kb / 1e6, '.0f')} | {f(ab and ab / 1e6, '.0f')} "
+                         f"| {f(mean(cv.get('VALUBusy', [])), '.1f')} | {f(mean(cv.get('VALUUtilization', [])), '.1f')} "
+                         f"| {f(mean(cv.get('OccupancyPercent', [])), '.1f')} "
+                         f"| {f(f64 / d / 1e9 if f64 is not None and d else None, '.0f')} "
+                         f"| {f(f32 / d / 1e9 if f32 is not None and d else None, '.0f')} "
+                         f"| {f(vi and vi / 1e6, '.1f')} | {f(vr and vr / 1e6, '.2f')} / {f(vw and vw / 1e6, '.2f')} |"))
+    rows_out.sort(key=lambda t: -t[0])
+    out += [r for _, r in rows_out]
     return "\n".join(out)
 
 
@@ -606,6 +721,10 @@ def measure(cfg, st, label, args, variants, sel_w, window, kernel_timing=True, r
 
 
 def main(argv=None):
+    if argv is None and len(sys.argv) > 2 and sys.argv[1] == "--pmc-summary":
+        # bench.py --pmc-summary N WORKLOAD csv...
+        print(pmc_summary(sys.argv[4:], int(sys.argv[2]), sys.argv[3]))
+        return 0
     if argv is None and len(sys.argv) > 2 and sys.argv[1] == "--trace-summary":
         n = int(sys.argv[3]) if len(sys.argv) > 3 else 256
         print(trace_summary(sys.argv[2], n, sys.argv[4] if len(sys.argv) > 4 else "turbulence"))
@@ -631,6 +750,8 @@ def main(argv=None):
     ap.add_argument("--variants", default=VARIANTS)
     ap.add_argument("--window-cap", type=int, default=200,
                     help="longest FP64 run probed for the finite window")
+    ap.add_argument("--window", type=int, default=0,
+                    help="take the FP64 finite window as given instead of probing it (profiling runs)")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="copy 1 GiB device to device first (the known-size kernel that scales the HBM counters)")
     ap.add_argument("--sel-every", type=int, default=8)
@@ -654,7 +775,7 @@ def main(argv=None):
     train_n = args.train_n or max(16, args.n // 2)
 
     print(f"[bench] {label}: FP64 finite window", file=sys.stderr, flush=True)
-    window = finite_window(cfg, st, max(args.window_cap, args.warmup + 1))
+    window = args.window or finite_window(cfg, st, max(args.window_cap, args.warmup + 1))
     print(f"[bench] FP64 state finite for {window} steps", file=sys.stderr, flush=True)
 
     sel_w, sel_info = None, None
