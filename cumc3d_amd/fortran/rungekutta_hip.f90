@@ -15,6 +15,19 @@
 ! model/torus-cyn, -DCUMC3D_TORUS -DCUMC3D_TORUS_SPH for model/torus-sph,
 ! -DCUMC3D_COLLAPSAR for model/collapsar; link libcumc3d_hip.so.
 !
+! Arithmetic (no counterpart in the reference, whose scheme flags live in
+! parameter.h:91-115): the default is the FP64 mode, bit-exact with the
+! reference.  -DCUMC3D_USE_FP32 selects the FP32 split (single-precision
+! reconstruction, Riemann solver and EMFs, double-precision RK accumulators
+! and energy recovery).  -DCUMC3D_USE_SELECTOR adds the learned reconstruction
+! selector (with -DCUMC3D_USE_FP32, or alone in the FP64 mode): at attach it
+! reads the 1699 single-precision parameters (CUMC3D_SEL_NPARAM, raw
+! little-endian float32, e.g. numpy's tofile of cumc3d_selector_get_params)
+! from the file named by the environment variable CUMC3D_SELECTOR_PARAMS
+! (default ./selector_params.bin) and refreshes the class masks every
+! CUMC3D_SELECTOR_EVERY steps (default 8).  The compile-time reconstruction
+! flag of parameter.h stays the scheme of the plain path.
+!
 ! The collapsar's CUSTOM_DEF values (prim_a, eps_a, m_bh, vel2code) and its
 ! param.h Poisson settings go over in cumc3d_gravity_params; the device runs
 ! CUSTOM_UPDATE itself (accretion, horizon check, the SOR potential), and
@@ -66,7 +79,11 @@ SUBROUTINE CUMC3D_HIP_ATTACH
   IF (ppmc_flag) d%recon = CUMC3D_PPM
   IF (weno_flag) d%recon = CUMC3D_WENO
   d%hlld_bn_mode = CUMC3D_BN_CARRY
+#ifdef CUMC3D_USE_FP32
+  d%precision = CUMC3D_FP32
+#else
   d%precision = CUMC3D_FP64
+#endif
   d%cfl = cfl
   d%ggas = ggas
   d%gravity = C_NULL_PTR
@@ -167,8 +184,37 @@ SUBROUTINE CUMC3D_HIP_ATTACH
   CALL cumc3d_check(cumc3d_upload(hip_ctx, prim, epsilon), 'cumc3d_upload')
 #endif
   CALL cumc3d_check(cumc3d_initial_update(hip_ctx), 'cumc3d_initial_update')
+#ifdef CUMC3D_USE_SELECTOR
+  CALL CUMC3D_HIP_SELECTOR
+#endif
   hip_attached = .true.
 END SUBROUTINE
+
+#ifdef CUMC3D_USE_SELECTOR
+! The learned selector's parameters from CUMC3D_SELECTOR_PARAMS (default
+! ./selector_params.bin, CUMC3D_SEL_NPARAM float32), refreshed every
+! CUMC3D_SELECTOR_EVERY steps (default 8).
+SUBROUTINE CUMC3D_HIP_SELECTOR
+  IMPLICIT NONE
+  INTEGER, PARAMETER :: nparam = 1699
+  REAL(C_FLOAT) :: w(nparam)
+  CHARACTER(len=1024) :: path, val
+  INTEGER :: u, ios, plen, vlen, every
+  CALL GET_ENVIRONMENT_VARIABLE('CUMC3D_SELECTOR_PARAMS', path, plen)
+  IF (plen == 0) path = 'selector_params.bin'
+  every = 8
+  CALL GET_ENVIRONMENT_VARIABLE('CUMC3D_SELECTOR_EVERY', val, vlen)
+  IF (vlen > 0) READ(val, *) every
+  OPEN(NEWUNIT=u, FILE=TRIM(path), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD', ACTION='READ', IOSTAT=ios)
+  IF (ios /= 0) STOP 'cumc3d drop-in: cannot open the selector parameter file (CUMC3D_SELECTOR_PARAMS)'
+  READ(u, IOSTAT=ios) w
+  CLOSE(u)
+  IF (ios /= 0) STOP 'cumc3d drop-in: the selector parameter file holds fewer than 1699 float32 values'
+  CALL cumc3d_check(cumc3d_selector_init(hip_ctx, 128, 1), 'cumc3d_selector_init')
+  CALL cumc3d_check(cumc3d_selector_set_params(hip_ctx, w, nparam), 'cumc3d_selector_set_params')
+  CALL cumc3d_check(cumc3d_selector_enable(hip_ctx, 1, every), 'cumc3d_selector_enable')
+END SUBROUTINE
+#endif
 
 ! prim, epsilon, bcell and cs of the host modules from the device state
 ! (for the collapsar also phi, m_bh and r_sh of CUSTOM_DEF).

@@ -49,6 +49,10 @@ DROPIN_SRC = [os.path.join(REPO, "cumc3d_amd", "fortran", f)
               for f in ("cumc3d_hip.f90", "rungekutta_hip.f90", "cumc3d_io.f90", "dataio_hip.f90")]
 DROPIN_CASES = ("c1_briowu", "ot_hlld_weno", "ot_bc_mixed", "torus_hlld_ppm", "ot2d_hlld_ppm",
                 "sph_torus_ppm", "sph_blast_weno", "collapsar2d_ppm", "collapsar_ppm")
+# drop-in variants with the arithmetic switches of rungekutta_hip.f90:
+# oracle/_ref/dropin_<variant>/<case>/cumc3d_dropin
+DROPIN_VARIANTS = {"fp32": ["-DCUMC3D_USE_FP32"], "fp32sel": ["-DCUMC3D_USE_FP32", "-DCUMC3D_USE_SELECTOR"]}
+DROPIN_VARIANT_CASES = ("ot_hlld_weno", "c4_turb24_weno")
 FFLAGS = ["-cpp", "-O2", "-ffp-contract=off", "-fopenmp", "-DDIVB", "-DFIXPOLE"]
 FFLAGS_BENCH = ["-cpp", "-O3", "-ffp-contract=off", "-fopenmp", "-DDIVB", "-DFIXPOLE"]
 
@@ -205,25 +209,30 @@ def build_case(name: str) -> str:
     libdir = os.path.join(os.path.dirname(os.path.dirname(FLANG)), "lib")
     run([FLANG, "-fopenmp", "-o", os.path.join(outdir, "cumc3d_ref"), *objs,
          f"-Wl,-rpath,{libdir}"], work)
-    if name in DROPIN_CASES and all(os.path.exists(os.path.join(HIPLIB_DIR, f))
-                                    for f in ("libcumc3d_hip.so", "libcumc3d_io.so")):
-        build_dropin(name, cfg, work, objs, libdir, dflags)
+    if all(os.path.exists(os.path.join(HIPLIB_DIR, f)) for f in ("libcumc3d_hip.so", "libcumc3d_io.so")):
+        if name in DROPIN_CASES:
+            build_dropin(name, cfg, work, objs, libdir, dflags)
+        if name in DROPIN_VARIANT_CASES:
+            for v, vflags in DROPIN_VARIANTS.items():
+                build_dropin(name, cfg, work, objs, libdir, dflags, vflags, "dropin_" + v)
     return outdir
 
 
-def build_dropin(name, cfg, work, objs, libdir, dflags=()):
-    """Link the reference program with the Fortran drop-in instead of rungekutta.f90."""
+def build_dropin(name, cfg, work, objs, libdir, dflags=(), vflags=(), subdir="dropin"):
+    """Link the reference program with the Fortran drop-in instead of rungekutta.f90
+    (vflags: the drop-in's arithmetic switches, into oracle/_ref/<subdir>/<case>)."""
     extra = {"torus": ["-DCUMC3D_TORUS"], "torus_sph": ["-DCUMC3D_TORUS", "-DCUMC3D_TORUS_SPH"],
-             "collapsar": ["-DCUMC3D_COLLAPSAR"]}.get(cfg.model, [])
+             "collapsar": ["-DCUMC3D_COLLAPSAR"]}.get(cfg.model, []) + list(vflags)
+    tag = subdir + "_"
     dobjs = [o for o in objs if o not in ("rungekutta.o", "driver.o")]
     for src in DROPIN_SRC:
-        obj = "dropin_" + os.path.basename(src).replace(".f90", ".o")
+        obj = tag + os.path.basename(src).replace(".f90", ".o")
         run([FLANG, *FFLAGS, *extra, "-c", src, "-o", obj], work)
         dobjs.append(obj)
     run([FLANG, *FFLAGS, "-DHIPDROP", *dflags, "-c", os.path.join(HERE, "ref_harness", "driver.f90"),
          "-o", "driver_hipdrop.o"], work)
     dobjs.append("driver_hipdrop.o")
-    outdir = os.path.join(OUT, "dropin", name)
+    outdir = os.path.join(OUT, subdir, name)
     os.makedirs(outdir, exist_ok=True)
     # the library is found relative to the binary, so the tree can move
     run([FLANG, "-fopenmp", "-o", os.path.join(outdir, "cumc3d_dropin"), *dobjs,

@@ -66,9 +66,11 @@ def read_grid(path: str) -> dict:
     return out
 
 
-def dropin_binary(case: str) -> str:
-    """The reference program stepping through the Fortran drop-in on the GPU."""
-    return os.path.join(HERE, "_ref", "dropin", case, "cumc3d_dropin")
+def dropin_binary(case: str, variant: str = "") -> str:
+    """The reference program stepping through the Fortran drop-in on the GPU
+    (variant "fp32" / "fp32sel": built with the drop-in's -DCUMC3D_USE_FP32 /
+    -DCUMC3D_USE_FP32 -DCUMC3D_USE_SELECTOR)."""
+    return os.path.join(HERE, "_ref", "dropin_" + variant if variant else "dropin", case, "cumc3d_dropin")
 
 
 def write_grid_files(d: str, cfg) -> None:
@@ -92,7 +94,7 @@ def read_grav(path: str, shape) -> dict:
 
 
 def run_ref(case: str, ic_path: str, steps: int, threads: int = 1, timeout: float = 600,
-            binary: str | None = None) -> dict:
+            binary: str | None = None, extra_env: dict | None = None) -> dict:
     ic_path = os.path.abspath(ic_path)
     try:
         from oracle.cases import ALL_CASES
@@ -109,6 +111,7 @@ def run_ref(case: str, ic_path: str, steps: int, threads: int = 1, timeout: floa
         # Fresh heap memory filled with a fixed byte pattern, so that nothing
         # the program reads before writing can differ between runs.
         env["MALLOC_PERTURB_"] = "255"
+        env.update(extra_env or {})
         r = subprocess.run([os.path.abspath(binary or ref_binary(case)), ic_path, out, str(steps), str(threads)],
                            capture_output=True, text=True, timeout=timeout, env=env, cwd=d)
         if r.returncode != 0:

@@ -58,3 +58,84 @@ def test_dropin_matches_reference(name):
     o = O.Oracle(case.cfg, st)         # the state after INITIAL_UPDATE
     check_profile(got["profiles"]["rkiter-0-nm.hdf5"], case.cfg, {f: o.get(f) for f in FIELDS[:9]}, 0.0)
 
+
+
+# ---------------------------------------------------------------------------
+# the drop-in's arithmetic switches: -DCUMC3D_USE_FP32 (the FP32 split) and
+# -DCUMC3D_USE_SELECTOR (the learned selector, parameters from the file named
+# by CUMC3D_SELECTOR_PARAMS, refreshed every CUMC3D_SELECTOR_EVERY steps)
+# ---------------------------------------------------------------------------
+from oracle.build_ref import DROPIN_VARIANT_CASES  # noqa: E402
+
+
+def python_run(case, st, fp32=True, sel_w=None, every=8):
+    from cumc3d_amd.solver import make_solver
+    s = make_solver(case.cfg.with_(precision="fp32" if fp32 else "fp64"), st)
+    if sel_w is not None:
+        s.selector_init(capacity=128, seed=1)
+        s.selector_params(sel_w)
+        s.selector_enable(True, every=every)
+    dts = s.run(case.steps)
+    return dts, s.state()
+
+
+@pytest.mark.parametrize("name", DROPIN_VARIANT_CASES)
+def test_dropin_fp32_switch(name):
+    """-DCUMC3D_USE_FP32: the reference's own driver steps the FP32 split.  The
+    result is bit for bit the Python mirror's FP32 run of the same case (the
+    same library and step sequence).  Against the reference's FP64 output
+    (the fixture), the Orszag-Tang case keeps the FP32 split's density bound,
+    absolute L1 <= 1e-4 on cells 1..n; the turbulence case, whose 11 steps
+    end one step before the reference scheme's first non-finite state, only
+    has to stay finite."""
+    binary = dropin_binary(name, "fp32")
+    if not os.path.isfile(binary):
+        pytest.skip(f"{binary} not built (oracle/build_ref.py needs the reference sources)")
+    case = CASES[name]
+    gold = load_golden(name)
+    print(f"case {name} (Fortran drop-in, -DCUMC3D_USE_FP32)", flush=True)
+    st = ic.GENERATORS[case.ic](case.cfg)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "ic.bin")
+        ic.write_ic(p, case.cfg, st)
+        got = run_ref(name, p, case.steps, threads=1, timeout=300, binary=binary)
+    dts, want = python_run(case, st)
+    assert np.array_equal(got["dts"], dts)
+    inner = (slice(3, -3),) * 3
+    for f in FIELDS:
+        assert np.array_equal(got[f][inner], want[f][inner]), f
+    w1 = tuple(slice(1, -1) for _ in range(3))       # cells 1..n of the fixture window 0..n+1
+    rho, ref = got["rho"][inner], gold["rho"][w1]
+    assert np.all(np.isfinite(rho))
+    l1 = float(np.mean(np.abs(rho - ref)))
+    print(f"  density L1 against the reference's FP64: {l1:.2e}", flush=True)
+    if name == "ot_hlld_weno":
+        assert l1 <= 1e-4
+
+
+@pytest.mark.parametrize("name", DROPIN_VARIANT_CASES)
+def test_dropin_selector_switch(name):
+    """-DCUMC3D_USE_FP32 -DCUMC3D_USE_SELECTOR: the drop-in reads the selector's
+    parameters from CUMC3D_SELECTOR_PARAMS and refreshes its masks every
+    CUMC3D_SELECTOR_EVERY steps.  With a fixed random parameter vector (every
+    class occurs) the run is bit for bit the Python mirror's selector run."""
+    from cumc3d_amd.solver import SEL_NPARAM
+    binary = dropin_binary(name, "fp32sel")
+    if not os.path.isfile(binary):
+        pytest.skip(f"{binary} not built (oracle/build_ref.py needs the reference sources)")
+    case = CASES[name]
+    print(f"case {name} (Fortran drop-in, -DCUMC3D_USE_FP32 -DCUMC3D_USE_SELECTOR)", flush=True)
+    st = ic.GENERATORS[case.ic](case.cfg)
+    w = (np.random.default_rng(11).standard_normal(SEL_NPARAM) * 0.4).astype(np.float32)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "ic.bin")
+        ic.write_ic(p, case.cfg, st)
+        wpath = os.path.join(d, "sel.bin")
+        w.tofile(wpath)
+        got = run_ref(name, p, case.steps, threads=1, timeout=300, binary=binary,
+                      extra_env={"CUMC3D_SELECTOR_PARAMS": wpath, "CUMC3D_SELECTOR_EVERY": "2"})
+    dts, want = python_run(case, st, sel_w=w, every=2)
+    assert np.array_equal(got["dts"], dts)
+    inner = (slice(3, -3),) * 3
+    for f in FIELDS:
+        assert np.array_equal(got[f][inner], want[f][inner]), f
