@@ -86,7 +86,7 @@ static int scratch_address(double **out) {
     return 0;
 }
 
-// dt = MIN(1e5, cell minimum), clipped to total_time (main.f90:384-389); a
+// dt = MIN(1e5, cell minimum), clipped to total_time (main.f90:101-106); a
 // step that starts at or past total_time gets dt = 0 and is a no-op in time.
 __global__ void k_dt_finalize(DevScalars *S) {
     S->active = S->t < S->total_time && !S->grav_stop;

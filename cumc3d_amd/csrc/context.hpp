@@ -143,7 +143,7 @@ template <class W> __global__ void k_widen(double *out, const W *in, long G) {
 // INITIAL_UPDATE interior part: CUSTOM_CHECKRHO, bcell of FROMRVETOU and
 // FINDPRESSURE on the interior (initial.f90:91-108).  The FP32 split starts
 // from the double copies k_split_prim left in U and writes the conserved
-// state there (FROMRVETOU, fromrvetou.f90:428-438).
+// state there (FROMRVETOU, fromrvetou.f90:94-108).
 template <class W> __global__ void k_init_interior(ArgsT<W> a) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)a.nx * a.ny * a.nz) return;

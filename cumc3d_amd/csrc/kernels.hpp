@@ -775,7 +775,7 @@ template <class A> __device__ __forceinline__ void checkrho(const A &a, long c, 
 }
 
 // FINDPRESSURE of the model (model/riemann/src/findpressure.f90:17-28,
-// model/torus-cyn/src/findpressure.f90:97-112; the collapsar's,
+// model/torus-cyn/src/findpressure.f90:18-32; the collapsar's,
 // model/collapsar/src/findpressure.f90:27-38, is the ideal gas of riemann's)
 template <class A> __device__ __forceinline__ void pressure(const A &a, long c, double rho, double eps, double &p, double &cs) {
     double dpdrho, dpdeps;
@@ -893,7 +893,7 @@ template <int STAGE, class W, int G>
 __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, int l, double dt) {
     constexpr bool split = sizeof(W) == 4;   // FP32 split: U holds the conserved state
     const long c = cidx(a, j, k, l), sy = a.NX, sz = a.NX * a.NY;
-    // stage-start primitives: FROMRVETOU input (fromrvetou.f90:428-438) in the
+    // stage-start primitives: FROMRVETOU input (fromrvetou.f90:94-108) in the
     // FP64 mode, source terms only in the FP32 split
     double rho = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
     if (!split || G != 0 || a.model != 0 || a.rt_gravity) {
@@ -1075,7 +1075,7 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
     double en = (un[4] - 0.5 * bsq) / rn - 0.5 * vsq;
     checkrho(a, c, rn, v, en);
     if (STAGE == 3) checkrho(a, c, rn, v, en);
-    // FROMRVETOU bcell (fromrvetou.f90:380-422)
+    // FROMRVETOU bcell (fromrvetou.f90:49-89)
     double bcf[3] = {bavg<0, G>(a, j, k, bn_up[0], bprim_lo[0]), bavg<1, G>(a, j, k, bn_up[1], bprim_lo[1]),
                      bavg<2, G>(a, j, k, bn_up[2], bprim_lo[2])};
     double p, cs;

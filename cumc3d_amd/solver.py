@@ -1,6 +1,6 @@
 """Host-side binding of libcumc3d_hip.so (include/cumc3d_hip.h).
 
-``Solver`` mirrors the reference's driver sequence (solver/mhd/main.f90:339-422):
+``Solver`` mirrors the reference's driver sequence (solver/mhd/main.f90:50-139):
 ``initial_model`` (INITIAL_MODEL + POPULATE_DEVICE), ``initial_update``
 (INITIAL_UPDATE), ``finddt`` (FINDDT), ``rungekutta`` (RUNGEKUTTA), ``run`` (the
 time loop), ``find_divb`` (FIND_DIVB) and ``state`` (the host update done by

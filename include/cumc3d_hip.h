@@ -147,7 +147,7 @@ int cumc3d_stage(cumc3d_ctx *ctx, int stage, double dt);
  * reads. */
 int cumc3d_spatial(cumc3d_ctx *ctx, double *l_rk);
 
-/* The time loop of main.f90:381-399 on the device: up to max_steps steps of
+/* The time loop of main.f90:97-139 on the device: up to max_steps steps of
  * FINDDT, dt clip to total_time, RUNGEKUTTA, global_time += dt, stopping once
  * global_time >= total_time.  With sync_each_step = 0 the steps are queued
  * without host round trips (steps past total_time become no-ops).  *t is the

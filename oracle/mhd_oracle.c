@@ -1378,7 +1378,7 @@ static void boundary_eps(oracle *o)
 static void boundary(oracle *o) { boundaryp(o); boundary_eps(o); }   /* boundary.f90:5-13 */
 
 /* UPDATE -> FINDPRESSURE: model/riemann/src/findpressure.f90:17-28,
- * model/torus-cyn/src/findpressure.f90:97-112 */
+ * model/torus-cyn/src/findpressure.f90:18-32 */
 static void update(oracle *o)
 {
     double g = o->d.ggas;

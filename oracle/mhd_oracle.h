@@ -56,7 +56,7 @@ double oracle_finddt(oracle *o);            /* rungekutta.f90:221-332 */
 void oracle_rungekutta(oracle *o, double dt);  /* rungekutta.f90:12-208 */
 void oracle_stage(oracle *o, int stage, double dt);   /* one stage of it, 1..3 in order */
 void oracle_spatial(oracle *o);             /* spatial.f90:11-162: l_rk (oracle_get "lrk1".."lrk8") */
-/* main.f90:381-399 loop: returns steps taken; *t is advanced */
+/* main.f90:97-139 loop: returns steps taken; *t is advanced */
 int oracle_run(oracle *o, int max_steps, double total_time, double *t, double *dts);
 double oracle_find_divb(oracle *o);         /* mhd_module.f90:110-201 */
 /* GETGRID metric arrays, index j+3 for j = -3..n+3: "dx","dy","dz","x","y","z",

@@ -1,5 +1,5 @@
 ! Harness driver for the reference oracle (test infrastructure only).
-! Runs the reference's own step sequence exactly as solver/mhd/main.f90:339-422
+! Runs the reference's own step sequence exactly as solver/mhd/main.f90:50-139
 ! does it (INITIAL_MODEL, INITIAL_UPDATE, then FINDDT / dt clip / RUNGEKUTTA),
 ! without the HDF5 writer, and dumps the final state as raw binary.
 !

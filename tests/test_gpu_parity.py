@@ -175,8 +175,8 @@ def test_gpu_host_dt_path_equals_device_loop():
 
 
 def test_gpu_run_stops_at_total_time():
-    """Steps queued past total_time are no-ops (main.f90:381 loop condition) and the
-    last dt is clipped (main.f90:387-389)."""
+    """Steps queued past total_time are no-ops (main.f90:98 loop condition) and the
+    last dt is clipped (main.f90:104-106)."""
     cfg = CASES["ot_hlld_ppm"].cfg
     st = ic.orszag_tang(cfg)
     a = make_solver(cfg, st)
