@@ -755,6 +755,9 @@ def main(argv=None):
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="copy 1 GiB device to device first (the known-size kernel that scales the HBM counters)")
     ap.add_argument("--sel-every", type=int, default=8)
+    ap.add_argument("--selector-sweep", action="store_true",
+                    help="instead of the bench line: the selector's time and L1 for several label thresholds "
+                         "beside the fixed-scheme FP32 controls")
     ap.add_argument("--thr-ppm", type=float, default=1e-3)
     ap.add_argument("--thr-weno", type=float, default=1e-2)
     args = ap.parse_args(argv)
@@ -765,6 +768,8 @@ def main(argv=None):
 
     if args.workload == "collapsar":
         return bench_collapsar(args)
+    if args.selector_sweep:
+        return selector_sweep(args)
     global SEL_EVERY
     SEL_EVERY = args.sel_every
     if args.pmc_calibrate:
@@ -867,6 +872,52 @@ def main(argv=None):
     if io is not None:
         line["output"] = io
     print(json.dumps(line))
+    return 0
+
+
+def selector_sweep(args):
+    """The selector's speed/accuracy trade-off on the workload: for each pair
+    of label thresholds (thr_ppm, thr_weno) the selector is retrained on the
+    half-size run, timed (median of 3 runs of warmup + steps), and its
+    density L1 against FP64 measured after 8, 13, 20 and `window` steps with
+    FP64's dt sequence, beside the fixed-scheme FP32 controls (TVD-MC, PPM,
+    WENO-Z).  Prints one JSON object."""
+    global SEL_EVERY
+    SEL_EVERY = args.sel_every
+    cfg, st, label = make_workload(args.workload, args.n)
+    window = args.window or finite_window(cfg, st, max(args.window_cap, args.warmup + 1))
+    seg = window - args.warmup
+    pairs = [(1e-3, 1e-2), (1e-3, 3e-3), (3e-4, 1e-3), (1e-4, 3e-4), (1.0, 1e-3), (1.0, 3e-4)]
+    runs = {v: (v, None) for v in ("fp32_tvdmc", "fp32_ppm", "fp32")}
+    info = {}
+    for tp, tw in pairs:
+        w, inf = train_selector(args.train_n or max(16, args.n // 2), args.workload, tp, tw, iters=args.train_iters)
+        key = f"sel_{tp:g}_{tw:g}"
+        runs[key] = ("fp32_sel", w)
+        info[key] = {"thresholds": [tp, tw], "label_counts": inf["label_counts"]}
+    out = {"workload": label, "window": window, "steps": args.steps, "warmup": args.warmup, "variants": {}}
+    for key, (var, w) in runs.items():
+        s = make_variant(cfg, st, var, w)
+        walls = sorted(time_variant(s, st, var, args.warmup, args.steps, seg)[0] for _ in range(3))
+        frac = selector_fractions(s, cfg) if var == "fp32_sel" else None
+        del s
+        out["variants"][key] = dict(info.get(key, {}), ms_per_step=1e3 * walls[1] / args.steps, class_fractions=frac)
+        print(f"[bench] {key}: {out['variants'][key]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    a = make_variant(cfg, st, "fp64")
+    ss = {k: make_variant(cfg, st, v, w) for k, (v, w) in runs.items()}
+    marks = sorted({8, 13, 20, window} & set(range(1, window + 1)))
+    for step in range(1, window + 1):
+        dt = a.finddt()
+        a.rungekutta(dt)
+        for b in ss.values():
+            b.finddt()
+            b.rungekutta(dt)
+        if step in marks:
+            x = a.state()["rho"][3:-3, 3:-3, 3:-3]
+            for k, b in ss.items():
+                y = b.state()["rho"][3:-3, 3:-3, 3:-3]
+                out["variants"][k].setdefault("rho_l1", {})[step] = float(np.mean(np.abs(x - y)))
+    print(json.dumps(out))
     return 0
 
 
