@@ -543,3 +543,109 @@ def test_collapsar_profile_reader():
     st = ic.collapsar(cfg, profile=prof)
     assert st["gravity"][0] == prof["rho"][nx + 2] and st["gravity"][9] == prof["m_bh"]
     assert np.array_equal(st["p"][5, 5, :], prof["p"])
+
+
+def hlld_mk05(qL, qR, g, bn_double_star):
+    """Independent restatement of the HLLD flux of Miyoshi & Kusano (2005,
+    J. Comput. Phys. 208, 315), x direction, from their equations: the star
+    states (eqs. 43-48), the Alfven speeds S*_L/R (51), the double-star states
+    (59-63) and the fluxes F* = F + S (U* - U), F** = F* + S* (U** - U*).
+    The outer wave speeds are the reference's estimate, s_L = min(u_L - c_fL,
+    u_bar - c_bar), s_R = max(u_R + c_fR, u_bar + c_bar), with sqrt(rho)-weighted
+    averages (rsolver_module.f90:601-606), and the fast speeds are the textbook
+    fast magnetosonic speed.  bn_double_star: the normal field the double-star
+    state carries into its energy (B_x in the paper; 0 for the zero mode).
+    Returns None where the reference switches to its degenerate branch
+    (S*_L - s_L < 1e-4 (S_M - s_L) or S*_R - s_R > -1e-4 (s_R - S_M)), which the
+    paper does not describe."""
+    def parts(q):
+        rho, v, p, b = q[0], q[1:4], q[4], q[5:8]
+        e = p / (g - 1.0) + 0.5 * rho * (v @ v) + 0.5 * (b @ b)
+        u = np.array([rho, *(rho * v), e, *b])
+        return rho, v, p, b, e, u, physical_flux(q, g, 0)
+
+    def cfast(rho, p, b):
+        a2, b2, bn2 = g * p / rho, (b @ b) / rho, b[0] ** 2 / rho
+        return math.sqrt(0.5 * (a2 + b2 + math.sqrt(max((a2 + b2) ** 2 - 4 * a2 * bn2, 0.0))))
+
+    rL, vL, pL, bL, eL, uL, fL = parts(qL)
+    rR, vR, pR, bR, eR, uR, fR = parts(qR)
+    bx = bL[0]
+    cL, cR = cfast(rL, pL, bL), cfast(rR, pR, bR)
+    wL, wR = math.sqrt(rL), math.sqrt(rR)
+    ubar, cbar = (wL * vL[0] + wR * vR[0]) / (wL + wR), (wL * cL + wR * cR) / (wL + wR)
+    sL, sR = min(vL[0] - cL, ubar - cbar), max(vR[0] + cR, ubar + cbar)
+    if sL >= 0:
+        return fL
+    if sR <= 0:
+        return fR
+    ptL, ptR = pL + 0.5 * (bL @ bL), pR + 0.5 * (bR @ bR)
+    sM = ((sR - vR[0]) * rR * vR[0] - (sL - vL[0]) * rL * vL[0] - ptR + ptL) / ((sR - vR[0]) * rR - (sL - vL[0]) * rL)
+    pts = ptL + rL * (sL - vL[0]) * (sM - vL[0])
+
+    def star(rho, v, b, e, pt, s):
+        rs = rho * (s - v[0]) / (s - sM)
+        den = rho * (s - v[0]) * (s - sM) - bx * bx
+        vs = v.copy()
+        bs = b.copy()
+        vs[0] = sM
+        vs[1:] = v[1:] - bx * b[1:] * (sM - v[0]) / den
+        bs[1:] = b[1:] * (rho * (s - v[0]) ** 2 - bx * bx) / den
+        es = ((s - v[0]) * e - pt * v[0] + pts * sM + bx * (v @ b - vs @ bs)) / (s - sM)
+        return rs, vs, bs, es
+
+    rsL, vsL, bsL, esL = star(rL, vL, bL, eL, ptL, sL)
+    rsR, vsR, bsR, esR = star(rR, vR, bR, eR, ptR, sR)
+    ssL, ssR = sM - abs(bx) / math.sqrt(rsL), sM + abs(bx) / math.sqrt(rsR)
+    if (ssL - sL) < 1e-4 * (sM - sL) or (ssR - sR) > -1e-4 * (sR - sM):
+        return None
+    usL = np.array([rsL, *(rsL * vsL), esL, *bsL])
+    usR = np.array([rsR, *(rsR * vsR), esR, *bsR])
+    fsL, fsR = fL + sL * (usL - uL), fR + sR * (usR - uR)
+    if ssL >= 0:
+        return fsL
+    if ssR <= 0:
+        return fsR
+    sgn = math.copysign(1.0, bx)
+    qL_, qR_ = math.sqrt(rsL), math.sqrt(rsR)
+    vss, bss = vsL.copy(), bsL.copy()
+    vss[1:] = (qL_ * vsL[1:] + qR_ * vsR[1:] + (bsR[1:] - bsL[1:]) * sgn) / (qL_ + qR_)
+    bss[1:] = (qL_ * bsR[1:] + qR_ * bsL[1:] + qL_ * qR_ * (vsR[1:] - vsL[1:]) * sgn) / (qL_ + qR_)
+    bss[0] = bn_double_star
+    essL = esL - qL_ * (vsL @ bsL - vss @ bss) * sgn
+    essR = esR + qR_ * (vsR @ bsR - vss @ bss) * sgn
+    ussL = np.array([rsL, *(rsL * vss), essL, *bss])
+    ussR = np.array([rsR, *(rsR * vss), essR, *bss])
+    if sM >= 0:
+        return fsL + ssL * (ussL - usL)
+    return fsR + ssR * (ussR - usR)
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_hlld_double_star_matches_miyoshi_kusano(mode):
+    """The oracle's HLLD against the restatement of Miyoshi & Kusano above on
+    random states, every branch (outer, star and double-star) present.  Mode 1
+    (`field`, the textbook B_x in the double-star state) is the paper's HLLD;
+    mode 0 (`zero`) carries 0 instead, which changes only the double-star
+    energy.  The normal-field flux component is not compared (the solver does
+    not store it).  A planted swap of the two modes' normal fields fails."""
+    rng = np.random.default_rng(2024 + mode)
+    g = 5.0 / 3.0
+    keep = [0, 1, 2, 3, 4, 6, 7]
+    counts = {"outer": 0, "star": 0, "double": 0}
+    for _ in range(400):
+        bx = rng.uniform(0.3, 1.5) * rng.choice([-1.0, 1.0])
+        qL = np.array([rng.uniform(0.5, 2.0), *rng.normal(scale=0.5, size=3), rng.uniform(0.2, 2.0), bx,
+                       *rng.normal(size=2)])
+        qR = np.array([rng.uniform(0.5, 2.0), *rng.normal(scale=0.5, size=3), rng.uniform(0.2, 2.0), bx,
+                       *rng.normal(size=2)])
+        want = hlld_mk05(qL, qR, g, bx if mode == 1 else 0.0)
+        if want is None:
+            continue
+        got = O.riemann_point("hlld", 1, g, qL, qR, hlld_bn_mode=mode)
+        np.testing.assert_allclose(got[keep], want[keep], rtol=1e-9, atol=1e-11)
+        wrong = hlld_mk05(qL, qR, g, 0.0 if mode == 1 else bx)
+        dbl = not np.allclose(wrong[keep], want[keep], rtol=1e-9, atol=1e-11)
+        counts["double" if dbl else ("outer" if np.allclose(got[keep], physical_flux(qL, g, 0)[keep])
+                                     or np.allclose(got[keep], physical_flux(qR, g, 0)[keep]) else "star")] += 1
+    assert counts["double"] >= 50 and counts["star"] >= 10, counts
