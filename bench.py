@@ -48,6 +48,7 @@ physics and scheme, on the host share of cores.
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--repeats R] [--n 256]
                        [--workload turbulence|ot|blast|torus_sph|torus_cyl|collapsar]
                        [--no-cpu] [--no-secondary]
+       python bench.py --orbits 5 --n 192     (configuration 5's accuracy over 5 orbits)
 """
 from __future__ import annotations
 
@@ -161,7 +162,7 @@ def make_workload(name: str, n: int):
         return (cfg, ic.torus_sph(cfg),
                 f"spherical torus (r, theta, phi) {cfg.nx}x{cfg.ny}x{cfg.nz} with FIXPOLE, HLL + WENO-Z")
     if name == "torus_cyl":
-        cfg = torus_cyl(n, n // 2, n, recon="ppm")
+        cfg = torus_cyl(n, 2 * n // 3, n, recon="ppm")
         return cfg, ic.torus(cfg), f"cylindrical torus (r, phi, z) {cfg.nx}x{cfg.ny}x{cfg.nz}, HLLD + PPM"
     if name == "collapsar":
         from cumc3d_amd.config import collapsar
@@ -296,7 +297,7 @@ def train_selector(n: int, workload: str, thr_ppm: float, thr_weno: float, seed:
     loss = s.selector_train(iters, blocks=128, lr=3e-3, class_weight=(1.0, 1.0, 2.0), seed=seed)
     w = s.selector_params()
     del s
-    info = {"train_grid": f"{n}^3", "snapshots": taken, "snapshot_every": every, "stride": stride,
+    info = {"train_grid": f"{cfg.nx}x{cfg.ny}x{cfg.nz}", "snapshots": taken, "snapshot_every": every, "stride": stride,
             "candidates_offered": offered, "samples_held": int(counts[0]), "label_counts": counts[1:].tolist(),
             "thresholds": [thr_ppm, thr_weno], "loss_first": float(loss[:10].mean()),
             "loss_last": float(loss[-10:].mean()), "iters": iters}
@@ -758,6 +759,12 @@ def main(argv=None):
     ap.add_argument("--selector-sweep", action="store_true",
                     help="instead of the bench line: the selector's time and L1 for several label thresholds "
                          "beside the fixed-scheme FP32 controls")
+    ap.add_argument("--orbits", type=float, default=0.0,
+                    help="instead of the bench line: the cylindrical torus (configuration 5) advanced this many "
+                         "orbital periods at r = 10 in FP64, the FP32 split and FP32 + selector, with density L1 "
+                         "against FP64 after each orbit")
+    ap.add_argument("--orbit-wall", type=float, default=900.0,
+                    help="with --orbits: stop (and report the orbits reached) after this many seconds")
     ap.add_argument("--thr-ppm", type=float, default=1e-3)
     ap.add_argument("--thr-weno", type=float, default=1e-2)
     args = ap.parse_args(argv)
@@ -770,6 +777,8 @@ def main(argv=None):
         return bench_collapsar(args)
     if args.selector_sweep:
         return selector_sweep(args)
+    if args.orbits > 0:
+        return orbit_accuracy(args)
     global SEL_EVERY
     SEL_EVERY = args.sel_every
     if args.pmc_calibrate:
@@ -917,6 +926,85 @@ def selector_sweep(args):
             for k, b in ss.items():
                 y = b.state()["rho"][3:-3, 3:-3, 3:-3]
                 out["variants"][k].setdefault("rho_l1", {})[step] = float(np.mean(np.abs(x - y)))
+    print(json.dumps(out))
+    return 0
+
+
+def orbit_accuracy(args):
+    """Configuration 5's accuracy check: the cylindrical torus (n x 2n/3 x n,
+    HLLD + PPM, the pressure maximum at r = 10 with GM = 1, so one orbit there
+    is T = 2 pi 10^1.5) advanced `args.orbits` orbital periods in FP64 (the
+    reference scheme), the FP32 split and FP32 + selector (trained on a
+    half-size run of the torus), every variant driven with FP64's dt
+    sequence, for at most `args.orbit_wall` seconds.  It prints progress
+    every 20 s; after each orbit (and where it stops) it records density L1
+    against FP64: absolute over
+    the whole grid, relative to the grid's mean density, and relative inside
+    the torus (cells where FP64 has rho > 1e-3).  Prints one JSON object."""
+    import math
+    from cumc3d_amd.solver import CumcError
+    global SEL_EVERY
+    SEL_EVERY = args.sel_every
+    cfg, st, label = make_workload("torus_cyl", args.n)
+    t_orbit = 2.0 * math.pi * 10.0 ** 1.5
+    t_end = args.orbits * t_orbit
+    train_n = args.train_n or max(24, args.n // 2)
+    print(f"[bench] {label}: training the selector on the {train_n}-cell run", file=sys.stderr, flush=True)
+    sel_w, info = train_selector(train_n, "torus_cyl", args.thr_ppm, args.thr_weno, iters=args.train_iters)
+    a = make_variant(cfg, st, "fp64")
+    others = {v: make_variant(cfg, st, v, sel_w) for v in ("fp32", "fp32_sel")}
+    marks = [min(t_end, (k + 1) * t_orbit) for k in range(int(math.ceil(args.orbits)))]
+    out = {"workload": label, "orbits": args.orbits, "t_orbit": t_orbit, "t_end": t_end,
+           "selector": {"label_counts": info["label_counts"], "refresh_every": SEL_EVERY}, "checks": []}
+    failed = {}
+    steps, t0, last_report = 0, time.time(), time.time()
+    w = (slice(3, -3),) * 3
+
+    def check():
+        x = a.state()["rho"][w]
+        torus = x > 1e-3
+        row = {"t": a.t, "orbits": a.t / t_orbit, "steps": steps}
+        for v, b in others.items():
+            if v in failed:
+                row[v] = {"error": failed[v]}
+                continue
+            d = np.abs(b.state()["rho"][w] - x)
+            row[v] = {"rho_l1": float(d.mean()), "rho_l1_rel": float(d.mean() / x.mean()),
+                      "rho_l1_rel_torus": float(d[torus].mean() / x[torus].mean()),
+                      "rho_linf_rel": float(d.max() / x.max())}
+            if v == "fp32_sel":
+                row[v]["class_fractions"] = selector_fractions(b, cfg)
+        out["checks"].append(row)
+        print(f"[bench] t = {a.t:.2f} ({a.t / t_orbit:.3f} orbits, {steps} steps): "
+              + ", ".join(f"{v} L1 {row[v].get('rho_l1', float('nan')):.3e}" for v in others),
+              file=sys.stderr, flush=True)
+
+    try:
+        for t_mark in marks:
+            while a.t < t_mark and time.time() - t0 < args.orbit_wall:
+                dt = min(a.finddt(), t_mark - a.t)
+                a.rungekutta(dt)
+                steps += 1
+                for v, b in others.items():
+                    if v in failed:
+                        continue
+                    try:
+                        b.finddt()
+                        b.rungekutta(dt)
+                    except CumcError as e:
+                        failed[v] = f"non-finite at step {steps}: {e}"
+                if time.time() - last_report > 20:
+                    print(f"[bench] t = {a.t:.2f} ({a.t / t_orbit:.3f} orbits): {steps} steps, dt {dt:.3e}, "
+                          f"{time.time() - t0:.0f} s", file=sys.stderr, flush=True)
+                    last_report = time.time()
+            check()
+            if a.t < t_mark:
+                out["stopped"] = f"wall-clock limit of {args.orbit_wall:.0f} s reached at t = {a.t:.3f}"
+                break
+    except CumcError as e:
+        out["fp64_error"] = f"FP64 non-finite at step {steps} (t = {a.t:.3f}): {e}"
+    out["steps"] = steps
+    out["seconds"] = time.time() - t0
     print(json.dumps(out))
     return 0
 
