@@ -646,18 +646,24 @@ template <class W> __global__ __launch_bounds__(256) void k_edge(ArgsT<W> a) {
     const long c = cidx(a, j, k, l), sy = a.NX, sz = a.NX * a.NY;
     const W *ex = a.Ec[0], *ey = a.Ec[1], *ez = a.Ec[2];
     const W h = W(0.50), q = W(0.25);
+    // the three EMFs are formed before any is stored, so that all their loads
+    // are in flight together
+    W e0 = W(0), e1 = W(0), e2 = W(0);
     if (j >= 1) {
         const W *fxy = a.F[1][5], *fxz = a.F[2][5];
-        a.E[0][c] = h * (fxy[c] + fxy[c + sz] + fxz[c] + fxz[c + sy]) - q * (ex[c] + ex[c + sz] + ex[c + sy] + ex[c + sy + sz]);
+        e0 = h * (fxy[c] + fxy[c + sz] + fxz[c] + fxz[c + sy]) - q * (ex[c] + ex[c + sz] + ex[c + sy] + ex[c + sy + sz]);
     }
     if (k >= 1) {
         const W *fyx = a.F[0][5], *fyz = a.F[2][6];
-        a.E[1][c] = h * (fyx[c] + fyx[c + sz] + fyz[c] + fyz[c + 1]) - q * (ey[c] + ey[c + sz] + ey[c + 1] + ey[c + 1 + sz]);
+        e1 = h * (fyx[c] + fyx[c + sz] + fyz[c] + fyz[c + 1]) - q * (ey[c] + ey[c + sz] + ey[c + 1] + ey[c + 1 + sz]);
     }
     if (l >= 1) {
         const W *fzx = a.F[0][6], *fzy = a.F[1][6];
-        a.E[2][c] = h * (fzx[c] + fzx[c + sy] + fzy[c] + fzy[c + 1]) - q * (ez[c] + ez[c + sy] + ez[c + 1] + ez[c + 1 + sy]);
+        e2 = h * (fzx[c] + fzx[c + sy] + fzy[c] + fzy[c + 1]) - q * (ez[c] + ez[c + sy] + ez[c + 1] + ez[c + 1 + sy]);
     }
+    if (j >= 1) a.E[0][c] = e0;
+    if (k >= 1) a.E[1][c] = e1;
+    if (l >= 1) a.E[2][c] = e2;
 }
 
 // FIXPOLE (mhd_module.f90:437-461): the radial edge EMF on each polar face
@@ -1041,8 +1047,16 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
             if (l == 1) o[8 * (c - sz) + 7] = lB_z<G>(a, j, k, ey_mz, ey_mxmz, ex_mz, ex_mymz);
             return 1.0e5;
         }
+        // all six face loads first: the stage-1 stores to Uo below would
+        // otherwise hold each next load back (the compiler cannot tell Uo
+        // from Bin), one memory round trip per face
+        double bup6[3], blo6[3];
         for (int d = 0; d < 3; ++d) {
-            double bup = a.Bin[d][c];
+            bup6[d] = a.Bin[d][c];
+            blo6[d] = a.Bin[d][cl[d]];
+        }
+        for (int d = 0; d < 3; ++d) {
+            double bup = bup6[d];
             double uoup;
             if (STAGE == 1) {
                 uoup = bup;
@@ -1051,7 +1065,7 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
                 uoup = a.Uo[5 + d][c];
             }
             bn_up[d] = rk<STAGE>(uoup, bup, dt, Lup[d]);
-            double blo = a.Bin[d][cl[d]], uolo;
+            double blo = blo6[d], uolo;
             if (STAGE == 1) {
                 uolo = blo;
                 if (fidx[d] == 1) a.Uo[5 + d][cl[d]] = blo;
@@ -1080,6 +1094,9 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
                      bavg<2, G>(a, j, k, bn_up[2], bprim_lo[2])};
     double p, cs;
     pressure(a, c, rn, en, p, cs);
+    // the CFL step (stage 3) before the stores, so that its grid-spacing
+    // loads do not wait behind them
+    const double dtc = STAGE == 3 ? cfl_dt<G>(a, j, k, l, cs, rn, v, bcf) : 1.0e5;
     st_w(&a.P[0][c], (W)rn);
     st_w(&a.P[1][c], (W)v[0]);
     st_w(&a.P[2][c], (W)v[1]);
@@ -1113,7 +1130,7 @@ __device__ __forceinline__ double update_cell(const ArgsT<W> &a, int j, int k, i
             else st_nt(&dst[i][c], un2[i]);
         }
     }
-    return STAGE == 3 ? cfl_dt<G>(a, j, k, l, cs, rn, v, bcf) : 1.0e5;
+    return dtc;
 }
 
 // One thread per interior cell, x fastest.  (Measured on MI355X at 256^3:
@@ -1201,42 +1218,62 @@ template <class W> __global__ __launch_bounds__(256) void k_ghost(ArgsT<W> a) {
         return;
     }
     const long c = t;
+    // Every load comes first, then every store.  A load placed after a store
+    // would wait for the loads feeding that store (the compiler cannot tell
+    // the arrays apart), one memory round trip each.  The sources are
+    // interior cells or ghosts of another axis that this launch does not
+    // write, so the order changes no value.
     // cell-centred primitives and bcell from the fully mapped interior cell
     const long cs_ = cidx(a, jj, kk, ll);
-    double rho = a.P[0][cs_] * (gsign(a, 0, sx, 1) * gsign(a, 1, sy, 1) * gsign(a, 2, sz, 1));
-    double v[3];
-    for (int d = 0; d < 3; ++d)
-        v[d] = a.P[1 + d][cs_] * (gsign(a, 0, sx, 2 + d) * gsign(a, 1, sy, 2 + d) * gsign(a, 2, sz, 2 + d));
+    const W P0 = a.P[0][cs_], P1 = a.P[1][cs_], P2 = a.P[2][cs_], P3 = a.P[3][cs_];
+    const W C0 = a.Bc[0][cs_], C1 = a.Bc[1][cs_], C2 = a.Bc[2][cs_];
+    // epsilon: BOUNDARY1D_NM takes the y-outer branch from boundary_flag(3) (boundary.f90:243)
+    int ey = kk;
+    bool eps_y = gy;
+    if (sy == 1) {
+        int f3 = a.bc[2], f4 = a.bc[3];
+        int m = k - a.ny;
+        if (f3 == 0) ey = m;
+        else if (f4 == 1) ey = a.ny;
+        else if (f4 >= 2) ey = a.ny + 1 - m;
+        else { ey = k; eps_y = false; }
+        if (eps_y) ey = ey < 1 ? 1 : (ey > a.ny ? a.ny : ey);   // as gmap's clamp
+    }
+    const bool eps_w = gx || eps_y || gz;
+    const bool eps_same_cell = eps_w && ey == kk;
+    const W epsv = eps_w ? a.eps[cidx(a, jj, ey, ll)] : a.eps[c];
+    // transverse face fields: Bx is filled in y/z ghosts, By in x/z, Bz in x/y
+    double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+    if (gy || gz) F0 = a.Bin[0][cidx(a, j, kk, ll)];
+    if (gx || gz) F1 = a.Bin[1][cidx(a, jj, k, ll)];
+    if (gx || gy) F2 = a.Bin[2][cidx(a, jj, kk, l)];
+    // FP32 split: the source cell's p and c_s were computed from its
+    // double-precision rho and eps; the same FINDPRESSURE of the
+    // single-precision copies could differ in the last bit
+    const bool copy_pc = sizeof(W) == 4 && a.model == 0 && eps_same_cell;
+    W Pp = W(0), Pcs = W(0);
+    if (copy_pc) {
+        Pp = a.P[4][cs_];
+        Pcs = a.cs[cs_];
+    }
+    double rho = P0 * (gsign(a, 0, sx, 1) * gsign(a, 1, sy, 1) * gsign(a, 2, sz, 1));
+    double v[3] = {P1 * (gsign(a, 0, sx, 2) * gsign(a, 1, sy, 2) * gsign(a, 2, sz, 2)),
+                   P2 * (gsign(a, 0, sx, 3) * gsign(a, 1, sy, 3) * gsign(a, 2, sz, 3)),
+                   P3 * (gsign(a, 0, sx, 4) * gsign(a, 1, sy, 4) * gsign(a, 2, sz, 4))};
     // cell EMFs of the ghost: the mapped interior value times the parity of
     // each component (BOUNDARY_EFIELD, mhd_module.f90:478-840), computed from
     // the source cell before the no-inflow clamps below touch v
-    cell_emf(a, c, a.P[1][cs_], a.P[2][cs_], a.P[3][cs_], a.Bc[0][cs_], a.Bc[1][cs_], a.Bc[2][cs_],
+    cell_emf(a, c, P1, P2, P3, C0, C1, C2,
              (W)(gsign(a, 0, sx, 9) * gsign(a, 1, sy, 9) * gsign(a, 2, sz, 9)),
              (W)(gsign(a, 0, sx, 10) * gsign(a, 1, sy, 10) * gsign(a, 2, sz, 10)),
              (W)(gsign(a, 0, sx, 11) * gsign(a, 1, sy, 11) * gsign(a, 2, sz, 11)));
-    for (int d = 0; d < 3; ++d)
-        a.Bc[d][c] = (W)(a.Bc[d][cs_] * (gsign(a, 0, sx, 6 + d) * gsign(a, 1, sy, 6 + d) * gsign(a, 2, sz, 6 + d)));
-    // epsilon: BOUNDARY1D_NM takes the y-outer branch from boundary_flag(3) (boundary.f90:243)
-    bool eps_same_cell = true;
-    {
-        int ey = kk;
-        bool eps_y = gy;
-        if (sy == 1) {
-            int f3 = a.bc[2], f4 = a.bc[3];
-            int m = k - a.ny;
-            if (f3 == 0) ey = m;
-            else if (f4 == 1) ey = a.ny;
-            else if (f4 >= 2) ey = a.ny + 1 - m;
-            else { ey = k; eps_y = false; }
-            if (eps_y) ey = ey < 1 ? 1 : (ey > a.ny ? a.ny : ey);   // as gmap's clamp
-        }
-        eps_same_cell = (gx || eps_y || gz) && ey == kk;
-        if (gx || eps_y || gz) a.eps[c] = a.eps[cidx(a, jj, ey, ll)];
-    }
-    // transverse face fields: Bx is filled in y/z ghosts, By in x/z, Bz in x/y
-    if (gy || gz) a.Bin[0][c] = a.Bin[0][cidx(a, j, kk, ll)] * (gsign(a, 1, sy, 6) * gsign(a, 2, sz, 6));
-    if (gx || gz) a.Bin[1][c] = a.Bin[1][cidx(a, jj, k, ll)] * (gsign(a, 0, sx, 7) * gsign(a, 2, sz, 7));
-    if (gx || gy) a.Bin[2][c] = a.Bin[2][cidx(a, jj, kk, l)] * (gsign(a, 0, sx, 8) * gsign(a, 1, sy, 8));
+    a.Bc[0][c] = (W)(C0 * (gsign(a, 0, sx, 6) * gsign(a, 1, sy, 6) * gsign(a, 2, sz, 6)));
+    a.Bc[1][c] = (W)(C1 * (gsign(a, 0, sx, 7) * gsign(a, 1, sy, 7) * gsign(a, 2, sz, 7)));
+    a.Bc[2][c] = (W)(C2 * (gsign(a, 0, sx, 8) * gsign(a, 1, sy, 8) * gsign(a, 2, sz, 8)));
+    if (eps_w) a.eps[c] = epsv;
+    if (gy || gz) a.Bin[0][c] = F0 * (gsign(a, 1, sy, 6) * gsign(a, 2, sz, 6));
+    if (gx || gz) a.Bin[1][c] = F1 * (gsign(a, 0, sx, 7) * gsign(a, 2, sz, 7));
+    if (gx || gy) a.Bin[2][c] = F2 * (gsign(a, 0, sx, 8) * gsign(a, 1, sy, 8));
     // torus no-inflow clamps: radial in both torus models
     // (model/torus-cyn/src/custom.f90:156-161, model/torus-sph/src/custom.f90:160-167),
     // vertical in the cylindrical one (model/torus-cyn/src/custom.f90:194-199)
@@ -1252,16 +1289,13 @@ template <class W> __global__ __launch_bounds__(256) void k_ghost(ArgsT<W> a) {
     a.P[1][c] = (W)v[0];
     a.P[2][c] = (W)v[1];
     a.P[3][c] = (W)v[2];
-    if (sizeof(W) == 4 && a.model == 0 && eps_same_cell) {
-        // FP32 split: the source cell's p and c_s were computed from its
-        // double-precision rho and eps; the same FINDPRESSURE of the
-        // single-precision copies could differ in the last bit
-        a.P[4][c] = a.P[4][cs_];
-        a.cs[c] = a.cs[cs_];
+    if (copy_pc) {
+        a.P[4][c] = Pp;
+        a.cs[c] = Pcs;
         return;
     }
     double p, cs;
-    pressure(a, c, rho, (double)a.eps[c], p, cs);
+    pressure(a, c, rho, (double)epsv, p, cs);
     a.P[4][c] = (W)p;
     a.cs[c] = (W)cs;
 }
