@@ -1278,11 +1278,11 @@ template <class W> __global__ __launch_bounds__(256) void k_ghost(ArgsT<W> a) {
     // (model/torus-cyn/src/custom.f90:156-161, model/torus-sph/src/custom.f90:160-167),
     // vertical in the cylindrical one (model/torus-cyn/src/custom.f90:194-199)
     if (a.model != 0) {
-        if (sx == 0) v[0] = fmin2(v[0], 0.0);
+        if (sx == 0) v[0] = fmin_zero(v[0]);
         if (sx == 1) v[0] = fmax2(v[0], 0.0);
     }
     if (a.model == 1) {
-        if (sz == 0) v[2] = fmin2(v[2], 0.0);
+        if (sz == 0) v[2] = fmin_zero(v[2]);
         if (sz == 1) v[2] = fmax2(v[2], 0.0);
     }
     a.P[0][c] = (W)rho;

@@ -19,6 +19,14 @@ template <class T> __device__ __forceinline__ T fsign1(T b) { return __builtin_c
 // NaN or a tie yields the second argument.
 template <class T> __device__ __forceinline__ T fmin2(T a, T b) { return (a < b) ? a : b; }
 template <class T> __device__ __forceinline__ T fmax2(T a, T b) { return (a > b) ? a : b; }
+// MIN(a, 0.0) as the reference evaluates it: (a < 0) ? a : +0.  The backend
+// turns that select into v_min (and drops a following + 0), which returns -0
+// for a = -0; a condition the compiler cannot see through keeps the select.
+template <class T> __device__ __forceinline__ T fmin_zero(T a) {
+    int lt = a < T(0);
+    __asm__("" : "+v"(lt));
+    return lt ? a : T(0);
+}
 template <class T> __device__ __forceinline__ T fabsT(T a) { return __builtin_fabs(a); }
 __device__ __forceinline__ float fabsT(float a) { return __builtin_fabsf(a); }
 __device__ __forceinline__ float fsign1(float b) { return __builtin_copysignf(1.0f, b); }

@@ -17,8 +17,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from cumc3d_amd.config import (Config, briowu_1d, collapsar, orszag_tang_3d, torus_sph,  # noqa: E402
-                               turbulence_3d)
+from cumc3d_amd.config import (Config, briowu_1d, collapsar, orszag_tang_3d, torus_cyl,  # noqa: E402
+                               torus_sph, turbulence_3d)
 
 
 @dataclasses.dataclass
@@ -126,6 +126,9 @@ LARGE_CASES = {c.name: c for c in [
     Case("c2_ot128_ppm", orszag_tang_3d(128, recon="ppm", riemann="hlld"), "orszag_tang", 20),
     # configuration 4's physics (256^3 decaying turbulence) at 128^3, HLLD + WENO-Z
     Case("c4_turb128_weno", turbulence_3d(128, recon="weno", riemann="hlld"), "turbulence", 12),
+    # configuration 5's cylindrical torus at half its resolution (96 x 64 x 96),
+    # HLLD + PPM: geometric sources, floors, point-mass gravity, no-inflow clamps
+    Case("c5_torus96_ppm", torus_cyl(96, 64, 96, recon="ppm", riemann="hlld"), "torus", 40),
 ]}
 
 # CPU-baseline samples for bench.py: the reference solver on the benchmark

@@ -88,3 +88,11 @@ def check_profile(prof, cfg, state, t):
     assert bf.shape == (nz + 3, ny + 3, nx + 3, 3)
     for d, name in enumerate(("bx", "by", "bz")):
         assert np.array_equal(bf[..., d], state[name][f], equal_nan=True), name
+
+
+def zero_signs_equal(a, b) -> bool:
+    """The cells holding zero carry the same sign in a and b (np.array_equal
+    counts -0.0 == +0.0; the reference's MIN/MAX and floors leave signed zeros
+    that a bit-exact result must reproduce)."""
+    a, b = np.asarray(a), np.asarray(b)
+    return np.array_equal(np.signbit(a) & (a == 0), np.signbit(b) & (b == 0))

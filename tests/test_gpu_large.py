@@ -1,9 +1,11 @@
 """FP64 parity at production sizes.  Needs an MI355X.
 
-* 128^3 against the reference: the reference Fortran's state after the last
-  step of each LARGE_CASES case of oracle/cases.py (configuration 2 exactly:
-  128^3 Orszag-Tang, HLLD + PPM; configuration 4's turbulence at 128^3, HLLD +
-  WENO-Z), stored as SHA-256 digests of every field on cells 0..n+1 plus
+* Production sizes against the reference: the reference Fortran's state
+  after the last step of each LARGE_CASES case of oracle/cases.py
+  (configuration 2 exactly: 128^3 Orszag-Tang, HLLD + PPM; configuration 4's
+  turbulence at 128^3, HLLD + WENO-Z; configuration 5's cylindrical torus at
+  96 x 64 x 96, HLLD + PPM, with its floors, point-mass gravity and
+  no-inflow clamps), stored as SHA-256 digests of every field on cells 0..n+1 plus
   per-plane sums (tests/golden/large_*.json, written by
   tests/golden/make_large.py).  The dt sequence, every field, max div B and
   the l_rk of one more SPATIAL must match bit for bit.  At this size the
@@ -49,7 +51,7 @@ def first_plane_off(a, planes):
 
 
 @pytest.mark.parametrize("name", list(LARGE_CASES))
-def test_gpu_matches_reference_at_128(name):
+def test_gpu_matches_reference_at_production_size(name):
     path = os.path.join(GOLDEN, f"large_{name}.json")
     if not os.path.exists(path):
         pytest.skip(f"fixture large_{name}.json missing")

@@ -4,7 +4,7 @@ bit for bit.  Needs an MI355X."""
 import numpy as np
 import pytest
 
-from conftest import check_lrk, check_profile, interior_window, load_golden, lrk_mask
+from conftest import check_lrk, check_profile, interior_window, load_golden, lrk_mask, zero_signs_equal
 from cumc3d_amd import hydroprofile
 from cumc3d_amd import ic
 from cumc3d_amd.config import Config, collapsar, orszag_tang_3d, torus_cyl, torus_sph
@@ -24,6 +24,8 @@ def assert_same(got, want, w=None, fields=FIELDS, label=""):
         if not np.array_equal(a, b, equal_nan=True):
             d = np.abs(a - b)
             raise AssertionError(f"{label}{f}: {np.count_nonzero(d)} cells differ, max {np.nanmax(d):.3e}")
+        if not zero_signs_equal(a, b):
+            raise AssertionError(f"{label}{f}: the sign of a zero differs")
 
 
 @pytest.mark.parametrize("name", list(CASES))

@@ -7,7 +7,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from conftest import interior_window, load_golden
+from conftest import interior_window, load_golden, zero_signs_equal
 from cumc3d_amd import ic
 from cumc3d_amd.config import Config
 from oracle import oracle as O
@@ -35,6 +35,7 @@ def test_oracle_matches_reference_fixture(name):
     for f in FIELDS:
         got = o.get(f)[w]
         assert np.array_equal(got, gold[f]), f"{f}: max diff {np.nanmax(np.abs(got - gold[f]))}"
+        assert zero_signs_equal(got, gold[f]), f"{f}: the sign of a zero differs"
     assert o.divb() == float(gold["maxdivb"])
     if "phi" in gold:   # collapsar: black hole and potential after the last UPDATE
         check_gravity(o.gravity(), gold, CASES[name].cfg)
