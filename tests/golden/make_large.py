@@ -10,6 +10,8 @@ stores in tests/golden/large_<case>.json:
   * per field (rho, vx, vy, vz, p, bx, by, bz, eps, bcx, bcy, bcz), on cells
     0..n+1 of every axis as C-order float64: the SHA-256 of the bytes, and the
     sum over each z plane (hex floats, to locate a difference);
+  * for the collapsar, the potential phi on cells 0..n+1 (digests) and the
+    black hole's m_bh and r_sh (hex floats);
   * the same digests of l_rk from one more SPATIAL on the final state, over
     the elements the library forms (conftest.lrk_mask: l_rk(1:5) on cells
     1..n, l_rk(6:8) on the faces the update advances), flattened.
@@ -69,6 +71,8 @@ def main(names):
                "fields": {f: digest(ref[f][w]) for f in FIELDS},
                "lrk": {k: digest(v[w][lrk_mask(case.cfg, int(k[3:]))]) for k, v in ref["lrk"].items()},
                "finite": bool(all(np.all(np.isfinite(ref[f][w])) for f in FIELDS)),
+               **({"phi": digest(ref["grav"]["phi"][w]), "m_bh": float(ref["grav"]["m_bh"]).hex(),
+                   "r_sh": float(ref["grav"]["r_sh"]).hex()} if "grav" in ref else {}),
                "reference_seconds": float(ref["seconds"])}
         path = os.path.join(HERE, f"large_{name}.json")
         with open(path, "w") as f:

@@ -5,7 +5,9 @@
   (configuration 2 exactly: 128^3 Orszag-Tang, HLLD + PPM; configuration 4's
   turbulence at 128^3, HLLD + WENO-Z; configuration 5's cylindrical torus at
   96 x 64 x 96, HLLD + PPM, with its floors, point-mass gravity and
-  no-inflow clamps), stored as SHA-256 digests of every field on cells 0..n+1 plus
+  no-inflow clamps; the spherical torus with FIXPOLE at 128 x 96 x 64; the
+  collapsar with self-gravity at 156 x 44 x 128, its potential and black hole
+  included), stored as SHA-256 digests of every field on cells 0..n+1 plus
   per-plane sums (tests/golden/large_*.json, written by
   tests/golden/make_large.py).  The dt sequence, every field, max div B and
   the l_rk of one more SPATIAL must match bit for bit.  At this size the
@@ -69,6 +71,10 @@ def test_gpu_matches_reference_at_production_size(name):
             bad.append((f, first_plane_off(st[f][w], gold["fields"][f]["plane_sums"])))
     assert not bad, f"fields differing (field, first differing z plane of cells 0..n+1): {bad}"
     assert float(s.find_divb()).hex() == gold["maxdivb"]
+    if "phi" in gold:   # collapsar: the potential and the black hole after the last step
+        g = s.gravity()
+        assert float(g["m_bh"]).hex() == gold["m_bh"] and float(g["r_sh"]).hex() == gold["r_sh"]
+        assert sha(g["phi"][w]) == gold["phi"]["sha256"], "phi differs"
     lrk = s.spatial()
     bad = [k for k, v in gold["lrk"].items() if sha(lrk[k][w][lrk_mask(case.cfg, int(k[3:]))]) != v["sha256"]]
     assert not bad, f"l_rk differs: {bad}"
