@@ -14,7 +14,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from conftest import check_lrk, check_profile, interior_window, load_golden
+from conftest import check_lrk, check_profile, interior_window, load_golden, zero_signs_equal
 from cumc3d_amd import ic
 from oracle import oracle as O
 from oracle.build_ref import DROPIN_CASES
@@ -46,6 +46,7 @@ def test_dropin_matches_reference(name):
             continue
         a, b = got[f][w], gold[f]
         assert np.array_equal(a, b, equal_nan=True), f"{name}: {f} differs, max {np.nanmax(np.abs(a - b)):.3e}"
+        assert zero_signs_equal(a, b), f"{name}: {f}: the sign of a zero differs"
     assert got["maxdivb"] == float(gold["maxdivb"])
     if "phi" in gold:   # collapsar: CUSTOM_DEF's phi, m_bh, r_sh synced back from the device
         cfg = case.cfg
