@@ -126,6 +126,8 @@ LARGE_CASES = {c.name: c for c in [
     Case("c2_ot128_ppm", orszag_tang_3d(128, recon="ppm", riemann="hlld"), "orszag_tang", 20),
     # configuration 4's physics (256^3 decaying turbulence) at 128^3, HLLD + WENO-Z
     Case("c4_turb128_weno", turbulence_3d(128, recon="weno", riemann="hlld"), "turbulence", 12),
+    # configuration 4's benchmark grid itself (256^3 turbulence, HLLD + WENO-Z)
+    Case("c4_turb256_weno", turbulence_3d(256, recon="weno", riemann="hlld"), "turbulence", 5),
     # configuration 5's cylindrical torus at half its resolution (96 x 64 x 96),
     # HLLD + PPM: geometric sources, floors, point-mass gravity, no-inflow clamps
     Case("c5_torus96_ppm", torus_cyl(96, 64, 96, recon="ppm", riemann="hlld"), "torus", 40),

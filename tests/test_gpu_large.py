@@ -3,7 +3,8 @@
 * Production sizes against the reference: the reference Fortran's state
   after the last step of each LARGE_CASES case of oracle/cases.py
   (configuration 2 exactly: 128^3 Orszag-Tang, HLLD + PPM; configuration 4's
-  turbulence at 128^3, HLLD + WENO-Z; configuration 5's cylindrical torus at
+  turbulence at 128^3 and at its own 256^3, HLLD + WENO-Z; configuration 5's
+  cylindrical torus at
   96 x 64 x 96, HLLD + PPM, with its floors, point-mass gravity and
   no-inflow clamps; the spherical torus with FIXPOLE at 128 x 96 x 64; the
   collapsar with self-gravity at 156 x 44 x 128, its potential and black hole
