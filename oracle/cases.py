@@ -17,8 +17,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from cumc3d_amd.config import (Config, briowu_1d, collapsar, orszag_tang_3d, torus_cyl,  # noqa: E402
-                               torus_sph, turbulence_3d)
+from cumc3d_amd.config import (Config, blast_3d, briowu_1d, collapsar, orszag_tang_3d,  # noqa: E402
+                               torus_cyl, torus_sph, turbulence_3d)
 
 
 @dataclasses.dataclass
@@ -128,6 +128,11 @@ LARGE_CASES = {c.name: c for c in [
     Case("c4_turb128_weno", turbulence_3d(128, recon="weno", riemann="hlld"), "turbulence", 12),
     # configuration 4's benchmark grid itself (256^3 turbulence, HLLD + WENO-Z)
     Case("c4_turb256_weno", turbulence_3d(256, recon="weno", riemann="hlld"), "turbulence", 5),
+    # the other Riemann solvers and limiters at 128^3: HLLC + WENO-Z and LF +
+    # van Leer on the vortex, HLL + MC on the magnetised blast (strong shocks)
+    Case("ot128_hllc_weno", orszag_tang_3d(128, recon="weno", riemann="hllc"), "orszag_tang", 10),
+    Case("ot128_lf_tvdvl", orszag_tang_3d(128, recon="tvdvl", riemann="lf"), "orszag_tang", 10),
+    Case("blast128_hll_tvdmc", blast_3d(128, recon="tvdmc", riemann="hll"), "blast", 6),
     # configuration 5's cylindrical torus at half its resolution (96 x 64 x 96),
     # HLLD + PPM: geometric sources, floors, point-mass gravity, no-inflow clamps
     Case("c5_torus96_ppm", torus_cyl(96, 64, 96, recon="ppm", riemann="hlld"), "torus", 40),
