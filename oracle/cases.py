@@ -138,6 +138,8 @@ LARGE_CASES = {c.name: c for c in [
     Case("c5_torus96_ppm", torus_cyl(96, 64, 96, recon="ppm", riemann="hlld"), "torus", 40),
     # the spherical torus with FIXPOLE at 128 x 96 x 64, HLLD + PPM
     Case("sph_torus128_ppm", torus_sph(128, 96, 64, recon="ppm"), "torus_sph", 30),
+    # and with the scheme of bench.py's spherical-torus workload, HLL + WENO-Z
+    Case("sph_torus128_hll_weno", torus_sph(128, 96, 64, recon="weno", riemann="hll"), "torus_sph", 30),
     # the collapsar at the reference's shipped progenitor grid size (156 x 44 x
     # 128): self-gravity solved at the initial update and at step 10
     Case("collapsar156_ppm", collapsar(156, 44, 128, recon="ppm", riemann="hlld"), "collapsar", 12),
