@@ -133,6 +133,9 @@ LARGE_CASES = {c.name: c for c in [
     Case("ot128_hllc_weno", orszag_tang_3d(128, recon="weno", riemann="hllc"), "orszag_tang", 10),
     Case("ot128_lf_tvdvl", orszag_tang_3d(128, recon="tvdvl", riemann="lf"), "orszag_tang", 10),
     Case("blast128_hll_tvdmc", blast_3d(128, recon="tvdmc", riemann="hll"), "blast", 6),
+    # a box of unequal sides that are not multiples of the 64-lane wave
+    # (150 x 130 x 110), so rows, march segments and waves do not line up
+    Case("ot150x130x110_ppm", orszag_tang_3d(16, recon="ppm").with_(nx=150, ny=130, nz=110), "orszag_tang", 10),
     # configuration 5's cylindrical torus at half its resolution (96 x 64 x 96),
     # HLLD + PPM: geometric sources, floors, point-mass gravity, no-inflow clamps
     Case("c5_torus96_ppm", torus_cyl(96, 64, 96, recon="ppm", riemann="hlld"), "torus", 40),

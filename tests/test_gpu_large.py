@@ -2,7 +2,8 @@
 
 * Production sizes against the reference: the reference Fortran's state
   after the last step of each LARGE_CASES case of oracle/cases.py
-  (configuration 2 exactly: 128^3 Orszag-Tang, HLLD + PPM; the vortex with
+  (configuration 2 exactly: 128^3 Orszag-Tang, HLLD + PPM; the vortex on a
+  150 x 130 x 110 box; the vortex with
   HLLC + WENO-Z and LF + van Leer and the magnetised blast with HLL + MC at
   128^3; configuration 4's
   turbulence at 128^3 and at its own 256^3, HLLD + WENO-Z; configuration 5's
