@@ -93,6 +93,20 @@ def test_fp32_torus_runs_close():
     assert l1 <= L1_TOL
 
 
+def test_fp32_configuration5_grid():
+    """Configuration 5's own grid, the cylindrical torus at 192 x 128 x 192
+    (HLLD + PPM): 200 steps of the FP32 split against FP64, with FP64's dt
+    sequence.  The density L1 bound is 1e-4; measured 6.7e-8, so 1e-6 guards
+    against a regression."""
+    cfg = torus_cyl(192, 128, 192, recon="ppm")
+    print("case torus 192x128x192 fp32 vs fp64, 200 steps", flush=True)
+    a, b, _ = run_pair(cfg, "torus", 200)
+    l1a = abs_l1(a, b, "rho")
+    l1, linf = rel_errors(a, b, "rho")
+    print(f"  rho: L1 {l1a:.2e}  rel L1 {l1:.2e}  rel Linf {linf:.2e}", flush=True)
+    assert l1a <= L1_ABS_TOL and l1a <= 1e-6
+
+
 def test_fp32_uniform_state_is_stationary():
     """A uniform moving state has zero flux divergence in any precision: every
     cell does the same arithmetic, so the state stays exactly uniform, and it
