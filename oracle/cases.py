@@ -136,6 +136,12 @@ LARGE_CASES = {c.name: c for c in [
     # a box of unequal sides that are not multiples of the 64-lane wave
     # (150 x 130 x 110), so rows, march segments and waves do not line up
     Case("ot150x130x110_ppm", orszag_tang_3d(16, recon="ppm").with_(nx=150, ny=130, nz=110), "orszag_tang", 10),
+    # 2D at 512^2 (the y march over whole columns, no z sweep) and configuration
+    # 1's Brio-Wu tube on 8192 cells (one row of 8192 interfaces, 128 waves)
+    Case("ot2d_512_ppm", Config(nx=512, ny=512, nz=1, n_dim=2, cfl=0.4, ggas=5.0 / 3.0,
+                                riemann="hlld", recon="ppm", boundary_flag=(0, 0, 0, 0, 0, 0)),
+         "orszag_tang", 40),
+    Case("briowu_8192", briowu_1d(8192), "briowu", 400),
     # configuration 5's cylindrical torus at half its resolution (96 x 64 x 96),
     # HLLD + PPM: geometric sources, floors, point-mass gravity, no-inflow clamps
     Case("c5_torus96_ppm", torus_cyl(96, 64, 96, recon="ppm", riemann="hlld"), "torus", 40),

@@ -32,7 +32,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
-from conftest import lrk_mask  # noqa: E402
+from conftest import interior_window, lrk_mask  # noqa: E402
 from cumc3d_amd import ic  # noqa: E402
 from oracle.cases import LARGE_CASES  # noqa: E402
 from oracle.refio import have_ref, run_ref  # noqa: E402
@@ -41,8 +41,8 @@ FIELDS = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz", "eps", "bcx", "bcy", "
 
 
 def window(cfg):
-    """cells 0..n+1 of every axis of a 3D box"""
-    return tuple(slice(2, n + 4) for n in (cfg.nz, cfg.ny, cfg.nx))
+    """cells 0..n+1 of every active axis, cell 1 of an inactive one (1D/2D)"""
+    return interior_window(cfg)
 
 
 def digest(a: np.ndarray) -> dict:

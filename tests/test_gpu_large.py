@@ -3,7 +3,7 @@
 * Production sizes against the reference: the reference Fortran's state
   after the last step of each LARGE_CASES case of oracle/cases.py
   (configuration 2 exactly: 128^3 Orszag-Tang, HLLD + PPM; the vortex on a
-  150 x 130 x 110 box; the vortex with
+  150 x 130 x 110 box and in 2D at 512^2; Brio-Wu on 8192 cells; the vortex with
   HLLC + WENO-Z and LF + van Leer and the magnetised blast with HLL + MC at
   128^3; configuration 4's
   turbulence at 128^3 and at its own 256^3, HLLD + WENO-Z; configuration 5's
@@ -29,7 +29,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, lrk_mask
+from conftest import GOLDEN, interior_window, lrk_mask
 from cumc3d_amd import ic
 from cumc3d_amd.config import turbulence_3d
 from cumc3d_amd.solver import make_solver
@@ -41,7 +41,8 @@ FIELDS = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz", "eps", "bcx", "bcy", "
 
 
 def window(cfg):
-    return tuple(slice(2, n + 4) for n in (cfg.nz, cfg.ny, cfg.nx))
+    """cells 0..n+1 of every active axis, cell 1 of an inactive one (1D/2D)"""
+    return interior_window(cfg)
 
 
 def sha(a):
