@@ -156,7 +156,8 @@ def make_workload(name: str, n: int):
         return cfg, ic.blast(cfg), f"3D magnetised blast {n}^3 periodic, HLLD + WENO-Z"
     if name == "turbulence":
         cfg = turbulence_3d(n)
-        return cfg, ic.turbulence(cfg), f"3D periodic decaying turbulence {n}^3, HLLD + WENO-Z"
+        log = (lambda m: print(f"[bench] initial state: {m}", file=sys.stderr, flush=True)) if n > 256 else None
+        return cfg, ic.turbulence(cfg, log=log), f"3D periodic decaying turbulence {n}^3, HLLD + WENO-Z"
     if name == "torus_sph":
         cfg = torus_sph(n, 3 * n // 4, n // 2, recon="weno", riemann="hll")
         return (cfg, ic.torus_sph(cfg),

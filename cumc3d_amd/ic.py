@@ -197,10 +197,11 @@ def blast(cfg: Config, seed: int = 20261202, edge_cells: float = 1.0) -> dict:
     return st
 
 
-def _solenoidal_modes(n: int, kmax: float, slope: float, seed: int, rng_stream: int):
+def _solenoidal_modes(n: int, kmax: float, slope: float, seed: int, rng_stream: int, log=None):
     """Random-phase solenoidal vector field on an n^3 periodic grid with
     |k| <= kmax and amplitude |a_k| ~ k**slope; returns real arrays (3, n, n, n)
-    indexed [comp, l, k, j]."""
+    indexed [comp, l, k, j].  log(msg), if given, hears after each component
+    (a 512^3 field takes about a minute)."""
     kk = np.fft.fftfreq(n, d=1.0 / n)
     KZ, KY, KX = np.meshgrid(kk, kk, kk, indexing="ij")
     K2 = KX * KX + KY * KY + KZ * KZ
@@ -215,6 +216,8 @@ def _solenoidal_modes(n: int, kmax: float, slope: float, seed: int, rng_stream: 
         f = np.zeros((n, n, n), dtype=np.complex128)
         f[mask] = g[mask] * np.exp(2j * math.pi * ph[mask]) * kmag[mask] ** slope
         field.append(f)
+        if log:
+            log(f"solenoidal field {rng_stream}: component {c + 1} of 3 drawn")
     fx, fy, fz = field
     # project out the compressive part: f - k (k.f)/k^2
     with np.errstate(invalid="ignore", divide="ignore"):
@@ -222,7 +225,10 @@ def _solenoidal_modes(n: int, kmax: float, slope: float, seed: int, rng_stream: 
     fx = fx - KX * kdotf
     fy = fy - KY * kdotf
     fz = fz - KZ * kdotf
-    return np.stack([np.fft.ifftn(f).real for f in (fx, fy, fz)])
+    out = np.stack([np.fft.ifftn(f).real for f in (fx, fy, fz)])
+    if log:
+        log(f"solenoidal field {rng_stream}: transformed")
+    return out
 
 
 def _periodic_ghost(a: np.ndarray) -> np.ndarray:
@@ -230,17 +236,18 @@ def _periodic_ghost(a: np.ndarray) -> np.ndarray:
     return np.pad(a, 3, mode="wrap")
 
 
-def turbulence(cfg: Config, seed: int = 20261202) -> dict:
+def turbulence(cfg: Config, seed: int = 20261202, log=None) -> dict:
     """Decaying turbulence: rho=1, beta=1, rms Mach 1 solenoidal velocity with
     |k|<=8 and E(k) ~ k^-2, mean field B0 along z plus a solenoidal dB at 50%
-    of B0 built from an edge vector potential (discrete div B = 0)."""
+    of B0 built from an edge vector potential (discrete div B = 0).  log(msg),
+    if given, hears the progress of the two random fields."""
     n = cfg.nx
     assert cfg.ny == n and cfg.nz == n, "turbulence needs a cubic grid"
     g = Grid(cfg)
     gam = cfg.ggas
     p0 = 1.0 / gam                          # c_s = 1
     b0 = math.sqrt(2.0 * p0)                # beta = 2 p / B0^2 = 1
-    v = _solenoidal_modes(n, 8.0, -2.0, seed, 0)     # amplitude ~ k^-2 gives E(k) ~ k^-2
+    v = _solenoidal_modes(n, 8.0, -2.0, seed, 0, log)     # amplitude ~ k^-2 gives E(k) ~ k^-2
     vrms = math.sqrt(float(np.mean(np.sum(v * v, axis=0))))
     v *= 1.0 / vrms                          # rms Mach 1 with c_s = 1
     st = empty_state(cfg)
@@ -251,7 +258,7 @@ def turbulence(cfg: Config, seed: int = 20261202) -> dict:
     st["vz"][:] = _periodic_ghost(v[2])
     # vector potential on edges: A_x at (x_j, y_{k+1/2}, z_{l+1/2}), etc. (same
     # periodic array indices), then face B = curl A.
-    a = _solenoidal_modes(n, 8.0, -3.0, seed, 1)
+    a = _solenoidal_modes(n, 8.0, -3.0, seed, 1, log)
     ax, ay, az = (np.asarray(c) for c in a)
     h = 1.0 / n
     # Bx(face j, k, l) = (Az(j,k) - Az(j,k-1))/dy - (Ay(j,l) - Ay(j,l-1))/dz
