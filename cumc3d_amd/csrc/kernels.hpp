@@ -364,7 +364,7 @@ __device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v
 // raised the maximum, and the atomics on the one address serialised: FP32
 // TVD-MC x sweep 0.79 ms against 0.39 ms for the z sweep on turbulence.)
 template <class T, int R, int S>
-__global__ __launch_bounds__(256) void k_flux_x(ArgsT<T> a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && R == RSEL ? 5 : 1))) void k_flux_x(ArgsT<T> a) {
     if (!a.S->active) return;
     const int lane = threadIdx.x & 63;
     const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
