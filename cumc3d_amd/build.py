@@ -13,19 +13,21 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libcumc3d_hip.so")
-# (source, extra flags): the FP32 split's unit may contract, use the short
+# (source, extra flags): the FP32 split's units may contract, use the short
 # single-precision divide and square root, and flush single-precision
 # denormals (no value of the solver comes near them: the smallest constant
-# is WENO-Z's 1e-30), which drops the range-scaling around v_sqrt_f32.
-# -fapprox-func with -freciprocal-math turns each divide into v_rcp_f32 and a
-# multiply, without the range check (compare, select and two multiplies) the
-# short divide keeps for denominators beyond 2^96, and shares the reciprocal
-# of a denominator between divides.  The FP64 unit may do none of these.
+# is WENO-Z's 1e-30), which drops the range-scaling around v_sqrt_f32.  In
+# the sweeps' unit (fp32.hip), -fapprox-func with -freciprocal-math also turns
+# each divide into v_rcp_f32 and a multiply, without the range check
+# (compare, select and two multiplies) the short divide keeps for
+# denominators beyond 2^96, and shares the reciprocal of a denominator
+# between divides.  The state unit (fp32_state.hip: update, ghost fill,
+# FINDDT) keeps IEEE division for its double arithmetic.  The FP64 unit may
+# do none of these.
+FP32_FLAGS = ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"]
 SOURCES = [(os.path.join(HERE, "csrc", "capi.hip"), []),
-           (os.path.join(HERE, "csrc", "fp32.hip"), ["-ffp-contract=fast",
-                                                     "-fno-hip-fp32-correctly-rounded-divide-sqrt",
-                                                     "-fgpu-flush-denormals-to-zero",
-                                                     "-fapprox-func", "-freciprocal-math"])]
+           (os.path.join(HERE, "csrc", "fp32.hip"), FP32_FLAGS + ["-fapprox-func", "-freciprocal-math"]),
+           (os.path.join(HERE, "csrc", "fp32_state.hip"), FP32_FLAGS)]
 DEPS = [os.path.join(HERE, "csrc", f) for f in ("kernels.hpp", "physics.hpp", "selector.hpp", "context.hpp")] + \
        [os.path.join(ROOT, "include", "cumc3d_hip.h")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,5 +1,6 @@
 // Host side of libcumc3d_hip.so: the C-ABI declared in include/cumc3d_hip.h,
-// step sequencing and the FP64 mode's kernels (the FP32 split's are in fp32.hip).
+// step sequencing and the FP64 mode's kernels (the FP32 split's are in
+// fp32.hip and fp32_state.hip).
 #include <hip/hip_runtime.h>
 
 #include <cmath>

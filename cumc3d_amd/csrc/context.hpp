@@ -1,7 +1,9 @@
-// Device context and the launch sequence of one RK stage, shared by the two
-// translation units of libcumc3d_hip.so: capi.hip (C-ABI, the FP64 mode) and
-// fp32.hip (the FP32 split's kernels, compiled with contraction and the
-// short single-precision divide and square root).
+// Device context and the launch sequence of one RK stage, shared by the
+// translation units of libcumc3d_hip.so: capi.hip (C-ABI, the FP64 mode),
+// fp32.hip (the FP32 split's sweeps and selector, compiled with contraction,
+// the short single-precision divide and square root and reciprocal divides)
+// and fp32_state.hip (the FP32 split's update, ghost fill and FINDDT, with
+// IEEE division in double).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -360,6 +362,22 @@ static inline void tmark(cumc3d_ctx *x, int slot) {
 // SPATIAL up to l_operator: the three sweeps (fluxes, face and cell EMFs)
 // and the edge EMFs of the current state; `marks` records the stage's
 // timing events
+// The FP32 split's state kernels (update, ghost fill) live in their own
+// translation unit, whose double arithmetic keeps IEEE division.
+namespace f32 {
+void update(cumc3d_ctx *x, const ArgsT<float> &a, int stage, dim3 grid);
+void ghost(cumc3d_ctx *x, const ArgsT<float> &g);
+}  // namespace f32
+
+template <class W> static void stage_update(cumc3d_ctx *x, const ArgsT<W> &a, int stage, dim3 grid) {
+    if constexpr (sizeof(W) == 4) f32::update(x, a, stage, grid);
+    else launch_update<W>(x->d.coordinate_flag, stage, grid, x->stream, a);
+}
+template <class W> static void stage_ghost(cumc3d_ctx *x, const ArgsT<W> &g) {
+    if constexpr (sizeof(W) == 4) f32::ghost(x, g);
+    else hipLaunchKernelGGL(k_ghost<W>, dim3(nblk(ghost_count(*x))), dim3(256), 0, x->stream, g);
+}
+
 template <class W> static void sweeps_t(cumc3d_ctx *x, const ArgsT<W> &a, hipStream_t st, bool marks) {
     auto mark = [&](int slot) { if (marks) tmark(x, slot); };
     mark(0);
@@ -382,7 +400,7 @@ template <class W> static void spatial_t(cumc3d_ctx *x, double *lrk) {
     ArgsT<W> a = make_args<W>(x);
     a.lrk = lrk;
     sweeps_t<W>(x, a, x->stream, false);
-    launch_update<W>(x->d.coordinate_flag, 0, dim3(nblk((long)x->nx * x->ny * x->nz)), x->stream, a);
+    stage_update<W>(x, a, 0, dim3(nblk((long)x->nx * x->ny * x->nz)));
 }
 
 template <class W> static void run_stage_t(cumc3d_ctx *x, int stage) {
@@ -390,11 +408,11 @@ template <class W> static void run_stage_t(cumc3d_ctx *x, int stage) {
     ArgsT<W> a = make_args<W>(x);
     const long nn = (long)x->nx * x->ny * x->nz;
     sweeps_t<W>(x, a, st, true);
-    launch_update<W>(x->d.coordinate_flag, stage, dim3(nblk(nn)), st, a);
+    stage_update<W>(x, a, stage, dim3(nblk(nn)));
     tmark(x, 5);
     x->cur = 1 - x->cur;
     ArgsT<W> g = make_args<W>(x);
-    hipLaunchKernelGGL(k_ghost<W>, dim3(nblk(ghost_count(*x))), dim3(256), 0, st, g);
+    stage_ghost<W>(x, g);
     tmark(x, 6);
     if (x->timing && x->tstage < TSTAGES) x->tstage += 1;
 }
@@ -450,7 +468,9 @@ template <class W> static void sel_collect_t(cumc3d_ctx *x, int stride, unsigned
 }
 
 
-// Entry points of the FP32 translation unit (fp32.hip).
+// Entry points of the FP32 translation units: fp32.hip (stage sequence,
+// sweeps, selector) and fp32_state.hip (FINDDT, the layout kernels and the
+// initial update).
 namespace f32 {
 void run_stage(cumc3d_ctx *x, int stage);
 void spatial(cumc3d_ctx *x, double *lrk);
