@@ -363,8 +363,12 @@ __device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v
 // one of the ~68k blocks of a 256^3 sweep with many double-star interfaces
 // raised the maximum, and the atomics on the one address serialised: FP32
 // TVD-MC x sweep 0.79 ms against 0.39 ms for the z sweep on turbulence.)
+// The single-precision selector sweep is held at 5 waves per SIMD (86
+// VGPRs, 0.41 ms at 256^3).  Left to itself the compiler gives it 112 VGPRs
+// and 4 waves (0.47 ms); at 6 waves it spills (0.44 ms).
 template <class T, int R, int S>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && R == RSEL ? 5 : 1))) void k_flux_x(ArgsT<T> a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && R == RSEL ? 5 : 1)))
+void k_flux_x(ArgsT<T> a) {
     if (!a.S->active) return;
     const int lane = threadIdx.x & 63;
     const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
