@@ -70,8 +70,10 @@ def divb_inner(s, cfg):
     (orszag_tang_3d(32, recon="weno"), "orszag_tang", 10),
     (orszag_tang_3d(24, recon="ppm", riemann="hllc"), "orszag_tang", 10),
     (orszag_tang_3d(24, recon="tvdmc"), "orszag_tang", 10),
+    (orszag_tang_3d(24, recon="tvdmm"), "orszag_tang", 10),
+    (orszag_tang_3d(24, recon="tvdvl", riemann="hll"), "orszag_tang", 10),
     (blast_3d(32), "blast", 4),
-], ids=["ot32_weno_hlld", "ot24_ppm_hllc", "ot24_tvdmc_hlld", "blast32_weno_hlld"])
+], ids=["ot32_weno_hlld", "ot24_ppm_hllc", "ot24_tvdmc_hlld", "ot24_tvdmm_hlld", "ot24_tvdvl_hll", "blast32_weno_hlld"])
 def test_fp32_close_to_fp64(cfg, ic_name, steps):
     print(f"case {cfg.nx}^3 {cfg.recon}/{cfg.riemann} fp32 vs fp64", flush=True)
     a, b, _ = run_pair(cfg, ic_name, steps)
